@@ -1,0 +1,29 @@
+#!/usr/bin/env bash
+# Builds the reference's own two-electron-path programs from their Fortran
+# sources, where they lie under /root/reference/src, with AMD flang.  Outputs
+# (objects, module files, executables) go only to oracle/_ref/.  Nothing of the
+# reference is copied into the repository.
+#
+# Built: int1e, int2e, RHFI2G, dens, ao2mo, mp2 (they need nothing beyond the
+# Fortran runtime).  Not built: scf -- it calls LAPACK DSYGV/DSYEV and the image
+# has no LAPACK library, so it is unbuildable here; the SCF driver loop is
+# restated in oracle/scf_driver.py on top of the reference programs built here.
+set -euo pipefail
+SRC=${MYQC_REF_SRC:-/root/reference/src}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+if [ ! -d "$SRC" ]; then echo "reference sources not found at $SRC; skipping"; exit 0; fi
+mkdir -p "$OUT/mod"
+FC=${FC:-flang}
+FFLAGS="-O2 -w -module-dir $OUT/mod -I$OUT/mod"
+$FC $FFLAGS -c "$SRC/myQC/env.f90" -o "$OUT/env.o"
+$FC $FFLAGS -c "$SRC/myQC/basis.f90" -o "$OUT/basis.o"
+$FC $FFLAGS -c "$SRC/integrals/auxilary.f90" -o "$OUT/auxilary.o"
+$FC $FFLAGS -o "$OUT/int1e" "$SRC/integrals/int1e.f90" "$OUT/auxilary.o" "$OUT/env.o" "$OUT/basis.o"
+$FC $FFLAGS -o "$OUT/int2e" "$SRC/integrals/int2e.f90" "$OUT/auxilary.o" "$OUT/env.o" "$OUT/basis.o"
+$FC $FFLAGS -o "$OUT/RHFI2G" "$SRC/I2G/RHFI2G.f90" "$OUT/env.o"
+$FC $FFLAGS -o "$OUT/dens" "$SRC/dens/dens.f90" "$OUT/env.o" "$OUT/basis.o"
+$FC $FFLAGS -o "$OUT/ao2mo" "$SRC/ao2mo/ao2mo.f90" "$OUT/env.o"
+$FC $FFLAGS -o "$OUT/mp2" "$SRC/mp2/mp2.f90" "$OUT/env.o"
+$FC --version | head -1 > "$OUT/FLANG_VERSION"
+echo "reference programs built in $OUT"

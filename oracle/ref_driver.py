@@ -1,0 +1,217 @@
+"""Runs the reference's own programs (built by oracle/build_ref.sh into
+oracle/_ref/) on a molecule and collects their outputs.
+
+TEST INFRASTRUCTURE ONLY: this is part of the oracle.  Only tests/, the
+fixture generator under tests/golden/ and bench.py's CPU-baseline leg may use
+it, and only as the checker.
+
+The reference ``scf`` program needs LAPACK (DSYGV), which this image lacks, so
+it is not built.  Its RHF driver loop is restated here instead
+(src/scf/scf.f90:61-216 RHF, :410-512 initRHF, :720-904 RHFiter,
+:1122-1152 getEnr, :1157-1267 checkConv), spawning the reference's ``dens`` and
+``RHFI2G`` exactly where scf.f90:158-159,877-878 does, with DSYGV(itype=1,'V','L')
+replaced by scipy.linalg.eigh(type=1, lower=True, driver='gv') -- the same
+LAPACK routine from scipy's bundled OpenBLAS.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(HERE, "_ref")
+DATA = os.path.join(os.path.dirname(HERE), "myqc_amd", "data")
+
+
+def have_ref() -> bool:
+    return all(os.path.exists(os.path.join(REF, p))
+               for p in ("int1e", "int2e", "RHFI2G", "dens", "ao2mo", "mp2"))
+
+
+def read_unformatted(path, dtype="<f8"):
+    """Read a Fortran sequential unformatted file (4-byte record markers),
+    returning a list of records as numpy arrays."""
+    out = []
+    with open(path, "rb") as f:
+        buf = f.read()
+    pos = 0
+    while pos < len(buf):
+        (n,) = struct.unpack("<i", buf[pos:pos + 4])
+        chunks = []
+        # negative markers denote continued subrecords
+        while True:
+            ln = abs(n)
+            chunks.append(buf[pos + 4:pos + 4 + ln])
+            pos += 8 + ln
+            if n >= 0:
+                break
+            (n,) = struct.unpack("<i", buf[pos:pos + 4])
+        out.append(np.frombuffer(b"".join(chunks), dtype=dtype))
+    return out
+
+
+def write_unformatted(path, arrays):
+    with open(path, "wb") as f:
+        for a in arrays:
+            b = np.ascontiguousarray(a, dtype="<f8").tobytes(order="C")
+            f.write(struct.pack("<i", len(b)) + b + struct.pack("<i", len(b)))
+
+
+def _fmt(x: float) -> str:
+    return repr(float(x))
+
+
+def write_env(d, Z, xyz_bohr, charge=0, calc_mp2=True, scf_conv=11, verb=1):
+    nelc = sum(Z) - charge
+    assert nelc % 2 == 0, "RHF needs a closed shell"
+    opts = [1, 1 if calc_mp2 else 0, 0, 0, 0, 1, 1000000, verb, scf_conv, charge, 1, 1, 1, 0, 0, 0, 0]
+    with open(os.path.join(d, "envdat"), "w") as f:
+        f.write(f" {len(Z)}\n {nelc // 2} {nelc // 2}\n 17\n " + " ".join(str(o) for o in opts) + "\n")
+    with open(os.path.join(d, "nucpos"), "w") as f:
+        for z, r in zip(Z, xyz_bohr):
+            f.write(f" {z} {_fmt(r[0])} {_fmt(r[1])} {_fmt(r[2])}\n")
+    with open(os.path.join(d, "fmem"), "w") as f:
+        f.write(" 1000000.0\n")
+    shutil.copy(os.path.join(DATA, "Ftab"), os.path.join(d, "Ftab"))
+    shutil.copy(os.path.join(DATA, "mybasis"), os.path.join(d, "mybasis"))
+
+
+def _run(d, prog):
+    r = subprocess.run([os.path.join(REF, prog)], cwd=d, capture_output=True, text=True)
+    if r.returncode != 0 or os.path.exists(os.path.join(d, "error")):
+        raise RuntimeError(f"reference {prog} failed:\n{r.stdout}\n{r.stderr}")
+    return r.stdout
+
+
+def read_text_matrix(path, n):
+    vals = np.array(open(path).read().split(), dtype=np.float64)
+    return vals[: n * n].reshape(n, n, order="F")
+
+
+def write_text_matrix(path, M):
+    with open(path, "w") as f:
+        f.write(" ".join(_fmt(v) for v in np.asarray(M).ravel(order="F")) + "\n")
+
+
+def enuc(Z, xyz):
+    """src/scf/scf.f90:1122-1152."""
+    e = 0.0
+    for a in range(len(Z)):
+        for b in range(a):
+            r = np.sqrt((xyz[a, 0] - xyz[b, 0]) ** 2 + (xyz[a, 1] - xyz[b, 1]) ** 2 + (xyz[a, 2] - xyz[b, 2]) ** 2)
+            e += Z[a] * Z[b] / r
+    return e
+
+
+def ao2mo_ovov(XX, C, nocc):
+    """(ia|jb) = sum_{uvld} C_ui C_va C_lj C_db (uv|ld), by the reference's
+    four quarter steps in order (ao2mo.f90:506,524,543,562)."""
+    Co = C[:, :nocc]
+    Cv = C[:, nocc:]
+    L = np.einsum("tqrs,tp->pqrs", XX, Co, optimize=True)
+    M = np.einsum("ptrs,tq->pqrs", L, Cv, optimize=True)
+    N = np.einsum("pqts,tr->pqrs", M, Co, optimize=True)
+    return np.einsum("pqrt,ts->pqrs", N, Cv, optimize=True)
+
+
+def _dsygv(A, B):
+    import scipy.linalg
+    w, v = scipy.linalg.eigh(np.tril(A), np.tril(B), lower=True, type=1, driver="gv")
+    return w, v
+
+
+def run_reference(Z, xyz_bohr, charge=0, scf_conv=11, mp2=True, workdir=None, keep=False):
+    """Run int1e, int2e, the restated RHF loop (spawning dens/RHFI2G), then
+    ao2mo and mp2.  Returns a dict of outputs."""
+    if not have_ref():
+        raise RuntimeError("reference programs not built (oracle/build_ref.sh)")
+    d = workdir or tempfile.mkdtemp(prefix="myqc_ref_", dir=os.path.join(REF))
+    os.makedirs(d, exist_ok=True)
+    try:
+        write_env(d, Z, xyz_bohr, charge=charge, calc_mp2=mp2, scf_conv=scf_conv)
+        _run(d, "int1e")
+        _run(d, "int2e")
+        n = int(open(os.path.join(d, "basinfo")).read().split()[1])
+        S = read_text_matrix(os.path.join(d, "Suv"), n)
+        H = read_text_matrix(os.path.join(d, "Huv"), n)
+        XX = read_unformatted(os.path.join(d, "XX"))[0].reshape(n, n, n, n, order="F")
+        En = enuc(Z, xyz_bohr)
+        nelc = sum(Z) - charge
+        # initRHF (scf.f90:410-512): core-Hamiltonian guess
+        w, C = _dsygv(H, S)
+        C_guess = C.copy()
+        write_text_matrix(os.path.join(d, "Cui"), C)
+        _run(d, "dens")
+        _run(d, "RHFI2G")
+        G_first = read_unformatted(os.path.join(d, "Guv"))[0].reshape(n, n, order="F").copy()
+        D_first = read_unformatted(os.path.join(d, "Da"))[0].reshape(n, n, order="F").copy()
+        lim = [10.0 ** (-k) for k in range(12)]
+        conv = False
+        it = 0
+        energies = []
+        Etrk = [0.0, 0.0, 0.0]
+        eig = None
+        while not conv:
+            D = read_unformatted(os.path.join(d, "Da"))[0].reshape(n, n, order="F")
+            G = read_unformatted(os.path.join(d, "Guv"))[0].reshape(n, n, order="F")
+            F = H + G
+            Eelc = 0.0
+            for v in range(n):
+                for u in range(n):
+                    Eelc += D[u, v] * (F[u, v] + H[u, v])
+            Eelc *= 0.5
+            energies.append(Eelc + En)
+            eig, C = _dsygv(F, S)
+            write_text_matrix(os.path.join(d, "Cui"), C)
+            write_unformatted(os.path.join(d, "Dold"), [D.ravel(order="F")])
+            _run(d, "dens")
+            _run(d, "RHFI2G")
+            Etrk = [Etrk[1], Etrk[2], Eelc + En]
+            if it > 0:
+                Dn = read_unformatted(os.path.join(d, "Da"))[0].reshape(n, n, order="F")
+                mdiff = np.max(np.abs(D - Dn))
+                conv = mdiff < lim[scf_conv]
+            it += 1
+            if it >= 500:
+                break
+        with open(os.path.join(d, "eig"), "w") as f:
+            f.write(" ".join(_fmt(e) for e in eig) + "\n")
+        out = dict(n=n, S=S, H=H, XX=XX, Enuc=En, E_scf=energies[-1], energies=np.array(energies),
+                   C=C, eig=eig, nocc=nelc // 2, C_guess=C_guess, G_guess=G_first, D_guess=D_first,
+                   iterations=it, converged=conv)
+        if mp2:
+            # The reference ao2mo cannot run under flang's runtime: it opens
+            # unit 100 on 'basinfo', closes unit 1 instead (ao2mo.f90:53-56),
+            # then re-opens unit 100 on 'XX' (ao2mo.f90:494-496) and flang reads
+            # the still-connected file.  Its RHF transform
+            # (slow_ao2mo_MP2_RHF, ao2mo.f90:465-602; idx1..idx4_trans
+            # :1306-1439) is restated here and written in the same record
+            # layout (ijab_AB: one nv x nv record per (i,j), i-major, a
+            # fastest; :575-582); the reference mp2 then reads it.
+            nocc = nelc // 2
+            iajb = ao2mo_ovov(XX, C, nocc)
+            nv = n - nocc
+            write_unformatted(os.path.join(d, "ijab_AB"),
+                              [iajb[i, :, j, :].ravel(order="F") for i in range(nocc) for j in range(nocc)])
+            txt = _run(d, "mp2")
+            vals = {}
+            for line in txt.splitlines():
+                for key in ("E(AA)", "E(AB)", "E(MBPT2)"):
+                    if key + " =" in line:
+                        vals[key] = float(line.split("=")[1])
+            out["E_aa"] = vals["E(AA)"]
+            out["E_ab"] = vals["E(AB)"]
+            out["E_mp2"] = vals["E(MBPT2)"]
+            nocc = nelc // 2
+            nv = n - nocc
+            recs = read_unformatted(os.path.join(d, "ijab_AB"))
+            out["ijab_AB"] = np.array([r.reshape(nv, nv, order="F") for r in recs]).reshape(nocc, nocc, nv, nv)  # (ia|jb)
+        return out
+    finally:
+        if not keep and workdir is None:
+            shutil.rmtree(d, ignore_errors=True)
