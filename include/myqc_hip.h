@@ -1,0 +1,114 @@
+/*
+ * myqc_hip.h -- C ABI of libmyqc_hip, the MI355X (gfx950) two-electron path
+ * of myQC.
+ *
+ * Conventions
+ *  - Arrays are Fortran column-major and are passed unchanged from
+ *    iso_c_binding callers (real(c_double) :: A(*)).
+ *  - Basis arrays are the reference's flat layouts exactly as buildBasis
+ *    produces them (src/myQC/basis.f90:23-226): bas(set*OpS+k), basinfo =
+ *    [OpS, norb, {pri,ang,ori,nprim,center}...], setinfo = [nset, setl,
+ *    {norb_in_set, maxL, center, orb0..orb3}...].
+ *  - The caller owns host buffers; the context owns device buffers.
+ *  - Every entry point returns 0 on success or a negative MYQC_E_* code; no
+ *    exception or abort crosses the ABI.  myqc_last_error() gives the text.
+ *    A Fortran caller then writes the `error` sentinel and STOPs, as the
+ *    reference does (src/myQC/env.f90:36-41).
+ *  - Calls are synchronous at return and must come from one host thread per
+ *    context.
+ */
+#ifndef MYQC_HIP_H
+#define MYQC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct myqc_ctx myqc_ctx;
+
+/* error codes */
+#define MYQC_OK 0
+#define MYQC_E_ARG (-1)      /* invalid argument / size                     */
+#define MYQC_E_STATE (-2)    /* call order (e.g. no system set)             */
+#define MYQC_E_HIP (-3)      /* HIP runtime or library failure              */
+#define MYQC_E_NOMEM (-4)    /* device allocation failed                    */
+#define MYQC_E_NOCONV (-5)   /* SCF did not converge                        */
+#define MYQC_E_DEVICE (-6)   /* no usable gfx950 device                     */
+
+/* set_system flags */
+#define MYQC_REF_COMPAT 0    /* reference numerics: FP32-rounded pi, Ftab Boys at Q=3*sum(set L) */
+
+/* precision selectors */
+#define MYQC_FP64 0
+#define MYQC_FP32 1          /* J/K digestion of FP32-stored integrals        */
+#define MYQC_BF16X3 2        /* ao2mo: split bf16 hi/lo MFMA products, FP32 accumulation */
+
+/* Context: one device, one HIP stream.  No counterpart in the reference
+ * (its programs own their arrays, src/myQC/env.f90). */
+int myqc_init(int device, myqc_ctx **out);
+int myqc_finalize(myqc_ctx *ctx);
+const char *myqc_last_error(const myqc_ctx *ctx);
+
+/* Molecule + basis + Boys table.  Replaces the inputs int2e reads
+ * (getenv: src/myQC/env.f90:16-73; buildBasis: src/myQC/basis.f90:23;
+ * Ftab: src/integrals/int2e.f90:161-163).  xyz_bohr is (nnuc,3) col-major as
+ * getenv holds it; Ftab is Ft(0:120,0:22) col-major (Tk fastest). */
+int myqc_set_system(myqc_ctx *ctx, int nnuc, const int *Z, const double *xyz_bohr, int nset,
+                    const double *set_alpha, const int *setinfo, const double *bas, int norb,
+                    const int *basinfo, const double *Ftab, int flags);
+
+/* int2e drop-in: the dense, 8-fold filled (ij|gh) = XX(i,j,g,h), n^4
+ * col-major (i fastest).  Replaces proc2e + fillsym
+ * (src/integrals/int2e.f90:78-351,540-613).  schwarz_tol = 0 computes every
+ * shell quartet (the reference has no Schwarz screen). */
+int myqc_eri_full(myqc_ctx *ctx, double schwarz_tol, double *XX);
+
+/* Device-resident, Schwarz-screened unique integrals for J/K digestion
+ * (the work int2e hands to RHFI2G through the XX file).  Returns the number
+ * of stored integral slots. */
+int myqc_eri_build(myqc_ctx *ctx, double schwarz_tol, long long *n_stored);
+
+/* RHFI2G drop-in: G(i,j) = sum_kl D(k,l)[(ij|kl) - 1/4 (ik|jl) - 1/4 (il|jk)]
+ * (src/I2G/RHFI2G.f90:80-90) from the integrals of myqc_eri_build.
+ * precision MYQC_FP64 or MYQC_FP32. */
+int myqc_fock_G(myqc_ctx *ctx, const double *D, double *G, int precision);
+
+/* densRHF drop-in: D(u,v) = 2 sum_{i<nocc} C(u,i) C(v,i)
+ * (src/dens/dens.f90:115-124). */
+int myqc_density_rhf(myqc_ctx *ctx, const double *C, int nocc, double *D);
+
+/* slow_ao2mo_MP2_RHF drop-in: (ia|jb) for all i,j < nocc, a,b < nvrt
+ * (src/ao2mo/ao2mo.f90:465-602).  iajb is written in ijab_AB record order:
+ * record (i,j) i-major, each record nvrt x nvrt with a fastest
+ * (src/ao2mo/ao2mo.f90:575-582); iajb may be NULL to keep it on the device
+ * for myqc_mp2_rhf.  precision MYQC_FP64 or MYQC_BF16X3. */
+int myqc_ao2mo_ovov(myqc_ctx *ctx, const double *C, int nocc, double *iajb, int precision);
+
+/* mp2_rhf drop-in: E(AA), E(AB) and E(MBPT2) = 2 E(AA) + E(AB)
+ * (src/mp2/mp2.f90:79-144) over the (ia|jb) of the last myqc_ao2mo_ovov. */
+int myqc_mp2_rhf(myqc_ctx *ctx, const double *eig, int nocc, double *e_aa, double *e_ab, double *e_tot);
+
+/* Core Hamiltonian and overlap, int1e drop-in (src/integrals/int1e.f90:132-280). */
+int myqc_int1e(myqc_ctx *ctx, double *S, double *H);
+
+/* In-process RHF: the scf.f90 RHF loop (src/scf/scf.f90:61-216,720-904)
+ * with the dens/RHFI2G spawns replaced by device-resident D, G and the
+ * integrals of myqc_eri_build, plus DIIS.  C_guess may be NULL (core-H guess,
+ * scf.f90:410-512).  Converged when |dE| < e_tol and max|dD| < d_tol.
+ * precision MYQC_FP64, or MYQC_FP32 for incremental dD Fock builds with FP32
+ * digestion and a final FP64 rebuild.  Outputs: total energy (with E_nuc),
+ * MO coefficients C (n x n), orbital energies eig (n), iteration count. */
+int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc, const double *C_guess,
+                 double e_tol, double d_tol, int max_iter, int precision, double *energy, double *C, double *eig,
+                 int *iterations);
+
+/* Kernel timing accumulated on the context's stream with HIP events since
+ * the last reset: which = 0 ERI, 1 J/K digestion, 2 ao2mo, 3 MP2.
+ * Returns launches, total milliseconds and algorithmic work (flops or bytes). */
+int myqc_timer_reset(myqc_ctx *ctx, int enable);
+int myqc_timer_get(myqc_ctx *ctx, int which, long long *launches, double *ms, double *work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

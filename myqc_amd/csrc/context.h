@@ -1,0 +1,136 @@
+// context.h -- the myqc_ctx object behind the C ABI: one device, one stream,
+// device-resident basis data, stored integrals and SCF work arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/myqc_hip.h"
+#include "eri_math.h"
+#include "system.h"
+
+namespace myqc {
+
+// Unique shell quartets of one (bra pair class, ket pair class), enumerated
+// from Schwarz-sorted pair lists: bra rank r owns ket ranks [k0(r), k0(r)+cnt(r)).
+struct QuartetClass {
+    int cb = 0, ck = 0;
+    long long nq = 0;
+    int nbra = 0;
+    int blk = 0;
+    long long *d_cum = nullptr;   // nbra + 1 prefix sums of counts
+    double *d_v64 = nullptr;      // stored weighted integrals, nq * blk
+    float *d_v32 = nullptr;
+};
+
+enum TimerCat { T_ERI = 0, T_JK = 1, T_AO2MO = 2, T_MP2 = 3, T_NCAT = 4 };
+
+struct EventRec {
+    int cat;
+    hipEvent_t a, b;
+    double work;
+};
+
+}  // namespace myqc
+
+struct myqc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string err;
+
+    // system
+    bool have_system = false;
+    int n = 0, nnuc = 0;
+    std::vector<int> Z;
+    std::vector<double> xyz;              // [atom][3]
+    std::vector<myqc::Shell> shells;
+    std::vector<myqc::ShellPair> pairs;
+    std::vector<myqc::PrimPair> prims;
+    std::vector<int> pair_ids[3];         // per pair class, sorted by Schwarz value (descending)
+    std::vector<double> schwarz;          // per pair
+    std::vector<double> ftT;              // Ftab transposed: [Tk][m]
+    myqc::ShellPair *d_pairs = nullptr;
+    myqc::PrimPair *d_prims = nullptr;
+    double *d_ftT = nullptr;
+    int *d_sorted[3] = {nullptr, nullptr, nullptr};
+    double *d_schwarz = nullptr;
+    bool have_schwarz = false;
+
+    // stored integrals
+    bool have_eri = false;
+    bool have_eri32 = false;
+    double eri_tol = 0.0;
+    myqc::QuartetClass cls[6];
+    long long n_stored = 0;
+    long long n_quartets = 0;
+    double eri_flops = 0.0;               // algorithmic FP64 work of the last build
+
+    // work arrays (n x n unless noted)
+    double *d_D = nullptr, *d_A = nullptr, *d_G = nullptr;
+    float *d_D32 = nullptr, *d_A32 = nullptr;
+    int work_n = -1;
+
+    // ao2mo output kept on the device for mp2
+    double *d_iajb = nullptr;             // [i][a][j][b] with i fastest (o*v*o*v)
+    size_t iajb_elems = 0;
+    int iajb_nocc = 0, iajb_nv = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<myqc::EventRec> events;
+    long long t_n[myqc::T_NCAT] = {0, 0, 0, 0};
+    double t_ms[myqc::T_NCAT] = {0, 0, 0, 0};
+    double t_work[myqc::T_NCAT] = {0, 0, 0, 0};
+};
+
+namespace myqc {
+
+int set_error(myqc_ctx *ctx, int code, const char *fmt, ...);
+
+#define MYQC_HIP(ctx, call)                                                                          \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return myqc::set_error(ctx, MYQC_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                   __FILE__, __LINE__);                                              \
+    } while (0)
+
+#define MYQC_BLAS(ctx, call)                                                                         \
+    do {                                                                                             \
+        rocblas_status s_ = (call);                                                                  \
+        if (s_ != rocblas_status_success)                                                            \
+            return myqc::set_error(ctx, MYQC_E_HIP, "%s failed: rocblas status %d (%s:%d)", #call,     \
+                                   (int)s_, __FILE__, __LINE__);                                     \
+    } while (0)
+
+#define MYQC_CHECK(x)          \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+// device-to-host copy ordered on ctx->stream, complete at return
+hipError_t d2h(myqc_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+// device allocation helper (frees and reallocates to `bytes`)
+int dalloc(myqc_ctx *ctx, void **p, size_t bytes);
+void dfree(void **p);
+
+// timing helpers: bracket a launch sequence on ctx->stream
+int timer_begin(myqc_ctx *ctx, int cat, hipEvent_t *ev);
+int timer_end(myqc_ctx *ctx, int cat, hipEvent_t ev, double work);
+
+// internals shared between translation units
+int eri_schwarz(myqc_ctx *ctx);
+int eri_store(myqc_ctx *ctx, double tol, bool want32);
+int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX);
+int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision);
+int ensure_work(myqc_ctx *ctx);
+void free_eri(myqc_ctx *ctx);
+
+}  // namespace myqc

@@ -1,0 +1,305 @@
+// eri_math.h -- contracted s/p shell-quartet ERIs by Obara-Saika vertical
+// recursion and Head-Gordon-Pople horizontal recursion, FP64.
+//
+// Everything here is __host__ __device__ so the same code runs in the gfx950
+// kernels and in the host-side check against the oracle.
+//
+// Numerics follow the reference's integral path where they change values at
+// the 1e-10 level (src/integrals/int2e.f90, src/integrals/auxilary.f90):
+//   * pi rounded to single precision (the `Pi = 3.1415926535897931` literals
+//     without a D0 suffix, int2e.f90:621, auxilary.f90:169,196,711);
+//   * Boys F_m(T): for T < 12 a 7-term Taylor expansion of F_Q from the
+//     tabulated Ftab at Tk = NINT(10 T), Q = 3 * (sum of the four sets' max L),
+//     then downward recursion (auxilary.f90:130-162); for 12 <= T < 2Q+36 the
+//     asymptotic F_0 with the piecewise g(T), upward recursion
+//     (auxilary.f90:167-189,265-285); beyond, the simplified upward form
+//     (auxilary.f90:194-215);
+//   * primitive quartets with EIJ*EGH < 1e-14 skipped (int2e.f90:257).
+// The recursion itself (OS/HGP rather than McMurchie-Davidson) differs from
+// the reference only at rounding level.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define MYQC_HD __host__ __device__ __forceinline__
+
+namespace myqc {
+
+// `REAL(KIND=8),PARAMETER :: Pi = 3.1415926535897931` holds the single-precision value
+constexpr double PI_SP = (double)3.1415926535897931f;
+
+constexpr int FT_NT = 121;   // Tk = 0..120  (T = 0 .. 12 step 0.1)
+constexpr int FT_NM = 23;    // m  = 0..22
+
+// Cartesian components up to l = 2: 0 s | 1 x 2 y 3 z | 4 xx 5 xy 6 xz 7 yy 8 yz 9 zz
+MYQC_HD constexpr int cexp(int idx, int d) {
+    // exponents (x,y,z) per component
+    return (d == 0) ? (idx == 1 ? 1 : idx == 4 ? 2 : (idx == 5 || idx == 6) ? 1 : 0)
+         : (d == 1) ? (idx == 2 ? 1 : idx == 7 ? 2 : (idx == 5 || idx == 8) ? 1 : 0)
+                    : (idx == 3 ? 1 : idx == 9 ? 2 : (idx == 6 || idx == 8) ? 1 : 0);
+}
+MYQC_HD constexpr int lvl(int idx) { return idx == 0 ? 0 : idx < 4 ? 1 : 2; }
+MYQC_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+MYQC_HD constexpr int nupto(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }  // components of level <= l
+MYQC_HD constexpr int first(int l) { return nupto(l - 1 < 0 ? -1 : l - 1) * (l > 0); }
+// index of the component with exponents (x,y,z), total <= 2
+MYQC_HD constexpr int cidx(int x, int y, int z) {
+    return (x + y + z == 0) ? 0
+         : (x + y + z == 1) ? (x ? 1 : y ? 2 : 3)
+         : (x == 2 ? 4 : (x == 1 && y == 1) ? 5 : (x == 1 && z == 1) ? 6 : y == 2 ? 7 : (y == 1 && z == 1) ? 8 : 9);
+}
+// component minus one unit along d (-1 if none)
+MYQC_HD constexpr int lower(int idx, int d) {
+    return cexp(idx, d) == 0 ? -1
+         : cidx(cexp(idx, 0) - (d == 0), cexp(idx, 1) - (d == 1), cexp(idx, 2) - (d == 2));
+}
+// component plus one unit along d (level <= 2 only)
+MYQC_HD constexpr int raise(int idx, int d) {
+    return cidx(cexp(idx, 0) + (d == 0), cexp(idx, 1) + (d == 1), cexp(idx, 2) + (d == 2));
+}
+// the direction used to build a component by recursion: its first nonzero axis
+MYQC_HD constexpr int bdir(int idx) { return cexp(idx, 0) ? 0 : cexp(idx, 1) ? 1 : 2; }
+
+// Boys F_0..F_M(T) as the reference computes them at order Q (M <= Q <= 16).
+// ft is the table transposed to ftT[Tk*23 + m].
+template <int M>
+MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&F)[M + 1]) {
+    if (T < 12.0) {
+        int Tk = (int)floor(T * 10.0 + 0.5);          // NINT, T >= 0
+        double x = Tk / 10.0 - T;
+        const double *r = ftT + Tk * FT_NM + Q;
+        // sum_{k=0..6} Ft(Tk,Q+k) x^k / k!   (auxilary.f90:152-155)
+        double xk = 1.0, fq = 0.0;
+        const double inv_fact[7] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0};
+#pragma unroll
+        for (int k = 0; k <= 6; ++k) {
+            fq += r[k] * xk * inv_fact[k];
+            xk *= x;
+        }
+        double e = exp(-T);
+        double t2 = 2.0 * T;
+        // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160)
+        for (int j = Q - 1; j >= M; --j) fq = (t2 * fq + e) / (2.0 * j + 1.0);
+        // fq now holds F_top, top = min(Q, M); the reference never forms F_j for
+        // j > Q, and callers only read F_j for j <= Q
+        const int top = Q < M ? Q : M;
+#pragma unroll
+        for (int j = M; j >= 0; --j) {
+            if (j > top) F[j] = 0.0;
+            else if (j == top) F[j] = fq;
+            else F[j] = (t2 * F[j + 1] + e) / (2.0 * j + 1.0);
+        }
+    } else {
+        const double half_sqrt_pi = 0.5 * sqrt(PI_SP);
+        double rt = 1.0 / sqrt(T);
+        double it2 = 1.0 / (2.0 * T);
+        if (T < 2.0 * Q + 36.0) {
+            // auxilary.f90:181-187 with BoysG (:271-283); beyond T = 30 the
+            // reference's g is undefined and 0.49 is used (changes F_0 by
+            // at most 3e-15 |g|)
+            double g;
+            double iT = 1.0 / T;
+            if (T < 15.0) g = 0.4999489092 - 0.2473631686 * iT + 0.321180909 * iT * iT - 0.3811559346 * iT * iT * iT;
+            else if (T < 18.0) g = 0.4998436875 - 0.24249438 * iT + 0.24642845 * iT * iT;
+            else if (T < 24.0) g = 0.499093162 - 0.2152832 * iT;
+            else g = 0.490;
+            double e = exp(-T);
+            F[0] = half_sqrt_pi * rt - e * g * iT;
+#pragma unroll
+            for (int j = 1; j <= M; ++j) F[j] = it2 * ((2.0 * (j - 1) + 1.0) * F[j - 1] - e);
+        } else {
+            F[0] = half_sqrt_pi * rt;
+#pragma unroll
+            for (int j = 1; j <= M; ++j) F[j] = it2 * (2.0 * (j - 1) + 1.0) * F[j - 1];
+        }
+    }
+}
+
+// One primitive pair of a shell pair (first shell carries the larger L).
+struct PrimPair {
+    double p;        // alpha + beta
+    double P[3];     // Gaussian product centre
+    double PA[3];    // P - A (A = centre of the first shell)
+    double K;        // EIJ * c_a * c_b  (normalised contraction weights folded in)
+    double EIJ;      // exp(-alpha beta |AB|^2 / p): the reference's pair screen factor
+    double beta;     // exponent of the second shell's primitive (kinetic energy)
+};
+
+// Contracted shell pair.
+struct ShellPair {
+    double AB[3];    // A - B
+    int pp0;         // first PrimPair
+    int npp;         // number of primitive pairs
+    int ao_a, ao_b;  // first AO of each shell
+    int la, lb;      // angular momentum of each shell (la >= lb)
+    int bl;          // sum of the two shells' set max-L (Boys order Q = 3*(bl_bra+bl_ket))
+    int same;        // both shells identical
+};
+
+// Contracted [e0|f0] for e in levels LA..LA+LB and f in LC..LC+LD, accumulated
+// over all primitive quartets of the shell quartet (bra, ket).
+template <int LA, int LB, int LC, int LD>
+struct QuartetAcc {
+    static constexpr int EB = LA + LB, EK = LC + LD, M = EB + EK;
+    static constexpr int E0 = first(LA), NE = nupto(EB) - E0;
+    static constexpr int F0 = first(LC), NF = nupto(EK) - F0;
+    double v[NE][NF];
+};
+
+template <int LA, int LB, int LC, int LD>
+MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
+                           const PrimPair *__restrict__ ppk, const double *__restrict__ ftT,
+                           QuartetAcc<LA, LB, LC, LD> &acc) {
+    using A = QuartetAcc<LA, LB, LC, LD>;
+    constexpr int EB = A::EB, EK = A::EK, M = A::M;
+    constexpr int NB = nupto(EB);
+    const int Q = 3 * (bra.bl + ket.bl);
+#pragma unroll
+    for (int e = 0; e < A::NE; ++e)
+#pragma unroll
+        for (int f = 0; f < A::NF; ++f) acc.v[e][f] = 0.0;
+    const double c0 = 2.0 * pow(PI_SP, 2.5);
+    for (int i = 0; i < bra.npp; ++i) {
+        const PrimPair b = ppb[bra.pp0 + i];
+        for (int j = 0; j < ket.npp; ++j) {
+            const PrimPair k = ppk[ket.pp0 + j];
+            if (b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
+            const double p = b.p, q = k.p, pq = p + q;
+            const double ipq = 1.0 / pq;
+            const double rho = p * q * ipq;
+            double PQ[3], WP[3], WQ[3], QC[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                PQ[d] = b.P[d] - k.P[d];
+                // W = (pP + qQ)/(p+q):  W - P = -q/(p+q) PQ,  W - Q = p/(p+q) PQ
+                WP[d] = -q * ipq * PQ[d];
+                WQ[d] = p * ipq * PQ[d];
+                QC[d] = k.PA[d];
+            }
+            const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = c0 / (p * q * sqrt(pq)) * b.K * k.K;
+            double F[M + 1];
+            boys_ref<M>(Q, T, ftT, F);
+            // bra vertical recursion [e0|00]^(m)
+            double Vb[NB][M + 1];
+#pragma unroll
+            for (int m = 0; m <= M; ++m) Vb[0][m] = pref * F[m];
+            if constexpr (EB > 0) {
+                const double i2p = 0.5 / p, rp = rho / p;
+#pragma unroll
+                for (int idx = 1; idx < NB; ++idx) {
+                    const int d = bdir(idx), lo = lower(idx, d);
+                    const int am1 = cexp(lo, d);
+#pragma unroll
+                    for (int m = 0; m <= M - lvl(idx); ++m) {
+                        double v = b.PA[d] * Vb[lo][m] + WP[d] * Vb[lo][m + 1];
+                        if (am1 > 0) {
+                            const int lo2 = lower(lo, d);
+                            v += am1 * i2p * (Vb[lo2][m] - rp * Vb[lo2][m + 1]);
+                        }
+                        Vb[idx][m] = v;
+                    }
+                }
+            }
+            // ket vertical recursion [e0|f0]^(m), then accumulate the targets
+            if constexpr (EK == 0) {
+#pragma unroll
+                for (int e = A::E0; e < NB; ++e) acc.v[e - A::E0][0] += Vb[e][0];
+            } else {
+                constexpr int NK = nupto(EK);
+                const double i2q = 0.5 / q, rq = rho / q, i2pq = 0.5 * ipq;
+                double W[NB][NK][M + 1];
+#pragma unroll
+                for (int e = 0; e < NB; ++e)
+#pragma unroll
+                    for (int m = 0; m <= M - lvl(e); ++m) W[e][0][m] = Vb[e][m];
+#pragma unroll
+                for (int f = 1; f < NK; ++f) {
+                    const int d = bdir(f), lo = lower(f, d);
+                    const int cm1 = cexp(lo, d);
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) {
+#pragma unroll
+                        for (int m = 0; m <= M - lvl(e) - lvl(f); ++m) {
+                            double v = QC[d] * W[e][lo][m] + WQ[d] * W[e][lo][m + 1];
+                            if (cm1 > 0) {
+                                const int lo2 = lower(lo, d);
+                                v += cm1 * i2q * (W[e][lo2][m] - rq * W[e][lo2][m + 1]);
+                            }
+                            if (cexp(e, d) > 0) {
+                                const int el = lower(e, d);
+                                v += cexp(e, d) * i2pq * W[el][lo][m + 1];
+                            }
+                            W[e][f][m] = v;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int e = A::E0; e < NB; ++e)
+#pragma unroll
+                    for (int f = A::F0; f < NK; ++f) acc.v[e - A::E0][f - A::F0] += W[e][f][0];
+            }
+        }
+    }
+}
+
+// Horizontal recursion: contracted (ab|cd) with a,b,c,d component indices
+// within each shell (s: 0; p: 0,1,2 = x,y,z).  out[((a*NBb + b)*NCc + c)*NDd + d].
+template <int LA, int LB, int LC, int LD>
+MYQC_HD void quartet_hrr(const QuartetAcc<LA, LB, LC, LD> &acc, const double AB[3], const double CD[3],
+                         double *out) {
+    using A = QuartetAcc<LA, LB, LC, LD>;
+    constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(LC), nd = ncart(LD);
+    // ket HRR for every bra level-(LA..LA+LB) component e: I[e][c][d]
+    double I[A::NE][nc][nd];
+#pragma unroll
+    for (int e = 0; e < A::NE; ++e)
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+            for (int d = 0; d < nd; ++d) {
+                const int ci = first(LC) + c;       // component index of c
+                if constexpr (LD == 0) {
+                    I[e][c][d] = acc.v[e][ci - A::F0];
+                } else {
+                    // (e0|c p_d) = [e0|(c+1_d)0] + CD_d [e0|c0]
+                    I[e][c][d] = acc.v[e][raise(ci, d) - A::F0] + CD[d] * acc.v[e][ci - A::F0];
+                }
+            }
+#pragma unroll
+    for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int b = 0; b < nb; ++b)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+                for (int d = 0; d < nd; ++d) {
+                    const int ai = first(LA) + a;
+                    double v;
+                    if constexpr (LB == 0) {
+                        v = I[ai - A::E0][c][d];
+                    } else {
+                        v = I[raise(ai, b) - A::E0][c][d] + AB[b] * I[ai - A::E0][c][d];
+                    }
+                    out[((a * nb + b) * nc + c) * nd + d] = v;
+                }
+}
+
+// Degeneracy weight of element (i,j|k,l) of a shell-quartet block so that
+// every unique integral is visited once in the J/K digestion: 0 for a repeat
+// within a block whose shells or pairs coincide, else 1/2 per index equality
+// (i==j, k==l, (ij)==(kl)).
+MYQC_HD double block_weight(int a, int b, int c, int d, int same_ab, int same_cd, int same_pair, int nb, int nd,
+                            int i, int j, int k, int l) {
+    if (same_ab && a > b) return 0.0;
+    if (same_cd && c > d) return 0.0;
+    if (same_pair && (a * nb + b) > (c * nd + d)) return 0.0;
+    double w = 1.0;
+    if (i == j) w *= 0.5;
+    if (k == l) w *= 0.5;
+    if ((i == k && j == l) || (i == l && j == k)) w *= 0.5;
+    return w;
+}
+
+}  // namespace myqc

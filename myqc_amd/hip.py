@@ -1,0 +1,224 @@
+"""Python host mirror of the C ABI (include/myqc_hip.h) over ctypes.
+
+Each method names the reference program it replaces.  Arrays are numpy
+float64 in Fortran order, exactly as the Fortran host passes them.  There is
+no CPU fallback: if libmyqc_hip.so is missing, or no gfx950 device is usable,
+construction raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .basis import Basis, build_basis, read_ftab
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmyqc_hip.so")
+
+FP64, FP32, BF16X3 = 0, 1, 2
+T_ERI, T_JK, T_AO2MO, T_MP2 = 0, 1, 2, 3
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+EXPORTS = {
+    "myqc_init": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "myqc_finalize": ([_vp], ctypes.c_int),
+    "myqc_last_error": ([_vp], ctypes.c_char_p),
+    "myqc_set_system": ([_vp, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _ip, _dp,
+                         ctypes.c_int], ctypes.c_int),
+    "myqc_eri_full": ([_vp, ctypes.c_double, _dp], ctypes.c_int),
+    "myqc_eri_build": ([_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "myqc_fock_G": ([_vp, _dp, _dp, ctypes.c_int], ctypes.c_int),
+    "myqc_density_rhf": ([_vp, _dp, ctypes.c_int, _dp], ctypes.c_int),
+    "myqc_ao2mo_ovov": ([_vp, _dp, ctypes.c_int, _dp, ctypes.c_int], ctypes.c_int),
+    "myqc_mp2_rhf": ([_vp, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+    "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
+    "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
+                      ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
+    "myqc_timer_reset": ([_vp, ctypes.c_int], ctypes.c_int),
+    "myqc_timer_get": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp, _dp], ctypes.c_int),
+}
+
+_LIB = None
+
+
+class MyqcError(RuntimeError):
+    """A nonzero return code from the C ABI (the reference would `touch error`
+    and STOP, src/myQC/env.f90:36-41)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"myqc error {code}: {msg}")
+        self.code = code
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} not built: run `python -m myqc_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(p)
+    for name, (argt, rest) in EXPORTS.items():
+        fn = getattr(L, name)
+        fn.argtypes = argt
+        fn.restype = rest
+    if path is None:
+        _LIB = L
+    return L
+
+
+def _f(a, dtype=np.float64):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class Context:
+    """One device, one stream (myqc_init / myqc_finalize)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = _vp()
+        rc = self.L.myqc_init(device, ctypes.byref(h))
+        if rc != 0:
+            raise MyqcError(rc, f"myqc_init(device={device}) failed: no usable gfx950 device")
+        self.h = h
+        self.n = 0
+        self.basis: Basis | None = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MyqcError(rc, self.L.myqc_last_error(self.h).decode())
+
+    def close(self):
+        if self.h is not None:
+            self.L.myqc_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- inputs ----------------------------------------------------------
+    def set_system(self, Z, xyz_bohr, basis: Basis | None = None, ftab=None):
+        basis = basis or build_basis(list(Z))
+        ftab = read_ftab() if ftab is None else np.asarray(ftab, dtype=np.float64)
+        self._Z = np.ascontiguousarray(Z, dtype=np.int32)
+        self._xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3).ravel(order="F").copy()
+        self._alpha = np.ascontiguousarray(basis.set_alpha, dtype=np.float64)
+        self._setinfo = np.ascontiguousarray(basis.setinfo, dtype=np.int32)
+        self._bas = np.ascontiguousarray(basis.bas, dtype=np.float64)
+        self._basinfo = np.ascontiguousarray(basis.basinfo, dtype=np.int32)
+        self._ftab = np.ascontiguousarray(ftab, dtype=np.float64)
+        self._chk(self.L.myqc_set_system(self.h, len(self._Z), _p(self._Z, _ip), _p(self._xyz), basis.nset,
+                                         _p(self._alpha), _p(self._setinfo, _ip), _p(self._bas), basis.norb,
+                                         _p(self._basinfo, _ip), _p(self._ftab), 0))
+        self.n = basis.norb
+        self.basis = basis
+
+    # -- drop-ins ----------------------------------------------------------
+    def eri_full(self, schwarz_tol: float = 0.0) -> np.ndarray:
+        """int2e: dense 8-fold filled XX(i,j,g,h)."""
+        n = self.n
+        XX = np.zeros(n ** 4)
+        self._chk(self.L.myqc_eri_full(self.h, schwarz_tol, _p(XX)))
+        return XX.reshape(n, n, n, n, order="F")
+
+    def eri_build(self, schwarz_tol: float = 1e-12) -> int:
+        ns = ctypes.c_longlong()
+        self._chk(self.L.myqc_eri_build(self.h, schwarz_tol, ctypes.byref(ns)))
+        return ns.value
+
+    def fock_G(self, D, precision: int = FP64) -> np.ndarray:
+        """RHFI2G: G from the stored integrals."""
+        n = self.n
+        Df = _f(D)
+        G = np.zeros((n, n), order="F")
+        self._chk(self.L.myqc_fock_G(self.h, _p(Df), _p(G), precision))
+        return G
+
+    def density(self, C, nocc: int) -> np.ndarray:
+        n = self.n
+        Cf = _f(C)
+        D = np.zeros((n, n), order="F")
+        self._chk(self.L.myqc_density_rhf(self.h, _p(Cf), nocc, _p(D)))
+        return D
+
+    def int1e(self):
+        n = self.n
+        S = np.zeros((n, n), order="F")
+        H = np.zeros((n, n), order="F")
+        self._chk(self.L.myqc_int1e(self.h, _p(S), _p(H)))
+        return S, H
+
+    def ao2mo(self, C, nocc: int, precision: int = FP64, download: bool = True):
+        """ao2mo: returns iajb[i, a, j, b] = (ia|jb) (or None if not downloaded)."""
+        n = self.n
+        nv = n - nocc
+        Cf = _f(C)
+        if download:
+            rec = np.zeros(nocc * nocc * nv * nv)
+            self._chk(self.L.myqc_ao2mo_ovov(self.h, _p(Cf), nocc, _p(rec), precision))
+            # record (i,j): (a,b) with a fastest
+            r = rec.reshape(nocc, nocc, nv, nv)          # [i][j][b][a]
+            return np.ascontiguousarray(r.transpose(0, 3, 1, 2))   # [i][a][j][b]
+        self._chk(self.L.myqc_ao2mo_ovov(self.h, _p(Cf), nocc, None, precision))
+        return None
+
+    def mp2(self, eig, nocc: int):
+        """mp2: (E(AA), E(AB), E(MBPT2))."""
+        e = np.ascontiguousarray(eig, dtype=np.float64)
+        a, b, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(self.L.myqc_mp2_rhf(self.h, _p(e), nocc, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        return a.value, b.value, t.value
+
+    def rhf(self, H, S, nocc: int, e_nuc: float, C_guess=None, e_tol=1e-10, d_tol=1e-8, max_iter=200,
+            precision: int = FP64):
+        """In-process RHF; returns dict(E, C, eig, iterations)."""
+        n = self.n
+        Hf, Sf = _f(H), _f(S)
+        Cg = _f(C_guess) if C_guess is not None else None
+        C = np.zeros((n, n), order="F")
+        eig = np.zeros(n)
+        E = ctypes.c_double()
+        it = ctypes.c_int()
+        self._chk(self.L.myqc_rhf_scf(self.h, _p(Hf), _p(Sf), nocc, e_nuc, _p(Cg) if Cg is not None else None,
+                                      e_tol, d_tol, max_iter, precision, ctypes.byref(E), _p(C), _p(eig),
+                                      ctypes.byref(it)))
+        return dict(E=E.value, C=C, eig=eig, iterations=it.value)
+
+    # -- timing -------------------------------------------------------------
+    def timer_reset(self, enable: bool = True):
+        self._chk(self.L.myqc_timer_reset(self.h, 1 if enable else 0))
+
+    def timer_get(self, which: int):
+        n = ctypes.c_longlong()
+        ms = ctypes.c_double()
+        w = ctypes.c_double()
+        self._chk(self.L.myqc_timer_get(self.h, which, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(w)))
+        return n.value, ms.value, w.value
+
+
+def enuc(Z, xyz) -> float:
+    """Nuclear repulsion (src/scf/scf.f90:1122-1152)."""
+    xyz = np.asarray(xyz).reshape(-1, 3)
+    e = 0.0
+    for a in range(len(Z)):
+        for b in range(a):
+            e += Z[a] * Z[b] / np.sqrt(((xyz[a] - xyz[b]) ** 2).sum())
+    return e

@@ -15,7 +15,7 @@ OUT=$HERE/_ref
 if [ ! -d "$SRC" ]; then echo "reference sources not found at $SRC; skipping"; exit 0; fi
 mkdir -p "$OUT/mod"
 FC=${FC:-flang}
-FFLAGS="-O2 -w -module-dir $OUT/mod -I$OUT/mod"
+FFLAGS="-O3 -w -module-dir $OUT/mod -I$OUT/mod"
 $FC $FFLAGS -c "$SRC/myQC/env.f90" -o "$OUT/env.o"
 $FC $FFLAGS -c "$SRC/myQC/basis.f90" -o "$OUT/basis.o"
 $FC $FFLAGS -c "$SRC/integrals/auxilary.f90" -o "$OUT/auxilary.o"
@@ -27,3 +27,5 @@ $FC $FFLAGS -o "$OUT/ao2mo" "$SRC/ao2mo/ao2mo.f90" "$OUT/env.o"
 $FC $FFLAGS -o "$OUT/mp2" "$SRC/mp2/mp2.f90" "$OUT/env.o"
 $FC --version | head -1 > "$OUT/FLANG_VERSION"
 echo "reference programs built in $OUT"
+# Golden-vector harness over the reference's Boys routine
+$FC $FFLAGS -o "$OUT/boys_dump" "$HERE/boys_dump.f90" "$OUT/auxilary.o"

@@ -1,0 +1,108 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own
+programs, built from its sources by oracle/build_ref.sh (flang) into
+oracle/_ref/.  Run in a container that has /root/reference:
+
+    make -C oracle && python tests/golden/make_fixtures.py
+
+Each <name>.npz holds inputs (Z, xyz in bohr, charge) and the reference's
+outputs: S and H (int1e), a set of canonical ERIs (int2e XX), the first
+density/G pair (dens + RHFI2G on the core-H guess), the converged SCF energy,
+orbital energies and MO coefficients (the scf.f90 loop, restated in
+oracle/ref_driver.py because scf needs LAPACK), and E(AA)/E(AB)/E(MBPT2)
+(mp2, on ijab_AB from the restated ao2mo transform).
+
+boys.npz holds the reference Boys routine's F_0..F_Q(T) for Q in
+{0,3,6,9,12} (oracle/boys_dump.f90 over the compiled auxilary module).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from myqc_amd import molecules as M  # noqa: E402
+from oracle.ref_driver import REF, have_ref, run_reference  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def canonical_quads(n):
+    out = []
+    for i in range(n):
+        for j in range(i, n):
+            for g in range(i, n):
+                for h in range(j if g == i else g, n):
+                    out.append((i, j, g, h))
+    return np.array(out, dtype=np.int32).reshape(-1, 4)
+
+
+def write(name, Z, xyz_ang, charge=0, max_eri=20000, seed=0, mp2=True):
+    xyz = M.to_bohr(Z, xyz_ang)
+    r = run_reference(Z, xyz, charge=charge, mp2=mp2)
+    n = r["n"]
+    q = canonical_quads(n)
+    if len(q) > max_eri:
+        rng = np.random.default_rng(seed)
+        q = q[np.sort(rng.choice(len(q), size=max_eri, replace=False))]
+    XX = r["XX"]
+    vals = XX[q[:, 0], q[:, 1], q[:, 2], q[:, 3]]
+    d = dict(Z=np.array(Z, dtype=np.int32), xyz=xyz, charge=np.int32(charge), n=np.int32(n),
+             S=r["S"], H=r["H"], eri_idx=q, eri_val=vals, D_guess=r["D_guess"], G_guess=r["G_guess"],
+             C_guess=r["C_guess"], E_scf=np.float64(r["E_scf"]), energies=r["energies"], eig=r["eig"], C=r["C"],
+             iterations=np.int32(r["iterations"]), converged=np.bool_(r["converged"]), Enuc=np.float64(r["Enuc"]),
+             nocc=np.int32(r["nocc"]))
+    if n <= 15:
+        d["XX_full"] = XX
+    if mp2:
+        d.update(E_aa=np.float64(r["E_aa"]), E_ab=np.float64(r["E_ab"]), E_mp2=np.float64(r["E_mp2"]))
+        if n <= 15:
+            d["ijab_AB"] = r["ijab_AB"]
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+    print(f"{name}: n={n} E_scf={r['E_scf']:.12f} iters={r['iterations']} conv={r['converged']}"
+          + (f" E_mp2={r['E_mp2']:.12f}" if mp2 else ""), flush=True)
+
+
+def boys():
+    ftab = os.path.join(ROOT, "myqc_amd", "data", "Ftab")
+    subprocess.run(["cp", ftab, os.path.join(REF, "Ftab")], check=True)
+    txt = subprocess.run([os.path.join(REF, "boys_dump")], cwd=REF, capture_output=True, text=True, check=True).stdout
+    Q, T, F = [], [], []
+    for line in txt.splitlines():
+        t = line.split()
+        Q.append(int(t[0]))
+        T.append(float(t[1]))
+        f = np.zeros(13)
+        vals = [float(x) for x in t[2:]]
+        f[: len(vals)] = vals
+        F.append(f)
+    np.savez_compressed(os.path.join(OUT, "boys.npz"), Q=np.array(Q, dtype=np.int32), T=np.array(T), F=np.array(F))
+    print(f"boys: {len(Q)} points")
+
+
+def main():
+    if not have_ref():
+        sys.exit("reference programs missing: run `make -C oracle` in a container with /root/reference")
+    boys()
+    write("co2", *M.co2())
+    write("heh", [2, 1], np.array([[0, 0, 0], [0, 0, 1.0]]), charge=1)
+    write("h2o", *M.water())
+    write("h2o_2", *M.water_cluster(2, seed=2, box=4.0))
+    write("h2o_4", *M.water_cluster(4, seed=4, box=5.5))
+    rng = np.random.default_rng(4096)
+    k = 0
+    while k < 6:
+        Z, x = M.random_molecule(rng)
+        try:
+            write(f"mol{k}", Z, x, mp2=True)
+            k += 1
+        except Exception as e:  # reference failure (e.g. no convergence) -> skip that draw
+            print("skip:", e)
+
+
+if __name__ == "__main__":
+    main()

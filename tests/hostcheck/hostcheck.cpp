@@ -1,0 +1,157 @@
+// hostcheck.cpp -- TEST INFRASTRUCTURE: runs the library's own integral math
+// (eri_math.h, one_e_math.h, system.h) on the CPU so it can be compared with
+// the oracle without a GPU.  Built host-only; never part of the product path.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../myqc_amd/csrc/eri_math.h"
+#include "../../myqc_amd/csrc/one_e_math.h"
+#include "../../myqc_amd/csrc/system.h"
+
+using namespace myqc;
+
+namespace {
+
+struct Sys {
+    std::vector<Shell> shells;
+    std::vector<ShellPair> pairs;
+    std::vector<PrimPair> prims;
+    std::vector<int> ids[3];
+    std::vector<double> ftT, xyz;
+    int n = 0;
+};
+
+std::string make(Sys &s, int nnuc, const double *xyzF, int nset, const double *alpha, const int *setinfo,
+                 const double *bas, int norb, const int *basinfo, const double *Ftab) {
+    s.xyz.resize(3 * nnuc);
+    for (int a = 0; a < nnuc; ++a)
+        for (int c = 0; c < 3; ++c) s.xyz[3 * a + c] = xyzF[a + nnuc * c];
+    s.n = norb;
+    s.ftT.assign(FT_NT * FT_NM, 0.0);
+    for (int tk = 0; tk < FT_NT; ++tk)
+        for (int m = 0; m < FT_NM; ++m) s.ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
+    return build_system(nnuc, s.xyz.data(), nset, alpha, setinfo, bas, norb, basinfo, s.shells, s.pairs, s.prims, s.ids);
+}
+
+int pclass(const ShellPair &p) { return p.la + p.lb; }
+
+template <int LA, int LB, int LC, int LD, typename Fn>
+void one(const Sys &s, int pb, int pk, Fn &&fn) {
+    QuartetAcc<LA, LB, LC, LD> acc;
+    const ShellPair &bra = s.pairs[pb], &ket = s.pairs[pk];
+    quartet_prims<LA, LB, LC, LD>(bra, ket, s.prims.data(), s.prims.data(), s.ftT.data(), acc);
+    constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(LC), nd = ncart(LD);
+    double out[na * nb * nc * nd];
+    quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, ket.AB, out);
+    for (int ia = 0; ia < na; ++ia)
+        for (int ib = 0; ib < nb; ++ib)
+            for (int ic = 0; ic < nc; ++ic)
+                for (int id = 0; id < nd; ++id) {
+                    const int i = bra.ao_a + ia, j = bra.ao_b + ib, k = ket.ao_a + ic, l = ket.ao_b + id;
+                    const double w = block_weight(ia, ib, ic, id, bra.same, ket.same, pb == pk, nb, nd, i, j, k, l);
+                    fn(i, j, k, l, out[((ia * nb + ib) * nc + ic) * nd + id], w);
+                }
+}
+
+// visit every unique shell quartet once, bra = pair of higher class
+template <typename Fn>
+void all_quartets(const Sys &s, Fn &&fn) {
+    const int np = (int)s.pairs.size();
+    for (int P = 0; P < np; ++P)
+        for (int Q = P; Q < np; ++Q) {
+            int pb = P, pk = Q;
+            if (pclass(s.pairs[P]) < pclass(s.pairs[Q])) std::swap(pb, pk);
+            const int cb = pclass(s.pairs[pb]), ck = pclass(s.pairs[pk]);
+            if (cb == 0 && ck == 0) one<0, 0, 0, 0>(s, pb, pk, fn);
+            else if (cb == 1 && ck == 0) one<1, 0, 0, 0>(s, pb, pk, fn);
+            else if (cb == 1 && ck == 1) one<1, 0, 1, 0>(s, pb, pk, fn);
+            else if (cb == 2 && ck == 0) one<1, 1, 0, 0>(s, pb, pk, fn);
+            else if (cb == 2 && ck == 1) one<1, 1, 1, 0>(s, pb, pk, fn);
+            else one<1, 1, 1, 1>(s, pb, pk, fn);
+        }
+}
+
+}  // namespace
+
+extern "C" int hc_eri_full(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                           const double *bas, int norb, const int *basinfo, const double *Ftab, double *XX) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    const size_t n = norb;
+    memset(XX, 0, sizeof(double) * n * n * n * n);
+    all_quartets(s, [&](size_t i, size_t j, size_t k, size_t l, double v, double w) {
+        if (w == 0.0) return;
+        XX[i + n * (j + n * (k + n * l))] = v;
+        XX[j + n * (i + n * (k + n * l))] = v;
+        XX[i + n * (j + n * (l + n * k))] = v;
+        XX[j + n * (i + n * (l + n * k))] = v;
+        XX[k + n * (l + n * (i + n * j))] = v;
+        XX[l + n * (k + n * (i + n * j))] = v;
+        XX[k + n * (l + n * (j + n * i))] = v;
+        XX[l + n * (k + n * (j + n * i))] = v;
+    });
+    return 0;
+}
+
+// the digestion formula of jk.hip applied to every unique integral
+extern "C" int hc_fock_G(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                         const double *bas, int norb, const int *basinfo, const double *Ftab, const double *D,
+                         double *G) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    const size_t n = norb;
+    std::vector<double> Ds(n * n), A(n * n, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) Ds[i + n * j] = 0.5 * (D[i + n * j] + D[j + n * i]);
+    all_quartets(s, [&](size_t i, size_t j, size_t k, size_t l, double v, double w) {
+        const double x = v * w;
+        A[i + n * j] += 2.0 * Ds[k + n * l] * x;
+        A[k + n * l] += 2.0 * Ds[i + n * j] * x;
+        A[i + n * k] -= 0.5 * Ds[j + n * l] * x;
+        A[j + n * k] -= 0.5 * Ds[i + n * l] * x;
+        A[i + n * l] -= 0.5 * Ds[j + n * k] * x;
+        A[j + n * l] -= 0.5 * Ds[i + n * k] * x;
+    });
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) G[i + n * j] = A[i + n * j] + A[j + n * i];
+    return 0;
+}
+
+extern "C" int hc_int1e(int nnuc, const int *Z, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                        const double *bas, int norb, const int *basinfo, const double *Ftab, double *S, double *H) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    std::vector<double> atoms(4 * nnuc);
+    for (int c = 0; c < nnuc; ++c) {
+        atoms[4 * c] = Z[c];
+        for (int d = 0; d < 3; ++d) atoms[4 * c + 1 + d] = s.xyz[3 * c + d];
+    }
+    const size_t n = norb;
+    for (const ShellPair &sp : s.pairs) {
+        double ss[3][3], hh[3][3];
+        one_e_pair(sp, s.prims.data(), s.ftT.data(), atoms.data(), nnuc, ss, hh);
+        for (int a = 0; a < (sp.la ? 3 : 1); ++a)
+            for (int b = 0; b < (sp.lb ? 3 : 1); ++b) {
+                const size_t i = sp.ao_a + a, j = sp.ao_b + b;
+                S[i + n * j] = S[j + n * i] = ss[a][b];
+                H[i + n * j] = H[j + n * i] = hh[a][b];
+            }
+    }
+    return 0;
+}
+
+extern "C" int hc_boys(int M, int Q, double T, const double *Ftab, double *F) {
+    std::vector<double> ftT(FT_NT * FT_NM);
+    for (int tk = 0; tk < FT_NT; ++tk)
+        for (int m = 0; m < FT_NM; ++m) ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
+    switch (M) {
+        case 0: { double f[1]; boys_ref<0>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
+        case 1: { double f[2]; boys_ref<1>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
+        case 2: { double f[3]; boys_ref<2>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
+        case 3: { double f[4]; boys_ref<3>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
+        case 4: { double f[5]; boys_ref<4>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
+        default: return -1;
+    }
+    return 0;
+}

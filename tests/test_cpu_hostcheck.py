@@ -1,0 +1,91 @@
+"""The library's own integral math (myqc_amd/csrc/eri_math.h, one_e_math.h,
+system.h) compiled host-only into tests/hostcheck/libhostcheck.so, against
+the reference's outputs: the OS/HGP recursions, the reference-compatible
+Boys function, the digestion weights and the 1e integrals are exercised here
+without a GPU.  The GPU tests run the same code in the kernels."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import MOLECULES, ROOT, golden
+from myqc_amd.basis import build_basis, read_ftab
+from oracle import oracle as O
+
+HC = os.path.join(ROOT, "tests", "hostcheck", "libhostcheck.so")
+FT = read_ftab()
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(HC):
+        pytest.skip("host check harness not built (run __graft_entry__.build())")
+    return ctypes.CDLL(HC)
+
+
+def _sys(name):
+    f = golden(name)
+    Z = [int(z) for z in f["Z"]]
+    return f, Z, O.System(Z, f["xyz"], build_basis(Z), FT)
+
+
+@pytest.mark.parametrize("name", MOLECULES)
+def test_host_eri_matches_reference(hc, name):
+    f, Z, s = _sys(name)
+    n = s.n
+    XX = np.zeros(n ** 4)
+    assert hc.hc_eri_full(*s.args(), XX.ctypes.data_as(_dp)) == 0
+    XX = XX.reshape(n, n, n, n, order="F")
+    q = f["eri_idx"]
+    assert np.max(np.abs(XX[q[:, 0], q[:, 1], q[:, 2], q[:, 3]] - f["eri_val"])) <= 1e-12
+
+
+@pytest.mark.parametrize("name", MOLECULES)
+def test_host_digestion_matches_RHFI2G(hc, name):
+    f, Z, s = _sys(name)
+    n = s.n
+    D = np.asfortranarray(f["D_guess"]).ravel(order="F").copy()
+    G = np.zeros(n * n)
+    assert hc.hc_fock_G(*s.args(), D.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
+    assert np.max(np.abs(G.reshape(n, n, order="F") - f["G_guess"])) <= 1e-11
+
+
+def test_host_digestion_nonsymmetric_density(hc):
+    # the reference's formula sees D only through D + D^T
+    f, Z, s = _sys("h2o_2")
+    n = s.n
+    rng = np.random.default_rng(3)
+    D = rng.standard_normal((n, n))
+    XX = O.eri_full(s)
+    ref = O.fock_G(XX, D)
+    Df = np.asfortranarray(D).ravel(order="F").copy()
+    G = np.zeros(n * n)
+    hc.hc_fock_G(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp))
+    assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
+
+
+@pytest.mark.parametrize("name", MOLECULES)
+def test_host_int1e_matches_reference(hc, name):
+    f, Z, s = _sys(name)
+    n = s.n
+    S = np.zeros(n * n)
+    H = np.zeros(n * n)
+    a = s.args()
+    assert hc.hc_int1e(s.nnuc, s.Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), *a[1:], S.ctypes.data_as(_dp),
+                       H.ctypes.data_as(_dp)) == 0
+    assert np.max(np.abs(S.reshape(n, n, order="F") - f["S"])) <= 1e-12
+    assert np.max(np.abs(H.reshape(n, n, order="F") - f["H"])) <= 1e-11
+
+
+def test_host_boys_matches_reference_routine(hc):
+    g = golden("boys")
+    F = np.zeros(5)
+    worst = 0.0
+    for Q, T, Fr in zip(g["Q"], g["T"], g["F"]):
+        Q = int(Q)
+        M = min(Q, 4)
+        assert hc.hc_boys(M, Q, ctypes.c_double(float(T)), FT.ctypes.data_as(_dp), F.ctypes.data_as(_dp)) == 0
+        worst = max(worst, np.max(np.abs(F[: M + 1] - Fr[: M + 1]) / np.abs(Fr[: M + 1])))
+    assert worst <= 1e-14
