@@ -88,6 +88,12 @@ int myqc_ao2mo_ovov(myqc_ctx *ctx, const double *C, int nocc, double *iajb, int 
  * (src/mp2/mp2.f90:79-144) over the (ia|jb) of the last myqc_ao2mo_ovov. */
 int myqc_mp2_rhf(myqc_ctx *ctx, const double *eig, int nocc, double *e_aa, double *e_ab, double *e_tot);
 
+/* mp2_rhf drop-in for a separate mp2 program: the same sums over (ia|jb)
+ * records read from an ijab_AB file (record (i,j) i-major, nvrt x nvrt with a
+ * fastest; src/mp2/mp2.f90:99-135), uploaded to the device. */
+int myqc_mp2_rhf_records(myqc_ctx *ctx, const double *iajb_records, const double *eig, int nocc, int nvrt,
+                         double *e_aa, double *e_ab, double *e_tot);
+
 /* Core Hamiltonian and overlap, int1e drop-in (src/integrals/int1e.f90:132-280). */
 int myqc_int1e(myqc_ctx *ctx, double *S, double *H);
 

@@ -206,3 +206,46 @@ extern "C" int myqc_mp2_rhf(myqc_ctx *ctx, const double *eig, int nocc, double *
     if (e_tot) *e_tot = 2.0 * s1 + s2;
     return 0;
 }
+
+// (ia|jb) records of an ijab_AB file (record (i,j) i-major, (a,b) a fastest)
+// to the device's [i][a][j][b] layout
+__global__ void k_from_records(const double *__restrict__ rec, double *__restrict__ O, int o, int nv) {
+    const size_t total = (size_t)o * o * nv * nv;
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (size_t)gridDim.x * blockDim.x) {
+        size_t x = r;
+        const int a = (int)(x % nv); x /= nv;
+        const int b = (int)(x % nv); x /= nv;
+        const int j = (int)(x % o); x /= o;
+        const int i = (int)x;
+        O[i + (size_t)o * (a + (size_t)nv * (j + (size_t)o * b))] = rec[r];
+    }
+}
+
+extern "C" int myqc_mp2_rhf_records(myqc_ctx *ctx, const double *iajb_records, const double *eig, int nocc, int nvrt,
+                                    double *e_aa, double *e_ab, double *e_tot) {
+    if (!ctx || !iajb_records || !eig) return MYQC_E_ARG;
+    if (nocc < 1 || nvrt < 1) return set_error(ctx, MYQC_E_ARG, "myqc_mp2_rhf_records: nocc=%d nvrt=%d", nocc, nvrt);
+    const size_t oe = (size_t)nocc * nocc * nvrt * nvrt;
+    double *d_rec = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&d_rec, sizeof(double) * oe));
+    if (ctx->iajb_elems != oe) {
+        int rc = dalloc(ctx, (void **)&ctx->d_iajb, sizeof(double) * oe);
+        if (rc) {
+            dfree((void **)&d_rec);
+            ctx->iajb_elems = 0;
+            return rc;
+        }
+        ctx->iajb_elems = oe;
+    }
+    if (hipMemcpyAsync(d_rec, iajb_records, sizeof(double) * oe, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        dfree((void **)&d_rec);
+        return set_error(ctx, MYQC_E_HIP, "myqc_mp2_rhf_records: upload failed");
+    }
+    k_from_records<<<1024, 256, 0, ctx->stream>>>(d_rec, ctx->d_iajb, nocc, nvrt);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    dfree((void **)&d_rec);
+    if (e != hipSuccess) return set_error(ctx, MYQC_E_HIP, "myqc_mp2_rhf_records: %s", hipGetErrorString(e));
+    ctx->iajb_nocc = nocc;
+    ctx->iajb_nv = nvrt;
+    return myqc_mp2_rhf(ctx, eig, nocc, e_aa, e_ab, e_tot);
+}

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair *__restrict__ p
     const int p = ids[t];
     const ShellPair sp = pairs[p];
     QuartetAcc<LA, LB, LA, LB> acc;
-    quartet_prims<LA, LB, LA, LB>(sp, sp, prims, prims, s_ft, acc);
+    quartet_prims<LA, LB, LA, LB, false>(sp, sp, prims, prims, s_ft, acc);
     double out[na * nb * na * nb];
     quartet_hrr<LA, LB, LA, LB>(acc, sp.AB, sp.AB, out);
     double m = 0.0;

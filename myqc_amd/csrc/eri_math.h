@@ -147,7 +147,10 @@ struct QuartetAcc {
     double v[NE][NF];
 };
 
-template <int LA, int LB, int LC, int LD>
+// SCREEN = false skips the reference's primitive-pair screen: used for the
+// Schwarz factors, since a primitive pair dropped from (ab|ab) by
+// EIJ^2 < 1e-14 can survive EIJ*EGH >= 1e-14 against a compact ket pair.
+template <int LA, int LB, int LC, int LD, bool SCREEN = true>
 MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
                            const PrimPair *__restrict__ ppk, const double *__restrict__ ftT,
                            QuartetAcc<LA, LB, LC, LD> &acc) {
@@ -164,7 +167,7 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
         const PrimPair b = ppb[bra.pp0 + i];
         for (int j = 0; j < ket.npp; ++j) {
             const PrimPair k = ppk[ket.pp0 + j];
-            if (b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
+            if (SCREEN && b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
             const double p = b.p, q = k.p, pq = p + q;
             const double ipq = 1.0 / pq;
             const double rho = p * q * ipq;

@@ -1,0 +1,45 @@
+!-----------------------------------------------------------------------
+! int2e -- the dense two-electron integral file XX on the GPU.  Program
+! contract of src/integrals/int2e.f90: reads envdat/nucpos/fmem, mybasis,
+! Ftab; writes basinfo/setinfo and XX, one unformatted record of
+! XX(i,j,g,h) = (ij|gh), n^4 column-major, 8-fold filled (int2e.f90:166,306);
+! skips if XX exists (int2e.f90:58-67).  Every shell quartet is computed
+! (the reference has no Schwarz screen).
+!-----------------------------------------------------------------------
+PROGRAM int2e
+  USE iso_c_binding
+  USE myqc_hip_api
+  USE myqc_host
+  IMPLICIT NONE
+  REAL(8), ALLOCATABLE :: xyz(:,:), bas(:), setv(:), XX(:,:,:,:)
+  INTEGER, ALLOCATABLE :: atoms(:), options(:), basinfo(:), setinfo(:)
+  REAL(8) :: fmem, t0, t1
+  INTEGER :: nnuc, nelcA, nelcB, n, rc, u, st
+  LOGICAL :: ex
+  TYPE(c_ptr) :: ctx
+
+  CALL CPU_TIME(t0)
+  WRITE(*,*)
+  WRITE(*,*) "int2e called"
+  CALL read_env(nnuc, nelcA, nelcB, xyz, atoms, fmem, options)
+  CALL check_error_sentinel()
+  CALL build_basis(options(2), atoms, bas, basinfo, setv, setinfo)
+  INQUIRE(file='XX', EXIST=ex)
+  IF (ex) THEN
+    WRITE(*,*) "Reading two electron integrals from intermediate"
+    STOP
+  END IF
+  WRITE(*,*) "Constructing two electron integrals"
+  n = basinfo(1)
+  ALLOCATE(XX(0:n-1,0:n-1,0:n-1,0:n-1), STAT=st)
+  IF (st /= 0) CALL host_fail("Could not allocate memory in int2e")
+  CALL open_system(ctx, atoms, xyz, bas, basinfo, setv, setinfo)
+  rc = myqc_eri_full(ctx, 0.0_c_double, XX)
+  IF (rc /= 0) CALL host_fail("myqc_eri_full: " // TRIM(myqc_error_text(ctx)))
+  rc = myqc_finalize(ctx)
+  OPEN(newunit=u, file='XX', status='replace', access='sequential', form='unformatted')
+  WRITE(u) XX
+  CLOSE(u)
+  CALL CPU_TIME(t1)
+  WRITE(*,'(1x,A41,2x,F8.4)') "Two electron integrals constructed in (s)", t1 - t0
+END PROGRAM int2e

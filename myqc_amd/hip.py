@@ -36,6 +36,7 @@ EXPORTS = {
     "myqc_density_rhf": ([_vp, _dp, ctypes.c_int, _dp], ctypes.c_int),
     "myqc_ao2mo_ovov": ([_vp, _dp, ctypes.c_int, _dp, ctypes.c_int], ctypes.c_int),
     "myqc_mp2_rhf": ([_vp, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+    "myqc_mp2_rhf_records": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
     "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
                       ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
@@ -185,6 +186,16 @@ class Context:
         e = np.ascontiguousarray(eig, dtype=np.float64)
         a, b, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self._chk(self.L.myqc_mp2_rhf(self.h, _p(e), nocc, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        return a.value, b.value, t.value
+
+    def mp2_records(self, iajb, eig, nocc: int):
+        """mp2 on host (ia|jb), iajb[i, a, j, b]: uploads ijab_AB-ordered records."""
+        nv = iajb.shape[1]
+        rec = np.ascontiguousarray(np.asarray(iajb, dtype=np.float64).transpose(0, 2, 3, 1))  # [i][j][b][a]
+        e = np.ascontiguousarray(eig, dtype=np.float64)
+        a, b, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(self.L.myqc_mp2_rhf_records(self.h, _p(rec), _p(e), nocc, nv, ctypes.byref(a), ctypes.byref(b),
+                                              ctypes.byref(t)))
         return a.value, b.value, t.value
 
     def rhf(self, H, S, nocc: int, e_nuc: float, C_guess=None, e_tol=1e-10, d_tol=1e-8, max_iter=200,
