@@ -23,8 +23,6 @@ struct QuartetClass {
     int nbra = 0;
     int blk = 0;
     long long *d_cum = nullptr;   // nbra + 1 prefix sums of counts
-    double *d_v64 = nullptr;      // stored weighted integrals, nq * blk
-    float *d_v32 = nullptr;
 };
 
 enum TimerCat { T_ERI = 0, T_JK = 1, T_AO2MO = 2, T_MP2 = 3, T_NCAT = 4 };
@@ -61,14 +59,35 @@ struct myqc_ctx {
     double *d_schwarz = nullptr;
     bool have_schwarz = false;
 
-    // stored integrals
+    // stored integrals: each unique shell quartet block is kept in up to four
+    // copies, one per ordered bra shell pair (A,B) that owns rows of G, in
+    // segments [(A*ns + B)*3 + ket class] of kets sorted by Schwarz factor
     bool have_eri = false;
     bool have_eri32 = false;
     double eri_tol = 0.0;
     myqc::QuartetClass cls[6];
-    long long n_stored = 0;
+    long long n_stored = 0;               // unique integral slots computed
+    long long n_expanded = 0;             // values held for digestion
     long long n_quartets = 0;
     double eri_flops = 0.0;               // algorithmic FP64 work of the last build
+    int ns = 0;                           // number of shells
+    std::vector<long long> segoff;        // ns*ns*3 segment offsets
+    std::vector<int> segcnt;              // ns*ns*3 kets per segment
+    long long *d_segoff = nullptr;
+    int *d_rank = nullptr;                // rank of each pair within its class's sorted list
+    double *d_x64 = nullptr;              // expanded values
+    float *d_x32 = nullptr;
+    // digestion schedule: work units (A, B, ket class) grouped into workgroups of one row shell A
+    int n_wg = 0, n_units = 0;
+    int *d_wg = nullptr;                  // [w] = {A, u0, u1}
+    int *d_unit = nullptr;                // [u] = {B, ck, count}
+    long long *d_unit_off = nullptr;
+    int *d_shell = nullptr;               // [s] = {ao, L, wg0, wg1}
+    int *d_ao_shell = nullptr;            // shell of each AO
+    double *d_partial = nullptr;          // n_wg x 3 x n
+    float *d_partial32 = nullptr;
+    double *d_Dpair = nullptr;            // packed density blocks per pair, 9 each
+    float *d_Dpair32 = nullptr;
 
     // work arrays (n x n unless noted)
     double *d_D = nullptr, *d_A = nullptr, *d_G = nullptr;

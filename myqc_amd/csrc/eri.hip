@@ -93,11 +93,35 @@ struct EriArgs {
     int nbra;
     long long nq;
     int same_class;
-    double *v64;
-    float *v32;
+    // MODE 0: expanded copies
+    const long long *segoff;   // [(X*ns + Y)*3 + ket class]
+    const int *rank;           // rank of a pair within its class's sorted list
+    int ns;
+    double *x64;
+    float *x32;
+    // MODE 1: dense XX
     double *XX;
     int n;
 };
+
+// one copy of a quartet block, permuted to the ordered bra (X,Y) with the ket
+// in its stored orientation: dst[((x*ny + y)*nu + u)*nw + w]
+template <int NX, int NY, int NU, int NW, typename F>
+__device__ __forceinline__ void put_copy(double *__restrict__ x64, float *__restrict__ x32, long long base, F &&src) {
+#pragma unroll
+    for (int x = 0; x < NX; ++x)
+#pragma unroll
+        for (int y = 0; y < NY; ++y)
+#pragma unroll
+            for (int u = 0; u < NU; ++u)
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const long long e = base + ((x * NY + y) * NU + u) * NW + w;
+                    const double v = src(x, y, u, w);
+                    x64[e] = v;
+                    if (x32) x32[e] = (float)v;
+                }
+}
 
 __device__ __forceinline__ void find_quartet(const long long *__restrict__ cum, int nbra, long long t, int same,
                                              int &rb, long long &rk) {
@@ -117,7 +141,7 @@ __device__ __forceinline__ void load_ft(const double *__restrict__ ftT, double *
     __syncthreads();
 }
 
-// MODE 0: store weighted block values;  MODE 1: scatter into dense XX (8-fold)
+// MODE 0: store the block's copies for digestion;  MODE 1: scatter into dense XX (8-fold)
 template <int CB, int CK, int MODE>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     __shared__ double s_ft[FT_NT * FT_NM];
@@ -137,6 +161,25 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     double CD[3] = {ket.AB[0], ket.AB[1], ket.AB[2]};
     double out[BLK];
     quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, CD, out);
+    if (MODE == 0) {
+        // copies for the row owners A (bra (A,B)), B (bra (B,A)), C and D
+        const int A = bra.sa, B = bra.sb, C = ket.sa, D = ket.sb, ns = a.ns;
+        const long long rq = a.rank[pk], rp = a.rank[pb];
+        auto at = [&](int ia, int ib, int ic, int id) { return out[((ia * nb + ib) * nc + ic) * nd + id]; };
+        put_copy<na, nb, nc, nd>(a.x64, a.x32, a.segoff[((long long)A * ns + B) * 3 + CK] + rq * BLK,
+                                 [&](int x, int y, int u, int w) { return at(x, y, u, w); });
+        if (!bra.same)
+            put_copy<nb, na, nc, nd>(a.x64, a.x32, a.segoff[((long long)B * ns + A) * 3 + CK] + rq * BLK,
+                                     [&](int x, int y, int u, int w) { return at(y, x, u, w); });
+        if (pb != pk) {
+            put_copy<nc, nd, na, nb>(a.x64, a.x32, a.segoff[((long long)C * ns + D) * 3 + CB] + rp * BLK,
+                                     [&](int x, int y, int u, int w) { return at(u, w, x, y); });
+            if (!ket.same)
+                put_copy<nd, nc, na, nb>(a.x64, a.x32, a.segoff[((long long)D * ns + C) * 3 + CB] + rp * BLK,
+                                         [&](int x, int y, int u, int w) { return at(u, w, y, x); });
+        }
+        return;
+    }
     const int same_pair = (pb == pk);
 #pragma unroll
     for (int ia = 0; ia < na; ++ia)
@@ -149,24 +192,18 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
                     const int e = ((ia * nb + ib) * nc + ic) * nd + id;
                     const int i = bra.ao_a + ia, j = bra.ao_b + ib, k = ket.ao_a + ic, l = ket.ao_b + id;
                     const double w = block_weight(ia, ib, ic, id, bra.same, ket.same, same_pair, nb, nd, i, j, k, l);
-                    if (MODE == 0) {
-                        const double v = out[e] * w;
-                        a.v64[t * BLK + e] = v;
-                        if (a.v32) a.v32[t * BLK + e] = (float)v;
-                    } else {
-                        if (w != 0.0) {
-                            const size_t n = a.n;
-                            const double v = out[e];
-                            const size_t I = i, J = j, K = k, L = l;
-                            a.XX[I + n * (J + n * (K + n * L))] = v;
-                            a.XX[J + n * (I + n * (K + n * L))] = v;
-                            a.XX[I + n * (J + n * (L + n * K))] = v;
-                            a.XX[J + n * (I + n * (L + n * K))] = v;
-                            a.XX[K + n * (L + n * (I + n * J))] = v;
-                            a.XX[L + n * (K + n * (I + n * J))] = v;
-                            a.XX[K + n * (L + n * (J + n * I))] = v;
-                            a.XX[L + n * (K + n * (J + n * I))] = v;
-                        }
+                    if (w != 0.0) {
+                        const size_t n = a.n;
+                        const double v = out[e];
+                        const size_t I = i, J = j, K = k, L = l;
+                        a.XX[I + n * (J + n * (K + n * L))] = v;
+                        a.XX[J + n * (I + n * (K + n * L))] = v;
+                        a.XX[I + n * (J + n * (L + n * K))] = v;
+                        a.XX[J + n * (I + n * (L + n * K))] = v;
+                        a.XX[K + n * (L + n * (I + n * J))] = v;
+                        a.XX[L + n * (K + n * (I + n * J))] = v;
+                        a.XX[K + n * (L + n * (J + n * I))] = v;
+                        a.XX[L + n * (K + n * (J + n * I))] = v;
                     }
                 }
 }
@@ -207,10 +244,16 @@ static const int kBLK[6] = {1, 3, 9, 9, 27, 81};
 void free_eri(myqc_ctx *ctx) {
     for (auto &c : ctx->cls) {
         dfree((void **)&c.d_cum);
-        dfree((void **)&c.d_v64);
-        dfree((void **)&c.d_v32);
         c.nq = 0;
     }
+    void **ptrs[] = {(void **)&ctx->d_segoff, (void **)&ctx->d_rank,     (void **)&ctx->d_x64,
+                     (void **)&ctx->d_x32,    (void **)&ctx->d_wg,       (void **)&ctx->d_unit,
+                     (void **)&ctx->d_unit_off, (void **)&ctx->d_shell, (void **)&ctx->d_ao_shell,
+                     (void **)&ctx->d_partial, (void **)&ctx->d_partial32, (void **)&ctx->d_Dpair,
+                     (void **)&ctx->d_Dpair32};
+    for (auto p : ptrs) dfree(p);
+    ctx->n_wg = ctx->n_units = 0;
+    ctx->n_expanded = 0;
     ctx->have_eri = false;
     ctx->have_eri32 = false;
 }
@@ -277,7 +320,7 @@ static void class_counts(myqc_ctx *ctx, int cb, int ck, double tol, std::vector<
 }
 
 template <int CB, int CK, int MODE>
-static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *v64, float *v32, double *XX) {
+static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX) {
     EriArgs a;
     a.pairs = ctx->d_pairs;
     a.prims = ctx->d_prims;
@@ -288,8 +331,11 @@ static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *v64, float
     a.nbra = qc.nbra;
     a.nq = qc.nq;
     a.same_class = (CB == CK);
-    a.v64 = v64;
-    a.v32 = v32;
+    a.segoff = ctx->d_segoff;
+    a.rank = ctx->d_rank;
+    a.ns = ctx->ns;
+    a.x64 = ctx->d_x64;
+    a.x32 = ctx->d_x32;
     a.XX = XX;
     a.n = ctx->n;
     const long long nblk = (qc.nq + 255) / 256;
@@ -297,16 +343,16 @@ static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *v64, float
 }
 
 template <int MODE>
-static void launch_class(myqc_ctx *ctx, int c, double *v64, float *v32, double *XX) {
+static void launch_class(myqc_ctx *ctx, int c, double *XX) {
     const QuartetClass &qc = ctx->cls[c];
     if (qc.nq == 0) return;
     switch (c) {
-        case 0: launch_eri<0, 0, MODE>(ctx, qc, v64, v32, XX); break;
-        case 1: launch_eri<1, 0, MODE>(ctx, qc, v64, v32, XX); break;
-        case 2: launch_eri<1, 1, MODE>(ctx, qc, v64, v32, XX); break;
-        case 3: launch_eri<2, 0, MODE>(ctx, qc, v64, v32, XX); break;
-        case 4: launch_eri<2, 1, MODE>(ctx, qc, v64, v32, XX); break;
-        case 5: launch_eri<2, 2, MODE>(ctx, qc, v64, v32, XX); break;
+        case 0: launch_eri<0, 0, MODE>(ctx, qc, XX); break;
+        case 1: launch_eri<1, 0, MODE>(ctx, qc, XX); break;
+        case 2: launch_eri<1, 1, MODE>(ctx, qc, XX); break;
+        case 3: launch_eri<2, 0, MODE>(ctx, qc, XX); break;
+        case 4: launch_eri<2, 1, MODE>(ctx, qc, XX); break;
+        case 5: launch_eri<2, 2, MODE>(ctx, qc, XX); break;
     }
 }
 
@@ -345,18 +391,142 @@ static int build_classes(myqc_ctx *ctx, double tol) {
     return 0;
 }
 
+// Segments of the expanded storage and the digestion schedule.  For every
+// ordered shell pair (X,Y) and ket class ck, the kets of class ck whose
+// Schwarz product with (X,Y) passes `tol` form a prefix of that class's
+// sorted pair list; their blocks (X Y|ket) are stored contiguously.
+static int build_expanded(myqc_ctx *ctx, double tol) {
+    const int ns = (int)ctx->shells.size();
+    ctx->ns = ns;
+    const int np = (int)ctx->pairs.size();
+    // pair of each unordered shell pair, and its Schwarz factor
+    std::vector<int> pair_of((size_t)ns * ns, -1);
+    for (int p = 0; p < np; ++p) {
+        pair_of[(size_t)ctx->pairs[p].sa * ns + ctx->pairs[p].sb] = p;
+        pair_of[(size_t)ctx->pairs[p].sb * ns + ctx->pairs[p].sa] = p;
+    }
+    std::vector<int> rank(np, 0);
+    std::vector<double> qsorted[3];
+    for (int c = 0; c < 3; ++c) {
+        const auto &ids = ctx->pair_ids[c];
+        qsorted[c].resize(ids.size());
+        for (size_t r = 0; r < ids.size(); ++r) {
+            rank[ids[r]] = (int)r;
+            qsorted[c][r] = ctx->schwarz[ids[r]];
+        }
+    }
+    ctx->segoff.assign((size_t)ns * ns * 3, 0);
+    ctx->segcnt.assign((size_t)ns * ns * 3, 0);
+    long long off = 0;
+    for (int X = 0; X < ns; ++X)
+        for (int Y = 0; Y < ns; ++Y) {
+            const int p = pair_of[(size_t)X * ns + Y];
+            const double qp = ctx->schwarz[p];
+            const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
+            for (int ck = 0; ck < 3; ++ck) {
+                const auto &qk = qsorted[ck];
+                long long cnt;
+                if (tol <= 0.0) cnt = (long long)qk.size();
+                else if (qp <= 0.0) cnt = 0;
+                else
+                    cnt = std::lower_bound(qk.begin(), qk.end(), tol / qp, [](double a, double b) { return a >= b; }) -
+                          qk.begin();
+                const size_t seg = ((size_t)X * ns + Y) * 3 + ck;
+                ctx->segoff[seg] = off;
+                ctx->segcnt[seg] = (int)cnt;
+                off += cnt * nxy * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
+            }
+        }
+    ctx->n_expanded = off;
+    // digestion schedule: per row shell X, its (Y, ck) units packed into
+    // workgroups of about kTarget values
+    const long long kTarget = 1 << 15;
+    std::vector<int> wg, unit, shell(4 * ns);
+    std::vector<long long> unit_off;
+    for (int X = 0; X < ns; ++X) {
+        shell[4 * X + 0] = ctx->shells[X].ao;
+        shell[4 * X + 1] = ctx->shells[X].L;
+        shell[4 * X + 2] = (int)(wg.size() / 3);
+        long long acc = 0;
+        int u0 = (int)(unit.size() / 3);
+        for (int Y = 0; Y < ns; ++Y)
+            for (int ck = 0; ck < 3; ++ck) {
+                const size_t seg = ((size_t)X * ns + Y) * 3 + ck;
+                const int cnt = ctx->segcnt[seg];
+                if (cnt == 0) continue;
+                const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
+                unit.push_back(Y);
+                unit.push_back(ck);
+                unit.push_back(cnt);
+                unit_off.push_back(ctx->segoff[seg]);
+                acc += (long long)cnt * nxy * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
+                if (acc >= kTarget) {
+                    const int u1 = (int)(unit.size() / 3);
+                    wg.push_back(X);
+                    wg.push_back(u0);
+                    wg.push_back(u1);
+                    u0 = u1;
+                    acc = 0;
+                }
+            }
+        const int u1 = (int)(unit.size() / 3);
+        if (u1 > u0) {
+            wg.push_back(X);
+            wg.push_back(u0);
+            wg.push_back(u1);
+        }
+        shell[4 * X + 3] = (int)(wg.size() / 3);
+    }
+    ctx->n_wg = (int)(wg.size() / 3);
+    ctx->n_units = (int)(unit.size() / 3);
+    std::vector<int> ao_shell(ctx->n);
+    for (int X = 0; X < ns; ++X)
+        for (int k = 0; k < (ctx->shells[X].L ? 3 : 1); ++k) ao_shell[ctx->shells[X].ao + k] = X;
+    const size_t nseg = (size_t)ns * ns * 3;
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_segoff, sizeof(long long) * nseg));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_rank, sizeof(int) * np));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_wg, sizeof(int) * std::max<size_t>(wg.size(), 1)));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit, sizeof(int) * std::max<size_t>(unit.size(), 1)));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit_off, sizeof(long long) * std::max<size_t>(unit_off.size(), 1)));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_shell, sizeof(int) * shell.size()));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_ao_shell, sizeof(int) * ao_shell.size()));
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_segoff, ctx->segoff.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_rank, rank.data(), sizeof(int) * np, hipMemcpyHostToDevice, ctx->stream));
+    if (!wg.empty())
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_wg, wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (!unit.empty()) {
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_unit, unit.data(), sizeof(int) * unit.size(), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_unit_off, unit_off.data(), sizeof(long long) * unit_off.size(),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    }
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_shell, shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_ao_shell, ao_shell.data(), sizeof(int) * ao_shell.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     MYQC_CHECK(eri_schwarz(ctx));
     free_eri(ctx);
     MYQC_CHECK(build_classes(ctx, tol));
+    MYQC_CHECK(build_expanded(ctx, tol));
+    const size_t nx = (size_t)std::max<long long>(ctx->n_expanded, 1);
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x64, sizeof(double) * nx));
+    if (want32) MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x32, sizeof(float) * nx));
+    const size_t part = (size_t)std::max(ctx->n_wg, 1) * 3 * ctx->n;
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * part));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial32, sizeof(float) * part));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair, sizeof(double) * 9 * ctx->pairs.size()));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair32, sizeof(float) * 9 * ctx->pairs.size()));
     hipEvent_t ev;
     MYQC_CHECK(timer_begin(ctx, T_ERI, &ev));
     for (int c = 0; c < 6; ++c) {
-        QuartetClass &qc = ctx->cls[c];
-        if (qc.nq == 0) continue;
-        MYQC_CHECK(dalloc(ctx, (void **)&qc.d_v64, sizeof(double) * qc.nq * qc.blk));
-        if (want32) MYQC_CHECK(dalloc(ctx, (void **)&qc.d_v32, sizeof(float) * qc.nq * qc.blk));
-        launch_class<0>(ctx, c, qc.d_v64, want32 ? qc.d_v32 : nullptr, nullptr);
+        if (ctx->cls[c].nq == 0) continue;
+        launch_class<0>(ctx, c, nullptr);
         MYQC_HIP(ctx, hipGetLastError());
     }
     MYQC_CHECK(timer_end(ctx, T_ERI, ev, ctx->eri_flops));
@@ -374,6 +544,8 @@ int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX) {
     // the dense fill shares the class enumeration with the stored build
     QuartetClass saved[6];
     bool had = ctx->have_eri;
+    const long long saved_stored = ctx->n_stored, saved_quartets = ctx->n_quartets;
+    const double saved_flops = ctx->eri_flops;
     for (int c = 0; c < 6; ++c) saved[c] = ctx->cls[c];
     for (int c = 0; c < 6; ++c) ctx->cls[c] = QuartetClass{};
     int rc = build_classes(ctx, tol);
@@ -381,7 +553,7 @@ int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX) {
         hipEvent_t ev;
         rc = timer_begin(ctx, T_ERI, &ev);
         for (int c = 0; c < 6 && rc == 0; ++c) {
-            launch_class<1>(ctx, c, nullptr, nullptr, d_XX);
+            launch_class<1>(ctx, c, d_XX);
             if (hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "ERI dense-fill launch failed");
         }
         if (rc == 0) rc = timer_end(ctx, T_ERI, ev, ctx->eri_flops);
@@ -391,6 +563,9 @@ int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX) {
     for (int c = 0; c < 6; ++c) dfree((void **)&ctx->cls[c].d_cum);
     for (int c = 0; c < 6; ++c) ctx->cls[c] = saved[c];
     ctx->have_eri = had;
+    ctx->n_stored = saved_stored;
+    ctx->n_quartets = saved_quartets;
+    ctx->eri_flops = saved_flops;
     return rc;
 }
 

@@ -135,6 +135,7 @@ struct ShellPair {
     int la, lb;      // angular momentum of each shell (la >= lb)
     int bl;          // sum of the two shells' set max-L (Boys order Q = 3*(bl_bra+bl_ket))
     int same;        // both shells identical
+    int sa, sb;      // shell indices of the first and second shell
 };
 
 // Contracted [e0|f0] for e in levels LA..LA+LB and f in LC..LC+LD, accumulated

@@ -30,125 +30,181 @@ template <> struct PL<2> { static constexpr int a = 1, b = 1; };
 template <typename TV>
 struct DigArgs {
     const ShellPair *pairs;
-    const int *sorted_b, *sorted_k;
-    const long long *cum;
-    int nbra;
-    long long nq;
-    int same_class;
-    const TV *v;
-    const TV *D;     // symmetric density (or density change), n x n
-    TV *A;           // accumulator, n x n
+    const int *sorted0, *sorted1, *sorted2;   // pair ids of each class, Schwarz-sorted
+    const int *wg;                            // [w] = {A, u0, u1}
+    const int *unit;                          // [u] = {B, ck, count}
+    const long long *unit_off;
+    const int *shell;                         // [s] = {ao, L, wg0, wg1}
+    const TV *x;                              // expanded values
+    const TV *D;                              // symmetric density (or change), n x n
+    const TV *Dpair;                          // D blocks per pair, [pair][c*3+d]
+    TV *partial;                              // [w][3][n]
     int n;
 };
 
-__device__ __forceinline__ void find_q(const long long *__restrict__ cum, int nbra, long long t, int same, int &rb,
-                                       long long &rk) {
-    int lo = 0, hi = nbra;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (cum[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    rb = lo;
-    rk = t - cum[lo] + (same ? lo : 0);
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
-__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
-
-template <int CB, int CK, typename TV>
-__global__ __launch_bounds__(256) void k_digest(DigArgs<TV> a) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nq) return;
-    constexpr int na = ncart(PL<CB>::a), nb = ncart(PL<CB>::b), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
+// One work unit: row shell A (LA), second bra shell B (LB), kets of class CK.
+// Lanes stride over the unit's kets; J (row A, columns of B) is reduced over
+// the wavefront, K (row A, columns of the ket shells) goes to LDS atomics.
+template <int LA, int LB, int CK, typename TV>
+__device__ __forceinline__ void digest_unit(const DigArgs<TV> &a, TV *s_rows, int aoB, int cnt, long long off,
+                                            int lane) {
+    constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
     constexpr int BLK = na * nb * nc * nd;
-    int rb;
-    long long rk;
-    find_q(a.cum, a.nbra, t, a.same_class, rb, rk);
-    const ShellPair bra = a.pairs[a.sorted_b[rb]], ket = a.pairs[a.sorted_k[rk]];
-    const int i0 = bra.ao_a, j0 = bra.ao_b, k0 = ket.ao_a, l0 = ket.ao_b;
+    const int *sorted = CK == 0 ? a.sorted0 : CK == 1 ? a.sorted1 : a.sorted2;
     const size_t n = a.n;
     const TV *__restrict__ D = a.D;
-    TV x[BLK];
-#pragma unroll
-    for (int e = 0; e < BLK; ++e) x[e] = a.v[t * BLK + e];
-    TV Jab[na][nb], Jcd[nc][nd], Kac[na][nc], Kbc[nb][nc], Kad[na][nd], Kbd[nb][nd];
+    TV J[na][nb];
 #pragma unroll
     for (int p = 0; p < na; ++p)
 #pragma unroll
-        for (int q = 0; q < nb; ++q) Jab[p][q] = 0;
+        for (int q = 0; q < nb; ++q) J[p][q] = 0;
+    for (int r = lane; r < cnt; r += 64) {
+        const int q = sorted[r];
+        const ShellPair kp = a.pairs[q];
+        const int aoC = kp.ao_a, aoD = kp.ao_b;
+        const TV *__restrict__ v = a.x + off + (long long)r * BLK;
+        TV x[BLK];
 #pragma unroll
-    for (int p = 0; p < nc; ++p)
+        for (int e = 0; e < BLK; ++e) x[e] = v[e];
+        TV dq[nc][nd];
 #pragma unroll
-        for (int q = 0; q < nd; ++q) Jcd[p][q] = 0;
+        for (int c = 0; c < nc; ++c)
 #pragma unroll
-    for (int p = 0; p < na; ++p)
+            for (int d = 0; d < nd; ++d) dq[c][d] = a.Dpair[(size_t)q * 9 + c * 3 + d];
+        const TV fj = kp.same ? (TV)1 : (TV)2;
+        // D(b, d) and D(b, c) for the exchange terms
+        TV dbd[nb][nd], dbc[nb][nc];
 #pragma unroll
-        for (int q = 0; q < nc; ++q) Kac[p][q] = 0;
+        for (int b = 0; b < nb; ++b) {
 #pragma unroll
-    for (int p = 0; p < nb; ++p)
+            for (int d = 0; d < nd; ++d) dbd[b][d] = D[(aoB + b) + n * (aoD + d)];
 #pragma unroll
-        for (int q = 0; q < nc; ++q) Kbc[p][q] = 0;
+            for (int c = 0; c < nc; ++c) dbc[b][c] = D[(aoB + b) + n * (aoC + c)];
+        }
 #pragma unroll
-    for (int p = 0; p < na; ++p)
+        for (int ia = 0; ia < na; ++ia) {
 #pragma unroll
-        for (int q = 0; q < nd; ++q) Kad[p][q] = 0;
+            for (int ib = 0; ib < nb; ++ib) {
+                TV js = 0;
 #pragma unroll
-    for (int p = 0; p < nb; ++p)
+                for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int q = 0; q < nd; ++q) Kbd[p][q] = 0;
+                    for (int d = 0; d < nd; ++d) js += dq[c][d] * x[((ia * nb + ib) * nc + c) * nd + d];
+                J[ia][ib] += fj * js;
+            }
+            // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
+#pragma unroll
+            for (int c = 0; c < nc; ++c) {
+                TV ks = 0;
+#pragma unroll
+                for (int ib = 0; ib < nb; ++ib)
+#pragma unroll
+                    for (int d = 0; d < nd; ++d) ks += dbd[ib][d] * x[((ia * nb + ib) * nc + c) * nd + d];
+                atomicAdd(&s_rows[ia * n + aoC + c], (TV)(-0.5) * ks);
+            }
+            // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
+            if (!kp.same) {
+#pragma unroll
+                for (int d = 0; d < nd; ++d) {
+                    TV ks = 0;
+#pragma unroll
+                    for (int ib = 0; ib < nb; ++ib)
+#pragma unroll
+                        for (int c = 0; c < nc; ++c) ks += dbc[ib][c] * x[((ia * nb + ib) * nc + c) * nd + d];
+                    atomicAdd(&s_rows[ia * n + aoD + d], (TV)(-0.5) * ks);
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int ia = 0; ia < na; ++ia)
 #pragma unroll
         for (int ib = 0; ib < nb; ++ib) {
-            const TV dij = D[(i0 + ia) + n * (j0 + ib)];
-#pragma unroll
-            for (int ic = 0; ic < nc; ++ic) {
-                const TV dik = D[(i0 + ia) + n * (k0 + ic)];
-                const TV djk = D[(j0 + ib) + n * (k0 + ic)];
-#pragma unroll
-                for (int id = 0; id < nd; ++id) {
-                    const TV v = x[((ia * nb + ib) * nc + ic) * nd + id];
-                    const TV dkl = D[(k0 + ic) + n * (l0 + id)];
-                    const TV djl = D[(j0 + ib) + n * (l0 + id)];
-                    const TV dil = D[(i0 + ia) + n * (l0 + id)];
-                    Jab[ia][ib] += dkl * v;
-                    Jcd[ic][id] += dij * v;
-                    Kac[ia][ic] += djl * v;
-                    Kbc[ib][ic] += dil * v;
-                    Kad[ia][id] += djk * v;
-                    Kbd[ib][id] += dik * v;
-                }
-            }
+            const TV t = wave_sum(J[ia][ib]);
+            if (lane == 0) atomicAdd(&s_rows[ia * n + aoB + ib], t);
         }
-    TV *A = a.A;
-#pragma unroll
-    for (int p = 0; p < na; ++p)
-#pragma unroll
-        for (int q = 0; q < nb; ++q) atomic_add(&A[(i0 + p) + n * (j0 + q)], (TV)2 * Jab[p][q]);
-#pragma unroll
-    for (int p = 0; p < nc; ++p)
-#pragma unroll
-        for (int q = 0; q < nd; ++q) atomic_add(&A[(k0 + p) + n * (l0 + q)], (TV)2 * Jcd[p][q]);
-#pragma unroll
-    for (int p = 0; p < na; ++p)
-#pragma unroll
-        for (int q = 0; q < nc; ++q) atomic_add(&A[(i0 + p) + n * (k0 + q)], (TV)(-0.5) * Kac[p][q]);
-#pragma unroll
-    for (int p = 0; p < nb; ++p)
-#pragma unroll
-        for (int q = 0; q < nc; ++q) atomic_add(&A[(j0 + p) + n * (k0 + q)], (TV)(-0.5) * Kbc[p][q]);
-#pragma unroll
-    for (int p = 0; p < na; ++p)
-#pragma unroll
-        for (int q = 0; q < nd; ++q) atomic_add(&A[(i0 + p) + n * (l0 + q)], (TV)(-0.5) * Kad[p][q]);
-#pragma unroll
-    for (int p = 0; p < nb; ++p)
-#pragma unroll
-        for (int q = 0; q < nd; ++q) atomic_add(&A[(j0 + p) + n * (l0 + q)], (TV)(-0.5) * Kbd[p][q]);
+}
+
+template <int LA, typename TV>
+__device__ __forceinline__ void digest_dispatch(const DigArgs<TV> &a, TV *s_rows, int LB, int ck, int aoB, int cnt,
+                                                long long off, int lane) {
+    if (LB == 0) {
+        if (ck == 0) digest_unit<LA, 0, 0>(a, s_rows, aoB, cnt, off, lane);
+        else if (ck == 1) digest_unit<LA, 0, 1>(a, s_rows, aoB, cnt, off, lane);
+        else digest_unit<LA, 0, 2>(a, s_rows, aoB, cnt, off, lane);
+    } else {
+        if (ck == 0) digest_unit<LA, 1, 0>(a, s_rows, aoB, cnt, off, lane);
+        else if (ck == 1) digest_unit<LA, 1, 1>(a, s_rows, aoB, cnt, off, lane);
+        else digest_unit<LA, 1, 2>(a, s_rows, aoB, cnt, off, lane);
+    }
+}
+
+// One workgroup per block of work units of one row shell A; writes the
+// rows of A it accumulated to its partial slot.
+template <typename TV>
+__global__ __launch_bounds__(256) void k_digest(DigArgs<TV> a) {
+    extern __shared__ unsigned char smem[];
+    TV *s_rows = reinterpret_cast<TV *>(smem);
+    const int w = blockIdx.x;
+    const int A = a.wg[3 * w], u0 = a.wg[3 * w + 1], u1 = a.wg[3 * w + 2];
+    const int LA = a.shell[4 * A + 1];
+    const int na = LA ? 3 : 1;
+    const int n = a.n;
+    for (int i = threadIdx.x; i < na * n; i += blockDim.x) s_rows[i] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int u = u0 + wave; u < u1; u += 4) {
+        const int B = a.unit[3 * u], ck = a.unit[3 * u + 1], cnt = a.unit[3 * u + 2];
+        const long long off = a.unit_off[u];
+        const int aoB = a.shell[4 * B], LB = a.shell[4 * B + 1];
+        if (LA == 0) digest_dispatch<0>(a, s_rows, LB, ck, aoB, cnt, off, lane);
+        else digest_dispatch<1>(a, s_rows, LB, ck, aoB, cnt, off, lane);
+    }
+    __syncthreads();
+    TV *dst = a.partial + (size_t)w * 3 * n;
+    for (int i = threadIdx.x; i < na * n; i += blockDim.x) dst[i] = s_rows[i];
+}
+
+// G(i,j) = sum over the workgroups of shell(i) of their partial rows
+template <typename TV>
+__global__ void k_gather_rows(const TV *__restrict__ partial, const int *__restrict__ shell,
+                              const int *__restrict__ ao_shell, int n, double *__restrict__ Graw) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= n) return;
+    const int A = ao_shell[i];
+    const int ia = i - shell[4 * A], w0 = shell[4 * A + 2], w1 = shell[4 * A + 3];
+    double s = 0.0;
+    for (int w = w0; w < w1; ++w) s += (double)partial[((size_t)w * 3 + ia) * n + j];
+    Graw[i + (size_t)n * j] = s;
+}
+
+// packed D blocks per pair: Dp[q][c*3+d] = D(ao_a + c, ao_b + d)
+template <typename TV>
+__global__ void k_pack_pairs(const ShellPair *__restrict__ pairs, int np, const TV *__restrict__ D, int n,
+                             TV *__restrict__ Dp) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const ShellPair sp = pairs[q];
+    const int nc = sp.la ? 3 : 1, nd = sp.lb ? 3 : 1;
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d)
+            Dp[(size_t)q * 9 + c * 3 + d] = (c < nc && d < nd) ? D[(sp.ao_a + c) + (size_t)n * (sp.ao_b + d)] : (TV)0;
 }
 
 // --- small elementwise kernels ------------------------------------------------
+
+__global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, size_t cnt);
 
 __global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
@@ -158,33 +214,9 @@ __global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ 
     if (Ds32) Ds32[i + (size_t)n * j] = (float)v;
 }
 
-// Ds32 = (float) sym(D - Dref)
-__global__ void k_symdiff32(const double *__restrict__ D, const double *__restrict__ Dref, float *__restrict__ Ds32,
-                            int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    const size_t a = i + (size_t)n * j, b = j + (size_t)n * i;
-    Ds32[a] = (float)(0.5 * ((D[a] - Dref[a]) + (D[b] - Dref[b])));
-}
 
-// G (+)= A + A^T
-__global__ void k_assemble(const double *__restrict__ A, double *__restrict__ G, int n, int accumulate) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    const double v = A[i + (size_t)n * j] + A[j + (size_t)n * i];
-    G[i + (size_t)n * j] = accumulate ? G[i + (size_t)n * j] + v : v;
-}
 
-__global__ void k_assemble32(const float *__restrict__ A, double *__restrict__ G, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    G[i + (size_t)n * j] += (double)A[i + (size_t)n * j] + (double)A[j + (size_t)n * i];
-}
 
-template <typename T>
-__global__ void k_zero(T *__restrict__ p, size_t cnt) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
-}
 
 static dim3 grid2(int n) { return dim3((n + 127) / 128, n); }
 
@@ -201,60 +233,57 @@ int ensure_work(myqc_ctx *ctx) {
 }
 
 template <typename TV>
-static int launch_digest(myqc_ctx *ctx, const TV *D, TV *A) {
-    for (int c = 0; c < 6; ++c) {
-        const QuartetClass &qc = ctx->cls[c];
-        if (qc.nq == 0) continue;
+static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *Dpair, TV *partial, double *Graw) {
+    const int np = (int)ctx->pairs.size();
+    k_pack_pairs<TV><<<(np + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_pairs, np, D, ctx->n, Dpair);
+    if (ctx->n_wg > 0) {
         DigArgs<TV> a;
         a.pairs = ctx->d_pairs;
-        a.sorted_b = ctx->d_sorted[qc.cb];
-        a.sorted_k = ctx->d_sorted[qc.ck];
-        a.cum = qc.d_cum;
-        a.nbra = qc.nbra;
-        a.nq = qc.nq;
-        a.same_class = qc.cb == qc.ck;
+        a.sorted0 = ctx->d_sorted[0];
+        a.sorted1 = ctx->d_sorted[1];
+        a.sorted2 = ctx->d_sorted[2];
+        a.wg = ctx->d_wg;
+        a.unit = ctx->d_unit;
+        a.unit_off = ctx->d_unit_off;
+        a.shell = ctx->d_shell;
+        a.x = x;
         a.D = D;
-        a.A = A;
+        a.Dpair = Dpair;
+        a.partial = partial;
         a.n = ctx->n;
-        if constexpr (sizeof(TV) == 8) a.v = (const TV *)qc.d_v64;
-        else a.v = (const TV *)qc.d_v32;
-        const unsigned nblk = (unsigned)((qc.nq + 255) / 256);
-        switch (c) {
-            case 0: k_digest<0, 0, TV><<<nblk, 256, 0, ctx->stream>>>(a); break;
-            case 1: k_digest<1, 0, TV><<<nblk, 256, 0, ctx->stream>>>(a); break;
-            case 2: k_digest<1, 1, TV><<<nblk, 256, 0, ctx->stream>>>(a); break;
-            case 3: k_digest<2, 0, TV><<<nblk, 256, 0, ctx->stream>>>(a); break;
-            case 4: k_digest<2, 1, TV><<<nblk, 256, 0, ctx->stream>>>(a); break;
-            case 5: k_digest<2, 2, TV><<<nblk, 256, 0, ctx->stream>>>(a); break;
-        }
-        MYQC_HIP(ctx, hipGetLastError());
+        const size_t shm = sizeof(TV) * 3 * ctx->n;
+        k_digest<TV><<<ctx->n_wg, 256, shm, ctx->stream>>>(a);
     }
+    k_gather_rows<TV><<<dim3((ctx->n + 127) / 128, ctx->n), 128, 0, ctx->stream>>>(partial, ctx->d_shell,
+                                                                                  ctx->d_ao_shell, ctx->n, Graw);
+    MYQC_HIP(ctx, hipGetLastError());
     return 0;
 }
 
-// G(D) into d_G (precision FP64), or G += G32(D) (precision FP32: D is then
-// the density change and the result is added to d_G)
+// G(D) into d_Gout (precision FP64), or d_Gout += G32(D) (precision FP32:
+// D is then the density change)
 int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) {
     if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "no stored integrals (call myqc_eri_build)");
     MYQC_CHECK(ensure_work(ctx));
     const int n = ctx->n;
     const size_t nn = (size_t)n * n;
+    if ((size_t)3 * n * sizeof(double) > 64 * 1024)
+        return set_error(ctx, MYQC_E_ARG, "digestion rows for n=%d exceed 64 KB of LDS", n);
     hipEvent_t ev;
     MYQC_CHECK(timer_begin(ctx, T_JK, &ev));
     if (precision == MYQC_FP64) {
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, ctx->d_D, nullptr, n);
-        k_zero<double><<<256, 256, 0, ctx->stream>>>(ctx->d_A, nn);
-        MYQC_CHECK(launch_digest<double>(ctx, ctx->d_D, ctx->d_A));
-        k_assemble<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, n, 0);
+        MYQC_CHECK(launch_digest<double>(ctx, ctx->d_D, ctx->d_x64, ctx->d_Dpair, ctx->d_partial, ctx->d_A));
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, nullptr, n);
     } else {
         if (!ctx->have_eri32) return set_error(ctx, MYQC_E_STATE, "no FP32 integrals stored");
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n);
-        k_zero<float><<<256, 256, 0, ctx->stream>>>(ctx->d_A32, nn);
-        MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_A32));
-        k_assemble32<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A32, d_Gout, n);
+        MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial32, ctx->d_A));
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n);
+        k_add<<<256, 256, 0, ctx->stream>>>(d_Gout, ctx->d_G, d_Gout, nn);
     }
     MYQC_HIP(ctx, hipGetLastError());
-    const double bytes = (precision == MYQC_FP64 ? 8.0 : 4.0) * (double)ctx->n_stored + 16.0 * nn;
+    const double bytes = (precision == MYQC_FP64 ? 8.0 : 4.0) * (double)ctx->n_expanded + 16.0 * nn;
     MYQC_CHECK(timer_end(ctx, T_JK, ev, bytes));
     return 0;
 }
@@ -263,10 +292,11 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
 // RHF driver
 // ---------------------------------------------------------------------------
 
-__global__ void k_scale_cols(double *__restrict__ U, const double *__restrict__ s, int n) {
+// U(:, j) *= s_j^power
+__global__ void k_scale_cols(double *__restrict__ U, const double *__restrict__ s, int n, double power) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= n) return;
-    U[i + (size_t)n * j] *= 1.0 / sqrt(s[j]);
+    U[i + (size_t)n * j] *= pow(s[j], power);
 }
 
 __global__ void k_antisym(const double *__restrict__ T, double *__restrict__ E, int n) {
@@ -286,12 +316,14 @@ __global__ void k_sub(const double *__restrict__ X, const double *__restrict__ Y
 }
 
 struct ScfBuffers {
-    double *H = nullptr, *S = nullptr, *X = nullptr, *F = nullptr, *Fx = nullptr, *D = nullptr, *Dn = nullptr,
-           *Dref = nullptr, *G = nullptr, *C = nullptr, *T1 = nullptr, *T2 = nullptr, *eig = nullptr, *work = nullptr;
+    double *H = nullptr, *S = nullptr, *X = nullptr, *Xi = nullptr, *F = nullptr, *Fo = nullptr, *Fx = nullptr,
+           *D = nullptr, *Dn = nullptr, *Do = nullptr, *Don = nullptr, *Dref = nullptr, *G = nullptr, *C = nullptr,
+           *T1 = nullptr, *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr;
     double *hF = nullptr, *hE = nullptr;   // DIIS history, m x n^2 each
     int *info = nullptr;
     void release() {
-        double **ps[] = {&H, &S, &X, &F, &Fx, &D, &Dn, &Dref, &G, &C, &T1, &T2, &eig, &work, &hF, &hE};
+        double **ps[] = {&H, &S, &X, &Xi, &F, &Fo, &Fx, &D, &Dn, &Do, &Don, &Dref, &G, &C, &T1, &T2, &P2, &eig, &work,
+                         &dsc, &hF, &hE};
         for (auto p : ps) dfree((void **)p);
         dfree((void **)&info);
     }
@@ -313,12 +345,136 @@ static int syevd(myqc_ctx *ctx, ScfBuffers &b, double *A, double *w, int n) {
     return 0;
 }
 
-// C = X C', eigenvalues into eig, from F (symmetric, AO basis)
-static int diag_fock(myqc_ctx *ctx, ScfBuffers &b, const double *F, int n) {
-    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, F, n, b.X, n, 0.0, b.T1, n));
-    MYQC_CHECK(gemm(ctx, rocblas_operation_transpose, rocblas_operation_none, n, n, n, 1.0, b.X, n, b.T1, n, 0.0, b.T2, n));
+// C = X C' and orbital energies from the orthonormal-basis Fock matrix Fo
+static int diag_orth(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, int n) {
+    MYQC_HIP(ctx, hipMemcpyAsync(b.T2, Fo, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
     MYQC_CHECK(syevd(ctx, b, b.T2, b.eig, n));
     MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.X, n, b.T2, n, 0.0, b.C, n));
+    return 0;
+}
+
+// Fo = X^T F X
+static int to_orth(myqc_ctx *ctx, ScfBuffers &b, const double *F, double *Fo, int n) {
+    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, F, n, b.X, n, 0.0, b.T1, n));
+    MYQC_CHECK(gemm(ctx, rocblas_operation_transpose, rocblas_operation_none, n, n, n, 1.0, b.X, n, b.T1, n, 0.0, Fo, n));
+    return 0;
+}
+
+// D = 2 X Do X^T
+static int to_ao_density(myqc_ctx *ctx, ScfBuffers &b, const double *Do, double *D, int n) {
+    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, n, 1.0, Do, n, b.X, n, 0.0, b.T1, n));
+    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 2.0, b.X, n, b.T1, n, 0.0, D, n));
+    return 0;
+}
+
+// Spectral bounds of a symmetric matrix by Gershgorin discs: out[0] = min, out[1] = max
+__global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ out) {
+    __shared__ double smin[1024], smax[1024];
+    double lo = 1e300, hi = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double r = 0.0;
+        for (int j = 0; j < n; ++j)
+            if (j != i) r += fabs(F[j + (size_t)n * i]);
+        const double c = F[i + (size_t)n * i];
+        lo = fmin(lo, c - r);
+        hi = fmax(hi, c + r);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smin[0];
+        out[1] = smax[0];
+    }
+}
+
+// P = (emax I - F) / (emax - emin): occupied levels map toward 1
+__global__ void k_sp2_init(const double *__restrict__ F, const double *__restrict__ bounds, double *__restrict__ P,
+                           int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    const double lo = bounds[0], hi = bounds[1];
+    const double s = 1.0 / (hi - lo);
+    P[i + (size_t)n * j] = ((i == j ? hi : 0.0) - F[i + (size_t)n * j]) * s;
+}
+
+// tr[0] = tr P, tr[1] = tr P^2
+__global__ __launch_bounds__(1024) void k_traces(const double *__restrict__ P, const double *__restrict__ P2, int n,
+                                                 double *__restrict__ tr) {
+    __shared__ double s0[1024], s1[1024];
+    double a = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a += P[i + (size_t)n * i];
+        c += P2[i + (size_t)n * i];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tr[0] = s0[0];
+        tr[1] = s1[0];
+    }
+}
+
+// one trace-correcting purification step (Niklasson, Phys. Rev. B 66,
+// 155115 (2002)): P <- P^2 or 2P - P^2, whichever trace lands nearer nocc
+__global__ void k_sp2_step(double *__restrict__ P, const double *__restrict__ P2, const double *__restrict__ tr, int n,
+                           double nocc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    const double t1 = tr[0], t2 = tr[1];
+    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
+    const size_t e = i + (size_t)n * j;
+    P[e] = square ? P2[e] : 2.0 * P[e] - P2[e];
+}
+
+// Orthonormal-basis density projector onto the nocc lowest levels of Fo by
+// purification; falls back to diagonalisation if it does not converge.
+static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int n, int nocc) {
+    if (nocc == 0) {
+        MYQC_HIP(ctx, hipMemsetAsync(P, 0, sizeof(double) * n * n, ctx->stream));
+        return 0;
+    }
+    k_gershgorin<<<1, 1024, 0, ctx->stream>>>(Fo, n, b.dsc);
+    k_sp2_init<<<grid2(n), 128, 0, ctx->stream>>>(Fo, b.dsc, P, n);
+    double tr[2] = {0.0, 0.0};
+    bool ok = false;
+    int extra = -1;
+    for (int it = 0; it < 120; ++it) {
+        MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, P, n, P, n, 0.0, b.P2, n));
+        k_traces<<<1, 1024, 0, ctx->stream>>>(P, b.P2, n, b.dsc + 2);
+        k_sp2_step<<<grid2(n), 128, 0, ctx->stream>>>(P, b.P2, b.dsc + 2, n, (double)nocc);
+        if (extra > 0 && --extra == 0) {
+            ok = true;
+            break;
+        }
+        if (extra < 0 && it >= 12 && (it & 3) == 3) {
+            MYQC_HIP(ctx, d2h(ctx, tr, b.dsc + 2, sizeof tr));
+            // tr P - tr P^2 = sum l(1-l) over the eigenvalues l of P; near
+            // convergence each step squares it, so two more steps finish
+            if (fabs(tr[0] - tr[1]) < 1e-8 && fabs(tr[0] - nocc) < 1e-6) extra = 2;
+        }
+    }
+    if (!ok) {
+        // no clean gap at this step: occupy the nocc lowest eigenvectors
+        MYQC_CHECK(diag_orth(ctx, b, Fo, n));
+        MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, 1.0, b.T2, n, b.T2, n, 0.0,
+                        P, n));
+    }
     return 0;
 }
 
@@ -392,22 +548,24 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
         b.release();
         return code;
     };
-#define TRY(x)                 \
-    do {                       \
-        rc = (x);              \
+#define TRY(x)                   \
+    do {                         \
+        rc = (x);                \
         if (rc) return fail(rc); \
     } while (0)
-    double **mats[] = {&b.H, &b.S, &b.X, &b.F, &b.Fx, &b.D, &b.Dn, &b.Dref, &b.G, &b.C, &b.T1, &b.T2};
+    double **mats[] = {&b.H, &b.S, &b.X, &b.Xi, &b.F, &b.Fo, &b.Fx, &b.D, &b.Dn, &b.Do, &b.Don, &b.Dref, &b.G, &b.C,
+                       &b.T1, &b.T2, &b.P2};
     for (auto m : mats) TRY(dalloc(ctx, (void **)m, sizeof(double) * nn));
     TRY(dalloc(ctx, (void **)&b.eig, sizeof(double) * n));
     TRY(dalloc(ctx, (void **)&b.work, sizeof(double) * n));
+    TRY(dalloc(ctx, (void **)&b.dsc, sizeof(double) * 8));
     TRY(dalloc(ctx, (void **)&b.hF, sizeof(double) * nn * MD));
     TRY(dalloc(ctx, (void **)&b.hE, sizeof(double) * nn * MD));
     TRY(dalloc(ctx, (void **)&b.info, sizeof(int)));
     if (hipMemcpyAsync(b.H, H, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
         hipMemcpyAsync(b.S, S, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         return fail(set_error(ctx, MYQC_E_HIP, "rhf: upload failed"));
-    // X = S^{-1/2}
+    // X = S^{-1/2}, Xi = S^{1/2} (symmetric orthogonalisation)
     if (hipMemcpyAsync(b.T2, b.S, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
         return fail(set_error(ctx, MYQC_E_HIP, "rhf: copy failed"));
     TRY(syevd(ctx, b, b.T2, b.eig, n));
@@ -417,16 +575,25 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
         if (s[0] <= 1e-10) return fail(set_error(ctx, MYQC_E_ARG, "rhf: overlap matrix singular (min eigenvalue %g)", s[0]));
     }
     hipMemcpyAsync(b.T1, b.T2, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
-    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n);
+    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n, -0.5);
     TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, n, 1.0, b.T1, n, b.T2, n, 0.0, b.X, n));
-    // guess
+    hipMemcpyAsync(b.T1, b.T2, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
+    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n, 0.5);
+    TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, n, 1.0, b.T1, n, b.T2, n, 0.0, b.Xi, n));
+    // guess density
     if (C_guess) {
         if (hipMemcpyAsync(b.C, C_guess, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
             return fail(set_error(ctx, MYQC_E_HIP, "rhf: upload failed"));
+        TRY(density(ctx, b.C, nocc, b.D, n));
+        // Do = S^{1/2} D S^{1/2} / 2
+        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.D, n, b.Xi, n, 0.0, b.T1, n));
+        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 0.5, b.Xi, n, b.T1, n, 0.0, b.Do, n));
     } else {
-        TRY(diag_fock(ctx, b, b.H, n));   // core Hamiltonian guess (scf.f90:410-512)
+        // core Hamiltonian guess (scf.f90:410-512)
+        TRY(to_orth(ctx, b, b.H, b.Fo, n));
+        TRY(purify(ctx, b, b.Fo, b.Do, n, nocc));
+        TRY(to_ao_density(ctx, b, b.Do, b.D, n));
     }
-    TRY(density(ctx, b.C, nocc, b.D, n));
     std::vector<double> Bm((size_t)MD * MD, 0.0);
     int hist = 0, head = 0;
     double E = 0.0, E_old = 0.0;
@@ -434,31 +601,27 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
     bool conv = false;
     // mixed precision: incremental FP32 builds F_k = F_{k-1} + G32(D_k - D_{k-1})
     // from an FP64 reference build; FP64 rebuild every 8 steps and at the end
-    bool incr = (precision == MYQC_FP32);
+    const bool incr = (precision == MYQC_FP32);
     int since_full = 0;
     bool final_fp64 = false;
     for (it = 1; it <= max_iter; ++it) {
         if (!incr || final_fp64 || since_full == 0 || since_full >= 8) {
             TRY(digest_G(ctx, b.D, b.G, MYQC_FP64));
-            hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
             since_full = 1;
         } else {
             k_sub<<<256, 256, 0, ctx->stream>>>(b.D, b.Dref, b.T1, nn);
             TRY(digest_G(ctx, b.T1, b.G, MYQC_FP32));   // G += G32(dD)
-            hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
             ++since_full;
         }
+        hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
         k_add<<<256, 256, 0, ctx->stream>>>(b.H, b.G, b.F, nn);
         E = 0.5 * (ddot(ctx, (int)nn, b.D, b.F, &rc) + ddot(ctx, (int)nn, b.D, b.H, &rc)) + e_nuc;
         if (rc) return fail(rc);
-        // DIIS error X^T (F D S - S D F) X
-        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.D, n, b.S, n, 0.0, b.T1, n));
-        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.F, n, b.T1, n, 0.0, b.T2, n));
-        k_antisym<<<grid2(n), 128, 0, ctx->stream>>>(b.T2, b.T1, n);
-        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.T1, n, b.X, n, 0.0, b.T2, n));
-        double *eslot = b.hE + nn * head, *fslot = b.hF + nn * head;
-        TRY(gemm(ctx, rocblas_operation_transpose, rocblas_operation_none, n, n, n, 1.0, b.X, n, b.T2, n, 0.0, eslot, n));
-        hipMemcpyAsync(fslot, b.F, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
+        // orthonormal basis: Fo = X^T F X, DIIS error Fo Do - Do Fo
+        double *fslot = b.hF + nn * head, *eslot = b.hE + nn * head;
+        TRY(to_orth(ctx, b, b.F, fslot, n));
+        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, fslot, n, b.Do, n, 0.0, b.T2, n));
+        k_antisym<<<grid2(n), 128, 0, ctx->stream>>>(b.T2, eslot, n);
         if (hist < MD) ++hist;
         for (int j = 0; j < hist; ++j) {
             const double v = ddot(ctx, (int)nn, eslot, b.hE + nn * j, &rc);
@@ -467,8 +630,7 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
         }
         if (rc) return fail(rc);
         const double err = sqrt(fabs(Bm[head * MD + head]));
-        // extrapolate
-        const double *Fuse = b.F;
+        const double *Fuse = fslot;
         if (hist >= 2) {
             std::vector<double> Bs((size_t)hist * hist), c;
             for (int i = 0; i < hist; ++i)
@@ -483,11 +645,12 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
         head = (head + 1) % MD;
         const double dE = fabs(E - E_old);
         E_old = E;
-        TRY(diag_fock(ctx, b, Fuse, n));
-        TRY(density(ctx, b.C, nocc, b.Dn, n));
+        TRY(purify(ctx, b, Fuse, b.Don, n, nocc));
+        TRY(to_ao_density(ctx, b, b.Don, b.Dn, n));
         const double dD = maxabs_diff(ctx, b, b.Dn, b.D, nn, &rc);
         if (rc) return fail(rc);
-        hipMemcpyAsync(b.D, b.Dn, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
+        std::swap(b.D, b.Dn);
+        std::swap(b.Do, b.Don);
         if (it > 1 && dE < e_tol && dD < d_tol && err < 100 * d_tol) {
             if (incr && !final_fp64) {
                 final_fp64 = true;   // confirm with FP64 builds
@@ -497,13 +660,14 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
             break;
         }
     }
-    // report E[D] with the orbitals of the plain Fock matrix of the last step
-    // (the reference prints E of the density it diagonalises, scf.f90:775-848)
+    // report E[D] with the orbitals of its plain Fock matrix (the reference
+    // prints E of the density it diagonalises, scf.f90:775-848)
     TRY(digest_G(ctx, b.D, b.G, MYQC_FP64));
     k_add<<<256, 256, 0, ctx->stream>>>(b.H, b.G, b.F, nn);
     E = 0.5 * (ddot(ctx, (int)nn, b.D, b.F, &rc) + ddot(ctx, (int)nn, b.D, b.H, &rc)) + e_nuc;
     if (rc) return fail(rc);
-    TRY(diag_fock(ctx, b, b.F, n));
+    TRY(to_orth(ctx, b, b.F, b.Fo, n));
+    TRY(diag_orth(ctx, b, b.Fo, n));
     if (energy) *energy = E;
     if (C_out) d2h(ctx, C_out, b.C, sizeof(double) * nn);
     if (eig_out) d2h(ctx, eig_out, b.eig, sizeof(double) * n);

@@ -111,6 +111,8 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
             sp.lb = sb.L;
             sp.bl = sa.bL + sb.bL;
             sp.same = (A == B);
+            sp.sa = f;
+            sp.sb = g;
             const double ab2 = pow(sp.AB[0], 2.0) + pow(sp.AB[1], 2.0) + pow(sp.AB[2], 2.0);
             for (size_t i = 0; i < sa.alpha.size(); ++i)
                 for (size_t j = 0; j < sb.alpha.size(); ++j) {

@@ -94,7 +94,8 @@ extern "C" int hc_eri_full(int nnuc, const double *xyz, int nset, const double *
     return 0;
 }
 
-// the digestion formula of jk.hip applied to every unique integral
+// the unique-integral form of the G build: each unique integral's six updates
+// with its degeneracy weight (block_weight, shared with the dense XX fill)
 extern "C" int hc_fock_G(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
                          const double *bas, int norb, const int *basinfo, const double *Ftab, const double *D,
                          double *G) {
@@ -153,5 +154,46 @@ extern "C" int hc_boys(int M, int Q, double T, const double *Ftab, double *F) {
         case 4: { double f[5]; boys_ref<4>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
         default: return -1;
     }
+    return 0;
+}
+
+// the row-owned digestion of jk.hip: for every ordered bra shell pair (A,B)
+// and every shell pair (C,D) in its stored orientation, row i in A gets
+//   J: G(i,j) += f D(k,l) (ij|kl),  f = 2 unless C == D
+//   K: G(i,k) -= D(j,l) (ij|kl)/2,  and if C != D  G(i,l) -= D(j,k) (ij|kl)/2
+// then G is symmetrised.
+extern "C" int hc_fock_G_rows(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                              const double *bas, int norb, const int *basinfo, const double *Ftab, const double *D,
+                              double *G) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    const size_t n = norb;
+    std::vector<double> XX(n * n * n * n);
+    if (hc_eri_full(nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab, XX.data()) != 0) return -1;
+    std::vector<double> Ds(n * n), Gr(n * n, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) Ds[i + n * j] = 0.5 * (D[i + n * j] + D[j + n * i]);
+    auto X = [&](size_t i, size_t j, size_t k, size_t l) { return XX[i + n * (j + n * (k + n * l))]; };
+    const int ns = (int)s.shells.size();
+    for (int A = 0; A < ns; ++A)
+        for (int B = 0; B < ns; ++B)
+            for (const ShellPair &q : s.pairs) {
+                const int na = s.shells[A].L ? 3 : 1, nb = s.shells[B].L ? 3 : 1;
+                const int nc = q.la ? 3 : 1, nd = q.lb ? 3 : 1;
+                const double fj = q.same ? 1.0 : 2.0;
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < nb; ++b)
+                        for (int c = 0; c < nc; ++c)
+                            for (int d = 0; d < nd; ++d) {
+                                const size_t i = s.shells[A].ao + a, j = s.shells[B].ao + b, k = q.ao_a + c,
+                                             l = q.ao_b + d;
+                                const double v = X(i, j, k, l);
+                                Gr[i + n * j] += fj * Ds[k + n * l] * v;
+                                Gr[i + n * k] -= 0.5 * Ds[j + n * l] * v;
+                                if (!q.same) Gr[i + n * l] -= 0.5 * Ds[j + n * k] * v;
+                            }
+            }
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) G[i + n * j] = 0.5 * (Gr[i + n * j] + Gr[j + n * i]);
     return 0;
 }

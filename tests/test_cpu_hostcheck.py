@@ -89,3 +89,21 @@ def test_host_boys_matches_reference_routine(hc):
         assert hc.hc_boys(M, Q, ctypes.c_double(float(T)), FT.ctypes.data_as(_dp), F.ctypes.data_as(_dp)) == 0
         worst = max(worst, np.max(np.abs(F[: M + 1] - Fr[: M + 1]) / np.abs(Fr[: M + 1])))
     assert worst <= 1e-14
+
+
+@pytest.mark.parametrize("name", ["co2", "h2o_2", "mol3"])
+def test_row_owned_digestion_formula(hc, name):
+    """The J/K formula the digestion kernel uses (rows owned by the first bra
+    shell, kets in their stored orientation) reproduces RHFI2G's G, for the
+    reference's density and for a nonsymmetric one."""
+    f, Z, s = _sys(name)
+    n = s.n
+    rng = np.random.default_rng(7)
+    for D, ref in ((f["D_guess"], f["G_guess"]), (None, None)):
+        if D is None:
+            D = rng.standard_normal((n, n))
+            ref = O.fock_G(O.eri_full(s), D)
+        Df = np.asfortranarray(D).ravel(order="F").copy()
+        G = np.zeros(n * n)
+        assert hc.hc_fock_G_rows(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
+        assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
