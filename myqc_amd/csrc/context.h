@@ -80,13 +80,14 @@ struct myqc_ctx {
     // digestion schedule: work units (A, B, ket class) grouped into workgroups of one row shell A
     int n_wg = 0, n_units = 0;
     int *d_wg = nullptr;                  // [w] = {A, u0, u1}
-    int *d_unit = nullptr;                // [u] = {B, ck, count}
-    long long *d_unit_off = nullptr;
+    int *d_unit = nullptr;                // [u] = {B, count ck=0, count ck=1, count ck=2}
+    long long *d_unit_off = nullptr;      // [u][3] segment offsets
+    int *d_meta[3] = {nullptr, nullptr, nullptr};   // per class, by Schwarz rank: aoC | aoD << 15 | same << 30
     int *d_shell = nullptr;               // [s] = {ao, L, wg0, wg1}
     int *d_ao_shell = nullptr;            // shell of each AO
     double *d_partial = nullptr;          // n_wg x 3 x n
     float *d_partial32 = nullptr;
-    double *d_Dpair = nullptr;            // packed density blocks per pair, 9 each
+    double *d_Dpair = nullptr;            // density blocks of the kets, by class and rank (see jk.hip)
     float *d_Dpair32 = nullptr;
 
     // work arrays (n x n unless noted)

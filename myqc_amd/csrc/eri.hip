@@ -252,6 +252,7 @@ void free_eri(myqc_ctx *ctx) {
                      (void **)&ctx->d_partial, (void **)&ctx->d_partial32, (void **)&ctx->d_Dpair,
                      (void **)&ctx->d_Dpair32};
     for (auto p : ptrs) dfree(p);
+    for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
     ctx->n_wg = ctx->n_units = 0;
     ctx->n_expanded = 0;
     ctx->have_eri = false;
@@ -438,8 +439,8 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
             }
         }
     ctx->n_expanded = off;
-    // digestion schedule: per row shell X, its (Y, ck) units packed into
-    // workgroups of about kTarget values
+    // digestion schedule: per row shell X, its units Y (all three ket
+    // classes) packed into workgroups of about kTarget values
     const long long kTarget = 1 << 15;
     std::vector<int> wg, unit, shell(4 * ns);
     std::vector<long long> unit_off;
@@ -448,28 +449,28 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
         shell[4 * X + 1] = ctx->shells[X].L;
         shell[4 * X + 2] = (int)(wg.size() / 3);
         long long acc = 0;
-        int u0 = (int)(unit.size() / 3);
-        for (int Y = 0; Y < ns; ++Y)
-            for (int ck = 0; ck < 3; ++ck) {
-                const size_t seg = ((size_t)X * ns + Y) * 3 + ck;
-                const int cnt = ctx->segcnt[seg];
-                if (cnt == 0) continue;
-                const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
-                unit.push_back(Y);
-                unit.push_back(ck);
-                unit.push_back(cnt);
-                unit_off.push_back(ctx->segoff[seg]);
-                acc += (long long)cnt * nxy * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
-                if (acc >= kTarget) {
-                    const int u1 = (int)(unit.size() / 3);
-                    wg.push_back(X);
-                    wg.push_back(u0);
-                    wg.push_back(u1);
-                    u0 = u1;
-                    acc = 0;
-                }
+        int u0 = (int)(unit.size() / 4);
+        for (int Y = 0; Y < ns; ++Y) {
+            const size_t seg = ((size_t)X * ns + Y) * 3;
+            const int c0 = ctx->segcnt[seg], c1 = ctx->segcnt[seg + 1], c2 = ctx->segcnt[seg + 2];
+            if (c0 + c1 + c2 == 0) continue;
+            const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
+            unit.push_back(Y);
+            unit.push_back(c0);
+            unit.push_back(c1);
+            unit.push_back(c2);
+            for (int ck = 0; ck < 3; ++ck) unit_off.push_back(ctx->segoff[seg + ck]);
+            acc += (long long)nxy * (c0 + 3LL * c1 + 9LL * c2);
+            if (acc >= kTarget) {
+                const int u1 = (int)(unit.size() / 4);
+                wg.push_back(X);
+                wg.push_back(u0);
+                wg.push_back(u1);
+                u0 = u1;
+                acc = 0;
             }
-        const int u1 = (int)(unit.size() / 3);
+        }
+        const int u1 = (int)(unit.size() / 4);
         if (u1 > u0) {
             wg.push_back(X);
             wg.push_back(u0);
@@ -478,7 +479,21 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
         shell[4 * X + 3] = (int)(wg.size() / 3);
     }
     ctx->n_wg = (int)(wg.size() / 3);
-    ctx->n_units = (int)(unit.size() / 3);
+    ctx->n_units = (int)(unit.size() / 4);
+    // ket metadata in Schwarz-rank order per class
+    for (int c = 0; c < 3; ++c) {
+        const auto &ids = ctx->pair_ids[c];
+        if (ids.empty()) continue;
+        std::vector<int> meta(ids.size());
+        for (size_t r = 0; r < ids.size(); ++r) {
+            const ShellPair &sp = ctx->pairs[ids[r]];
+            meta[r] = sp.ao_a | (sp.ao_b << 15) | (sp.same << 30);
+        }
+        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_meta[c], sizeof(int) * meta.size()));
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_meta[c], meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     std::vector<int> ao_shell(ctx->n);
     for (int X = 0; X < ns; ++X)
         for (int k = 0; k < (ctx->shells[X].L ? 3 : 1); ++k) ao_shell[ctx->shells[X].ao + k] = X;

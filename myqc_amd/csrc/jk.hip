@@ -29,15 +29,14 @@ template <> struct PL<2> { static constexpr int a = 1, b = 1; };
 
 template <typename TV>
 struct DigArgs {
-    const ShellPair *pairs;
-    const int *sorted0, *sorted1, *sorted2;   // pair ids of each class, Schwarz-sorted
+    const int *meta0, *meta1, *meta2;         // ket AOs by class and Schwarz rank
+    const TV *dk0, *dk1, *dk2;                // ket density blocks by class and rank (1, 3, 9 values)
     const int *wg;                            // [w] = {A, u0, u1}
-    const int *unit;                          // [u] = {B, ck, count}
-    const long long *unit_off;
+    const int *unit;                          // [u] = {B, c0, c1, c2}
+    const long long *unit_off;                // [u][3]
     const int *shell;                         // [s] = {ao, L, wg0, wg1}
     const TV *x;                              // expanded values
     const TV *D;                              // symmetric density (or change), n x n
-    const TV *Dpair;                          // D blocks per pair, [pair][c*3+d]
     TV *partial;                              // [w][3][n]
     int n;
 };
@@ -53,123 +52,126 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// One work unit: row shell A (LA), second bra shell B (LB), kets of class CK.
-// Lanes stride over the unit's kets; J (row A, columns of B) is reduced over
-// the wavefront, K (row A, columns of the ket shells) goes to LDS atomics.
+// One ket of rank r in class CK against the bra (A, B): J into the thread's
+// accumulator, K into the workgroup's LDS rows of A.  s_drow holds D's rows
+// of shell B.
 template <int LA, int LB, int CK, typename TV>
-__device__ __forceinline__ void digest_unit(const DigArgs<TV> &a, TV *s_rows, int aoB, int cnt, long long off,
-                                            int lane) {
+__device__ __forceinline__ void digest_ket(const DigArgs<TV> &a, TV *s_rows, const TV *s_drow, int r, long long off,
+                                           TV (&J)[3][3]) {
     constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
     constexpr int BLK = na * nb * nc * nd;
-    const int *sorted = CK == 0 ? a.sorted0 : CK == 1 ? a.sorted1 : a.sorted2;
-    const size_t n = a.n;
-    const TV *__restrict__ D = a.D;
-    TV J[na][nb];
+    const int n = a.n;
+    const int m = (CK == 0 ? a.meta0 : CK == 1 ? a.meta1 : a.meta2)[r];
+    const int aoC = m & 0x7fff, aoD = (m >> 15) & 0x7fff, same = m >> 30;
+    const TV *__restrict__ v = a.x + off + (long long)r * BLK;
+    TV x[BLK];
 #pragma unroll
-    for (int p = 0; p < na; ++p)
+    for (int e = 0; e < BLK; ++e) x[e] = v[e];
+    const TV *dk = (CK == 0 ? a.dk0 : CK == 1 ? a.dk1 : a.dk2) + (size_t)r * (nc * nd);
+    TV dq[nc][nd];
 #pragma unroll
-        for (int q = 0; q < nb; ++q) J[p][q] = 0;
-    for (int r = lane; r < cnt; r += 64) {
-        const int q = sorted[r];
-        const ShellPair kp = a.pairs[q];
-        const int aoC = kp.ao_a, aoD = kp.ao_b;
-        const TV *__restrict__ v = a.x + off + (long long)r * BLK;
-        TV x[BLK];
+    for (int c = 0; c < nc; ++c)
 #pragma unroll
-        for (int e = 0; e < BLK; ++e) x[e] = v[e];
-        TV dq[nc][nd];
+        for (int d = 0; d < nd; ++d) dq[c][d] = dk[c * nd + d];
+    const TV fj = same ? (TV)1 : (TV)2;
+    TV dbd[nb][nd], dbc[nb][nc];
 #pragma unroll
-        for (int c = 0; c < nc; ++c)
+    for (int b = 0; b < nb; ++b) {
 #pragma unroll
-            for (int d = 0; d < nd; ++d) dq[c][d] = a.Dpair[(size_t)q * 9 + c * 3 + d];
-        const TV fj = kp.same ? (TV)1 : (TV)2;
-        // D(b, d) and D(b, c) for the exchange terms
-        TV dbd[nb][nd], dbc[nb][nc];
+        for (int d = 0; d < nd; ++d) dbd[b][d] = s_drow[b * n + aoD + d];
 #pragma unroll
-        for (int b = 0; b < nb; ++b) {
+        for (int c = 0; c < nc; ++c) dbc[b][c] = s_drow[b * n + aoC + c];
+    }
 #pragma unroll
-            for (int d = 0; d < nd; ++d) dbd[b][d] = D[(aoB + b) + n * (aoD + d)];
+    for (int ia = 0; ia < na; ++ia) {
 #pragma unroll
-            for (int c = 0; c < nc; ++c) dbc[b][c] = D[(aoB + b) + n * (aoC + c)];
+        for (int ib = 0; ib < nb; ++ib) {
+            TV js = 0;
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+                for (int d = 0; d < nd; ++d) js += dq[c][d] * x[((ia * nb + ib) * nc + c) * nd + d];
+            J[ia][ib] += fj * js;
         }
+        // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
 #pragma unroll
-        for (int ia = 0; ia < na; ++ia) {
+        for (int c = 0; c < nc; ++c) {
+            TV ks = 0;
 #pragma unroll
-            for (int ib = 0; ib < nb; ++ib) {
-                TV js = 0;
+            for (int ib = 0; ib < nb; ++ib)
 #pragma unroll
-                for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < nd; ++d) ks += dbd[ib][d] * x[((ia * nb + ib) * nc + c) * nd + d];
+            atomicAdd(&s_rows[ia * n + aoC + c], (TV)(-0.5) * ks);
+        }
+        // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
+        if (!same) {
 #pragma unroll
-                    for (int d = 0; d < nd; ++d) js += dq[c][d] * x[((ia * nb + ib) * nc + c) * nd + d];
-                J[ia][ib] += fj * js;
-            }
-            // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
-#pragma unroll
-            for (int c = 0; c < nc; ++c) {
+            for (int d = 0; d < nd; ++d) {
                 TV ks = 0;
 #pragma unroll
                 for (int ib = 0; ib < nb; ++ib)
 #pragma unroll
-                    for (int d = 0; d < nd; ++d) ks += dbd[ib][d] * x[((ia * nb + ib) * nc + c) * nd + d];
-                atomicAdd(&s_rows[ia * n + aoC + c], (TV)(-0.5) * ks);
-            }
-            // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
-            if (!kp.same) {
-#pragma unroll
-                for (int d = 0; d < nd; ++d) {
-                    TV ks = 0;
-#pragma unroll
-                    for (int ib = 0; ib < nb; ++ib)
-#pragma unroll
-                        for (int c = 0; c < nc; ++c) ks += dbc[ib][c] * x[((ia * nb + ib) * nc + c) * nd + d];
-                    atomicAdd(&s_rows[ia * n + aoD + d], (TV)(-0.5) * ks);
-                }
+                    for (int c = 0; c < nc; ++c) ks += dbc[ib][c] * x[((ia * nb + ib) * nc + c) * nd + d];
+                atomicAdd(&s_rows[ia * n + aoD + d], (TV)(-0.5) * ks);
             }
         }
     }
+}
+
+// All kets of the unit (A, B): the workgroup's threads stride over the
+// concatenated ket-class segments.
+template <int LA, int LB, typename TV>
+__device__ __forceinline__ void digest_unit(const DigArgs<TV> &a, TV *s_rows, const TV *s_drow, int u, int aoB) {
+    constexpr int na = ncart(LA), nb = ncart(LB);
+    const int c0 = a.unit[4 * u + 1], c1 = a.unit[4 * u + 2], c2 = a.unit[4 * u + 3];
+    const long long o0 = a.unit_off[3 * u], o1 = a.unit_off[3 * u + 1], o2 = a.unit_off[3 * u + 2];
+    TV J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const int total = c0 + c1 + c2;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        if (k < c0) digest_ket<LA, LB, 0>(a, s_rows, s_drow, k, o0, J);
+        else if (k < c0 + c1) digest_ket<LA, LB, 1>(a, s_rows, s_drow, k - c0, o1, J);
+        else digest_ket<LA, LB, 2>(a, s_rows, s_drow, k - c0 - c1, o2, J);
+    }
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int ia = 0; ia < na; ++ia)
 #pragma unroll
         for (int ib = 0; ib < nb; ++ib) {
             const TV t = wave_sum(J[ia][ib]);
-            if (lane == 0) atomicAdd(&s_rows[ia * n + aoB + ib], t);
+            if (lane == 0) atomicAdd(&s_rows[ia * a.n + aoB + ib], t);
         }
 }
 
-template <int LA, typename TV>
-__device__ __forceinline__ void digest_dispatch(const DigArgs<TV> &a, TV *s_rows, int LB, int ck, int aoB, int cnt,
-                                                long long off, int lane) {
-    if (LB == 0) {
-        if (ck == 0) digest_unit<LA, 0, 0>(a, s_rows, aoB, cnt, off, lane);
-        else if (ck == 1) digest_unit<LA, 0, 1>(a, s_rows, aoB, cnt, off, lane);
-        else digest_unit<LA, 0, 2>(a, s_rows, aoB, cnt, off, lane);
-    } else {
-        if (ck == 0) digest_unit<LA, 1, 0>(a, s_rows, aoB, cnt, off, lane);
-        else if (ck == 1) digest_unit<LA, 1, 1>(a, s_rows, aoB, cnt, off, lane);
-        else digest_unit<LA, 1, 2>(a, s_rows, aoB, cnt, off, lane);
-    }
-}
-
-// One workgroup per block of work units of one row shell A; writes the
-// rows of A it accumulated to its partial slot.
+// One workgroup per block of units of one row shell A; writes the rows of A
+// it accumulated to its partial slot.
 template <typename TV>
 __global__ __launch_bounds__(256) void k_digest(DigArgs<TV> a) {
     extern __shared__ unsigned char smem[];
+    const int n = a.n;
     TV *s_rows = reinterpret_cast<TV *>(smem);
+    TV *s_drow = s_rows + 3 * n;
     const int w = blockIdx.x;
     const int A = a.wg[3 * w], u0 = a.wg[3 * w + 1], u1 = a.wg[3 * w + 2];
     const int LA = a.shell[4 * A + 1];
     const int na = LA ? 3 : 1;
-    const int n = a.n;
     for (int i = threadIdx.x; i < na * n; i += blockDim.x) s_rows[i] = 0;
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int u = u0 + wave; u < u1; u += 4) {
-        const int B = a.unit[3 * u], ck = a.unit[3 * u + 1], cnt = a.unit[3 * u + 2];
-        const long long off = a.unit_off[u];
+    for (int u = u0; u < u1; ++u) {
+        const int B = a.unit[4 * u];
         const int aoB = a.shell[4 * B], LB = a.shell[4 * B + 1];
-        if (LA == 0) digest_dispatch<0>(a, s_rows, LB, ck, aoB, cnt, off, lane);
-        else digest_dispatch<1>(a, s_rows, LB, ck, aoB, cnt, off, lane);
+        const int nb = LB ? 3 : 1;
+        __syncthreads();   // previous unit done with s_drow
+        for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
+            const int b = i / n, j = i - b * n;
+            s_drow[i] = a.D[(aoB + b) + (size_t)n * j];
+        }
+        __syncthreads();
+        if (LA == 0) {
+            if (LB == 0) digest_unit<0, 0>(a, s_rows, s_drow, u, aoB);
+            else digest_unit<0, 1>(a, s_rows, s_drow, u, aoB);
+        } else {
+            if (LB == 0) digest_unit<1, 0>(a, s_rows, s_drow, u, aoB);
+            else digest_unit<1, 1>(a, s_rows, s_drow, u, aoB);
+        }
     }
     __syncthreads();
     TV *dst = a.partial + (size_t)w * 3 * n;
@@ -189,17 +191,16 @@ __global__ void k_gather_rows(const TV *__restrict__ partial, const int *__restr
     Graw[i + (size_t)n * j] = s;
 }
 
-// packed D blocks per pair: Dp[q][c*3+d] = D(ao_a + c, ao_b + d)
+// the kets' density blocks in Schwarz-rank order: dk[r*(nc*nd) + c*nd + d] =
+// D(ao_a + c, ao_b + d) of pair sorted[r]
 template <typename TV>
-__global__ void k_pack_pairs(const ShellPair *__restrict__ pairs, int np, const TV *__restrict__ D, int n,
-                             TV *__restrict__ Dp) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
-    const ShellPair sp = pairs[q];
-    const int nc = sp.la ? 3 : 1, nd = sp.lb ? 3 : 1;
-    for (int c = 0; c < 3; ++c)
-        for (int d = 0; d < 3; ++d)
-            Dp[(size_t)q * 9 + c * 3 + d] = (c < nc && d < nd) ? D[(sp.ao_a + c) + (size_t)n * (sp.ao_b + d)] : (TV)0;
+__global__ void k_pack_kets(const ShellPair *__restrict__ pairs, const int *__restrict__ sorted, int np, int nc, int nd,
+                            const TV *__restrict__ D, int n, TV *__restrict__ dk) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= np) return;
+    const ShellPair sp = pairs[sorted[r]];
+    for (int c = 0; c < nc; ++c)
+        for (int d = 0; d < nd; ++d) dk[(size_t)r * nc * nd + c * nd + d] = D[(sp.ao_a + c) + (size_t)n * (sp.ao_b + d)];
 }
 
 // --- small elementwise kernels ------------------------------------------------
@@ -233,25 +234,34 @@ int ensure_work(myqc_ctx *ctx) {
 }
 
 template <typename TV>
-static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *Dpair, TV *partial, double *Graw) {
-    const int np = (int)ctx->pairs.size();
-    k_pack_pairs<TV><<<(np + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_pairs, np, D, ctx->n, Dpair);
+static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, TV *partial, double *Graw) {
+    const int np0 = (int)ctx->pair_ids[0].size(), np1 = (int)ctx->pair_ids[1].size(),
+              np2 = (int)ctx->pair_ids[2].size();
+    TV *dk0 = dk, *dk1 = dk + np0, *dk2 = dk + np0 + 3 * (size_t)np1;
+    const int nps[3] = {np0, np1, np2};
+    TV *dks[3] = {dk0, dk1, dk2};
+    for (int c = 0; c < 3; ++c)
+        if (nps[c] > 0)
+            k_pack_kets<TV><<<(nps[c] + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_sorted[c], nps[c],
+                                                                            c ? 3 : 1, c == 2 ? 3 : 1, D, ctx->n,
+                                                                            dks[c]);
     if (ctx->n_wg > 0) {
         DigArgs<TV> a;
-        a.pairs = ctx->d_pairs;
-        a.sorted0 = ctx->d_sorted[0];
-        a.sorted1 = ctx->d_sorted[1];
-        a.sorted2 = ctx->d_sorted[2];
+        a.meta0 = ctx->d_meta[0];
+        a.meta1 = ctx->d_meta[1];
+        a.meta2 = ctx->d_meta[2];
+        a.dk0 = dk0;
+        a.dk1 = dk1;
+        a.dk2 = dk2;
         a.wg = ctx->d_wg;
         a.unit = ctx->d_unit;
         a.unit_off = ctx->d_unit_off;
         a.shell = ctx->d_shell;
         a.x = x;
         a.D = D;
-        a.Dpair = Dpair;
         a.partial = partial;
         a.n = ctx->n;
-        const size_t shm = sizeof(TV) * 3 * ctx->n;
+        const size_t shm = sizeof(TV) * 6 * ctx->n;
         k_digest<TV><<<ctx->n_wg, 256, shm, ctx->stream>>>(a);
     }
     k_gather_rows<TV><<<dim3((ctx->n + 127) / 128, ctx->n), 128, 0, ctx->stream>>>(partial, ctx->d_shell,
@@ -267,7 +277,7 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     MYQC_CHECK(ensure_work(ctx));
     const int n = ctx->n;
     const size_t nn = (size_t)n * n;
-    if ((size_t)3 * n * sizeof(double) > 64 * 1024)
+    if ((size_t)6 * n * sizeof(double) > 64 * 1024)
         return set_error(ctx, MYQC_E_ARG, "digestion rows for n=%d exceed 64 KB of LDS", n);
     hipEvent_t ev;
     MYQC_CHECK(timer_begin(ctx, T_JK, &ev));
