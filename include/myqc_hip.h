@@ -108,6 +108,11 @@ int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, doub
                  double e_tol, double d_tol, int max_iter, int precision, double *energy, double *C, double *eig,
                  int *iterations);
 
+/* Diagnostics of the last myqc_rhf_scf: densities obtained by purification
+ * and the iterations that fell back to diagonalisation.  No counterpart in
+ * the reference (it diagonalises every iteration, src/scf/scf.f90:848). */
+int myqc_scf_info(myqc_ctx *ctx, int *purifications, int *diag_fallbacks);
+
 /* Kernel timing accumulated on the context's stream with HIP events since
  * the last reset: which = 0 ERI, 1 J/K digestion, 2 ao2mo, 3 MP2.
  * Returns launches, total milliseconds and algorithmic work (flops or bytes). */

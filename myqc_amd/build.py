@@ -24,9 +24,12 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
+    # the shell-quartet recursions are nested constant-trip loops that must
+    # unroll completely, or their arrays fall back to scratch memory
+    "-mllvm", "-pragma-unroll-threshold=100000",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
-SOURCES = ["capi.hip", "eri.hip", "jk.hip", "one_e.hip", "ao2mo.hip", "mfma_gemm.hip"]
+SOURCES = ["capi.hip", "eri.hip", "jk.hip", "sp2.hip", "one_e.hip", "ao2mo.hip", "mfma_gemm.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -100,6 +100,9 @@ struct myqc_ctx {
     size_t iajb_elems = 0;
     int iajb_nocc = 0, iajb_nv = 0;
 
+    // last RHF run: purification steps and diagonalisation fallbacks
+    int scf_purify = 0, scf_diag = 0;
+
     // timing
     bool timing = false;
     std::vector<myqc::EventRec> events;

@@ -61,6 +61,15 @@ MYQC_HD constexpr int raise(int idx, int d) {
 // the direction used to build a component by recursion: its first nonzero axis
 MYQC_HD constexpr int bdir(int idx) { return cexp(idx, 0) ? 0 : cexp(idx, 1) ? 1 : 2; }
 
+// 1/sqrt(x): the device's reciprocal square root, or its host equivalent
+MYQC_HD double rsqrt_hd(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrt(x);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 // Boys F_0..F_M(T) as the reference computes them at order Q (M <= Q <= 16).
 // ft is the table transposed to ftT[Tk*23 + m].
 template <int M>
@@ -79,8 +88,11 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
         }
         double e = exp(-T);
         double t2 = 2.0 * T;
-        // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160)
-        for (int j = Q - 1; j >= M; --j) fq = (t2 * fq + e) / (2.0 * j + 1.0);
+        // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160),
+        // with 1/(2j+1) as constants; the reference divides (rounding-level difference)
+#pragma unroll
+        for (int j = 15; j >= 0; --j)
+            if (j >= M && j < Q) fq = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
         // fq now holds F_top, top = min(Q, M); the reference never forms F_j for
         // j > Q, and callers only read F_j for j <= Q
         const int top = Q < M ? Q : M;
@@ -88,18 +100,18 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
         for (int j = M; j >= 0; --j) {
             if (j > top) F[j] = 0.0;
             else if (j == top) F[j] = fq;
-            else F[j] = (t2 * F[j + 1] + e) / (2.0 * j + 1.0);
+            else F[j] = (t2 * F[j + 1] + e) * (1.0 / (2.0 * j + 1.0));
         }
     } else {
         const double half_sqrt_pi = 0.5 * sqrt(PI_SP);
-        double rt = 1.0 / sqrt(T);
-        double it2 = 1.0 / (2.0 * T);
+        const double rt = rsqrt_hd(T);
+        const double iT = rt * rt;
+        const double it2 = 0.5 * iT;
         if (T < 2.0 * Q + 36.0) {
             // auxilary.f90:181-187 with BoysG (:271-283); beyond T = 30 the
             // reference's g is undefined and 0.49 is used (changes F_0 by
             // at most 3e-15 |g|)
             double g;
-            double iT = 1.0 / T;
             if (T < 15.0) g = 0.4999489092 - 0.2473631686 * iT + 0.321180909 * iT * iT - 0.3811559346 * iT * iT * iT;
             else if (T < 18.0) g = 0.4998436875 - 0.24249438 * iT + 0.24642845 * iT * iT;
             else if (T < 24.0) g = 0.499093162 - 0.2152832 * iT;
@@ -124,6 +136,7 @@ struct PrimPair {
     double K;        // EIJ * c_a * c_b  (normalised contraction weights folded in)
     double EIJ;      // exp(-alpha beta |AB|^2 / p): the reference's pair screen factor
     double beta;     // exponent of the second shell's primitive (kinetic energy)
+    double ip;       // 1 / p
 };
 
 // Contracted shell pair.
@@ -172,6 +185,7 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
             const double p = b.p, q = k.p, pq = p + q;
             const double ipq = 1.0 / pq;
             const double rho = p * q * ipq;
+            const double rsq = rsqrt_hd(pq);
             double PQ[3], WP[3], WQ[3], QC[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -182,7 +196,7 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
                 QC[d] = k.PA[d];
             }
             const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-            const double pref = c0 / (p * q * sqrt(pq)) * b.K * k.K;
+            const double pref = c0 * b.ip * k.ip * rsq * b.K * k.K;
             double F[M + 1];
             boys_ref<M>(Q, T, ftT, F);
             // bra vertical recursion [e0|00]^(m)
@@ -190,7 +204,7 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
 #pragma unroll
             for (int m = 0; m <= M; ++m) Vb[0][m] = pref * F[m];
             if constexpr (EB > 0) {
-                const double i2p = 0.5 / p, rp = rho / p;
+                const double i2p = 0.5 * b.ip, rp = q * ipq;
 #pragma unroll
                 for (int idx = 1; idx < NB; ++idx) {
                     const int d = bdir(idx), lo = lower(idx, d);
@@ -212,7 +226,7 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
                 for (int e = A::E0; e < NB; ++e) acc.v[e - A::E0][0] += Vb[e][0];
             } else {
                 constexpr int NK = nupto(EK);
-                const double i2q = 0.5 / q, rq = rho / q, i2pq = 0.5 * ipq;
+                const double i2q = 0.5 * k.ip, rq = p * ipq, i2pq = 0.5 * ipq;
                 double W[NB][NK][M + 1];
 #pragma unroll
                 for (int e = 0; e < NB; ++e)

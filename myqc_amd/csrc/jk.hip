@@ -22,6 +22,9 @@
 
 namespace myqc {
 
+int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, int n, int nocc,
+               int *ok);
+
 template <int C> struct PL;
 template <> struct PL<0> { static constexpr int a = 0, b = 0; };
 template <> struct PL<1> { static constexpr int a = 1, b = 0; };
@@ -328,12 +331,13 @@ __global__ void k_sub(const double *__restrict__ X, const double *__restrict__ Y
 struct ScfBuffers {
     double *H = nullptr, *S = nullptr, *X = nullptr, *Xi = nullptr, *F = nullptr, *Fo = nullptr, *Fx = nullptr,
            *D = nullptr, *Dn = nullptr, *Do = nullptr, *Don = nullptr, *Dref = nullptr, *G = nullptr, *C = nullptr,
-           *T1 = nullptr, *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr;
+           *T1 = nullptr, *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr,
+           *lohi = nullptr;
     double *hF = nullptr, *hE = nullptr;   // DIIS history, m x n^2 each
     int *info = nullptr;
     void release() {
         double **ps[] = {&H, &S, &X, &Xi, &F, &Fo, &Fx, &D, &Dn, &Do, &Don, &Dref, &G, &C, &T1, &T2, &P2, &eig, &work,
-                         &dsc, &hF, &hE};
+                         &dsc, &lohi, &hF, &hE};
         for (auto p : ps) dfree((void **)p);
         dfree((void **)&info);
     }
@@ -377,81 +381,6 @@ static int to_ao_density(myqc_ctx *ctx, ScfBuffers &b, const double *Do, double 
     return 0;
 }
 
-// Spectral bounds of a symmetric matrix by Gershgorin discs: out[0] = min, out[1] = max
-__global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ out) {
-    __shared__ double smin[1024], smax[1024];
-    double lo = 1e300, hi = -1e300;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double r = 0.0;
-        for (int j = 0; j < n; ++j)
-            if (j != i) r += fabs(F[j + (size_t)n * i]);
-        const double c = F[i + (size_t)n * i];
-        lo = fmin(lo, c - r);
-        hi = fmax(hi, c + r);
-    }
-    smin[threadIdx.x] = lo;
-    smax[threadIdx.x] = hi;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
-            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        out[0] = smin[0];
-        out[1] = smax[0];
-    }
-}
-
-// P = (emax I - F) / (emax - emin): occupied levels map toward 1
-__global__ void k_sp2_init(const double *__restrict__ F, const double *__restrict__ bounds, double *__restrict__ P,
-                           int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    const double lo = bounds[0], hi = bounds[1];
-    const double s = 1.0 / (hi - lo);
-    P[i + (size_t)n * j] = ((i == j ? hi : 0.0) - F[i + (size_t)n * j]) * s;
-}
-
-// tr[0] = tr P, tr[1] = tr P^2
-__global__ __launch_bounds__(1024) void k_traces(const double *__restrict__ P, const double *__restrict__ P2, int n,
-                                                 double *__restrict__ tr) {
-    __shared__ double s0[1024], s1[1024];
-    double a = 0.0, c = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        a += P[i + (size_t)n * i];
-        c += P2[i + (size_t)n * i];
-    }
-    s0[threadIdx.x] = a;
-    s1[threadIdx.x] = c;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            s0[threadIdx.x] += s0[threadIdx.x + w];
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        tr[0] = s0[0];
-        tr[1] = s1[0];
-    }
-}
-
-// one trace-correcting purification step (Niklasson, Phys. Rev. B 66,
-// 155115 (2002)): P <- P^2 or 2P - P^2, whichever trace lands nearer nocc
-__global__ void k_sp2_step(double *__restrict__ P, const double *__restrict__ P2, const double *__restrict__ tr, int n,
-                           double nocc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    const double t1 = tr[0], t2 = tr[1];
-    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
-    const size_t e = i + (size_t)n * j;
-    P[e] = square ? P2[e] : 2.0 * P[e] - P2[e];
-}
-
 // Orthonormal-basis density projector onto the nocc lowest levels of Fo by
 // purification; falls back to diagonalisation if it does not converge.
 static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int n, int nocc) {
@@ -459,28 +388,12 @@ static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int
         MYQC_HIP(ctx, hipMemsetAsync(P, 0, sizeof(double) * n * n, ctx->stream));
         return 0;
     }
-    k_gershgorin<<<1, 1024, 0, ctx->stream>>>(Fo, n, b.dsc);
-    k_sp2_init<<<grid2(n), 128, 0, ctx->stream>>>(Fo, b.dsc, P, n);
-    double tr[2] = {0.0, 0.0};
-    bool ok = false;
-    int extra = -1;
-    for (int it = 0; it < 120; ++it) {
-        MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, P, n, P, n, 0.0, b.P2, n));
-        k_traces<<<1, 1024, 0, ctx->stream>>>(P, b.P2, n, b.dsc + 2);
-        k_sp2_step<<<grid2(n), 128, 0, ctx->stream>>>(P, b.P2, b.dsc + 2, n, (double)nocc);
-        if (extra > 0 && --extra == 0) {
-            ok = true;
-            break;
-        }
-        if (extra < 0 && it >= 12 && (it & 3) == 3) {
-            MYQC_HIP(ctx, d2h(ctx, tr, b.dsc + 2, sizeof tr));
-            // tr P - tr P^2 = sum l(1-l) over the eigenvalues l of P; near
-            // convergence each step squares it, so two more steps finish
-            if (fabs(tr[0] - tr[1]) < 1e-8 && fabs(tr[0] - nocc) < 1e-6) extra = 2;
-        }
-    }
+    int ok = 0;
+    MYQC_CHECK(sp2_purify(ctx, Fo, P, b.P2, b.lohi, b.dsc, n, nocc, &ok));
+    ++ctx->scf_purify;
     if (!ok) {
         // no clean gap at this step: occupy the nocc lowest eigenvectors
+        ++ctx->scf_diag;
         MYQC_CHECK(diag_orth(ctx, b, Fo, n));
         MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, 1.0, b.T2, n, b.T2, n, 0.0,
                         P, n));
@@ -569,6 +482,7 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
     TRY(dalloc(ctx, (void **)&b.eig, sizeof(double) * n));
     TRY(dalloc(ctx, (void **)&b.work, sizeof(double) * n));
     TRY(dalloc(ctx, (void **)&b.dsc, sizeof(double) * 8));
+    TRY(dalloc(ctx, (void **)&b.lohi, sizeof(double) * 2 * n));
     TRY(dalloc(ctx, (void **)&b.hF, sizeof(double) * nn * MD));
     TRY(dalloc(ctx, (void **)&b.hE, sizeof(double) * nn * MD));
     TRY(dalloc(ctx, (void **)&b.info, sizeof(int)));
@@ -732,6 +646,13 @@ extern "C" int myqc_density_rhf(myqc_ctx *ctx, const double *C, int nocc, double
     return rc;
 }
 
+extern "C" int myqc_scf_info(myqc_ctx *ctx, int *purifications, int *diag_fallbacks) {
+    if (!ctx) return MYQC_E_ARG;
+    if (purifications) *purifications = ctx->scf_purify;
+    if (diag_fallbacks) *diag_fallbacks = ctx->scf_diag;
+    return 0;
+}
+
 extern "C" int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
                             const double *C_guess, double e_tol, double d_tol, int max_iter, int precision,
                             double *energy, double *C, double *eig, int *iterations) {
@@ -743,5 +664,6 @@ extern "C" int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int
         return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf: precision %d", precision);
     if (!ctx->have_eri || (precision == MYQC_FP32 && !ctx->have_eri32))
         return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf: call myqc_eri_build first");
+    ctx->scf_purify = ctx->scf_diag = 0;
     return rhf_scf(ctx, H, S, nocc, e_nuc, C_guess, e_tol, d_tol, max_iter, precision, energy, C, eig, iterations);
 }

@@ -126,6 +126,7 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
                     pp.EIJ = exp(-aa * bb * ab2 / p);       // int2e.f90:223
                     pp.K = pp.EIJ * sa.coef[i] * sb.coef[j];
                     pp.beta = bb;
+                    pp.ip = 1.0 / p;
                     prims.push_back(pp);
                 }
             sp.npp = (int)(sa.alpha.size() * sb.alpha.size());

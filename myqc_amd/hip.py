@@ -40,6 +40,7 @@ EXPORTS = {
     "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
     "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
                       ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
+    "myqc_scf_info": ([_vp, _ip, _ip], ctypes.c_int),
     "myqc_timer_reset": ([_vp, ctypes.c_int], ctypes.c_int),
     "myqc_timer_get": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp, _dp], ctypes.c_int),
 }
@@ -211,7 +212,10 @@ class Context:
         self._chk(self.L.myqc_rhf_scf(self.h, _p(Hf), _p(Sf), nocc, e_nuc, _p(Cg) if Cg is not None else None,
                                       e_tol, d_tol, max_iter, precision, ctypes.byref(E), _p(C), _p(eig),
                                       ctypes.byref(it)))
-        return dict(E=E.value, C=C, eig=eig, iterations=it.value)
+        npur, ndiag = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.L.myqc_scf_info(self.h, ctypes.byref(npur), ctypes.byref(ndiag)))
+        return dict(E=E.value, C=C, eig=eig, iterations=it.value, purifications=npur.value,
+                    diag_fallbacks=ndiag.value)
 
     # -- timing -------------------------------------------------------------
     def timer_reset(self, enable: bool = True):

@@ -1,0 +1,36 @@
+// mfma_probe.hip -- TEST INFRASTRUCTURE: one v_mfma_f64_16x16x4f64 on
+// caller-given 16x4 A and 4x16 B, every lane's inputs and outputs recorded,
+// so a test can check the lane layout the SP2 kernel assumes.  All accesses
+// are within the 64-lane, 16x16 arrays below.
+#include <hip/hip_runtime.h>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_probe(const double *A, const double *B, double *raw) {
+    const int l = threadIdx.x;
+    // assumed: lane l feeds A[l%16][l/16] and B[l/16][l%16]  (A row-major 16x4, B row-major 4x16)
+    const double a = A[(l % 16) * 4 + l / 16];
+    const double b = B[(l / 16) * 16 + l % 16];
+    d4_t c = {0.0, 0.0, 0.0, 0.0};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) raw[l * 4 + v] = c[v];
+}
+
+extern "C" int mfma_probe(const double *A, const double *B, double *raw) {
+    double *dA = nullptr, *dB = nullptr, *dR = nullptr;
+    if (hipMalloc(&dA, 64 * 8) != hipSuccess || hipMalloc(&dB, 64 * 8) != hipSuccess ||
+        hipMalloc(&dR, 256 * 8) != hipSuccess)
+        return -1;
+    int rc = 0;
+    if (hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = -2;
+    if (rc == 0) {
+        k_probe<<<1, 64>>>(dA, dB, dR);
+        if (hipMemcpy(raw, dR, 256 * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = -3;
+    }
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dR);
+    return rc;
+}

@@ -70,6 +70,8 @@ def test_rhf_energy_matches_reference(ctx, name):
     S, H = ctx.int1e()
     r = ctx.rhf(H, S, int(f["nocc"]), enuc(Z, f["xyz"]), e_tol=1e-10, d_tol=1e-8)
     assert abs(r["E"] - float(f["E_scf"])) <= 1e-8
+    # densities came from MFMA purification, not the diagonalisation fallback
+    assert r["purifications"] >= r["iterations"] and r["diag_fallbacks"] == 0
     # orbital energies up to rounding of the converged Fock matrix
     assert np.max(np.abs(r["eig"] - f["eig"])) <= 1e-6
 
