@@ -25,6 +25,8 @@ struct QuartetClass {
     long long *d_cum = nullptr;   // nbra + 1 prefix sums of counts
 };
 
+struct ScfBuffers;
+
 enum TimerCat { T_ERI = 0, T_JK = 1, T_AO2MO = 2, T_MP2 = 3, T_NCAT = 4 };
 
 struct EventRec {
@@ -101,7 +103,9 @@ struct myqc_ctx {
     int iajb_nocc = 0, iajb_nv = 0;
 
     // last RHF run: purification steps and diagonalisation fallbacks
+    myqc::ScfBuffers *scf = nullptr;       // RHF work arrays, kept across calls
     int scf_purify = 0, scf_diag = 0;
+    int scf_sp2_steps = 0;                // steps the last purification took
 
     // timing
     bool timing = false;
@@ -155,5 +159,6 @@ int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX);
 int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision);
 int ensure_work(myqc_ctx *ctx);
 void free_eri(myqc_ctx *ctx);
+void free_scf(myqc_ctx *ctx);
 
 }  // namespace myqc

@@ -1,17 +1,21 @@
-// jk.hip -- J/K (Fock G) digestion of the stored unique integrals, the RHF
-// density, and the in-process RHF driver.
+// jk.hip -- J/K (Fock G) digestion of the stored integrals.
 //
-// Digestion: every stored value is v' = (ij|kl) times its degeneracy weight
-// (0 for repeats inside a block, 1/2 per index coincidence), so one pass over
-// the unique integrals with
-//     A_ij += 2 D_kl v'   A_kl += 2 D_ij v'
-//     A_ik -= D_jl v'/2   A_jk -= D_il v'/2   A_il -= D_jk v'/2   A_jl -= D_ik v'/2
-// and G = A + A^T equals the reference's
+// The reference's two-electron part of the Fock matrix is
 //     G(i,j) = sum_kl D(k,l)[XX(i,j,k,l) - XX(i,k,j,l)/4 - XX(i,l,j,k)/4]
-// (src/I2G/RHFI2G.f90:80-90) for the symmetrised D (the reference's formula
-// only sees D through D + D^T).  One thread per shell quartet accumulates its
-// six J/K blocks in registers and adds them to A with hardware atomics; A and
-// D (n^2 each) stay L2-resident while the integrals stream from HBM.
+// (src/I2G/RHFI2G.f90:80-90), which only sees D through D + D^T, so it is
+// evaluated for the symmetrised D.  Each unique shell quartet (AB|CD) is
+// stored once per ordered bra shell pair (A,B) that owns rows of G -- up to
+// four copies -- in segments of kets sorted by Schwarz rank.  Row ownership
+// turns every update into a row-local one:
+//     J: G(i,j) += f D(k,l) (ij|kl),              f = 2 unless C == D
+//     K: G(i,k) -= D(j,l) (ij|kl) / 2,  and if C != D  G(i,l) -= D(j,k) (ij|kl) / 2
+// for i in A, j in B.  One workgroup owns the rows of one shell A for a run
+// of units (A, B, ket class): it stages D's rows of B in LDS, streams the
+// unit's integrals once from HBM (the dominant traffic), reduces J across the
+// wavefront and adds K into LDS rows of A, then writes those rows to its own
+// partial slot.  A gather sums the partial rows of each shell in a fixed
+// order and G = (A + A^T)/2 restores symmetry: no global atomics, and
+// bitwise-reproducible results.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -21,9 +25,6 @@
 #include "context.h"
 
 namespace myqc {
-
-int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, int n, int nocc,
-               int *ok);
 
 template <int C> struct PL;
 template <> struct PL<0> { static constexpr int a = 0, b = 0; };
@@ -208,7 +209,10 @@ __global__ void k_pack_kets(const ShellPair *__restrict__ pairs, const int *__re
 
 // --- small elementwise kernels ------------------------------------------------
 
-__global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, size_t cnt);
+__global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, size_t cnt) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+        Z[i] = X[i] + Y[i];
+}
 
 __global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
@@ -301,308 +305,6 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     return 0;
 }
 
-// ---------------------------------------------------------------------------
-// RHF driver
-// ---------------------------------------------------------------------------
-
-// U(:, j) *= s_j^power
-__global__ void k_scale_cols(double *__restrict__ U, const double *__restrict__ s, int n, double power) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    U[i + (size_t)n * j] *= pow(s[j], power);
-}
-
-__global__ void k_antisym(const double *__restrict__ T, double *__restrict__ E, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    E[i + (size_t)n * j] = T[i + (size_t)n * j] - T[j + (size_t)n * i];
-}
-
-__global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, size_t cnt) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
-        Z[i] = X[i] + Y[i];
-}
-
-__global__ void k_sub(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, size_t cnt) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
-        Z[i] = X[i] - Y[i];
-}
-
-struct ScfBuffers {
-    double *H = nullptr, *S = nullptr, *X = nullptr, *Xi = nullptr, *F = nullptr, *Fo = nullptr, *Fx = nullptr,
-           *D = nullptr, *Dn = nullptr, *Do = nullptr, *Don = nullptr, *Dref = nullptr, *G = nullptr, *C = nullptr,
-           *T1 = nullptr, *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr,
-           *lohi = nullptr;
-    double *hF = nullptr, *hE = nullptr;   // DIIS history, m x n^2 each
-    int *info = nullptr;
-    void release() {
-        double **ps[] = {&H, &S, &X, &Xi, &F, &Fo, &Fx, &D, &Dn, &Do, &Don, &Dref, &G, &C, &T1, &T2, &P2, &eig, &work,
-                         &dsc, &lohi, &hF, &hE};
-        for (auto p : ps) dfree((void **)p);
-        dfree((void **)&info);
-    }
-};
-
-static int gemm(myqc_ctx *ctx, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha,
-                const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc) {
-    MYQC_BLAS(ctx, rocblas_dgemm(ctx->blas, ta, tb, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc));
-    return 0;
-}
-
-// eigen-decomposition of the symmetric n x n A (overwritten by eigenvectors)
-static int syevd(myqc_ctx *ctx, ScfBuffers &b, double *A, double *w, int n) {
-    MYQC_BLAS(ctx, rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower, n, A, n, w, b.work, b.info));
-    int info = 0;
-    MYQC_HIP(ctx, hipMemcpyAsync(&info, b.info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (info != 0) return set_error(ctx, MYQC_E_HIP, "rocsolver_dsyevd info=%d", info);
-    return 0;
-}
-
-// C = X C' and orbital energies from the orthonormal-basis Fock matrix Fo
-static int diag_orth(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, int n) {
-    MYQC_HIP(ctx, hipMemcpyAsync(b.T2, Fo, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
-    MYQC_CHECK(syevd(ctx, b, b.T2, b.eig, n));
-    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.X, n, b.T2, n, 0.0, b.C, n));
-    return 0;
-}
-
-// Fo = X^T F X
-static int to_orth(myqc_ctx *ctx, ScfBuffers &b, const double *F, double *Fo, int n) {
-    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, F, n, b.X, n, 0.0, b.T1, n));
-    MYQC_CHECK(gemm(ctx, rocblas_operation_transpose, rocblas_operation_none, n, n, n, 1.0, b.X, n, b.T1, n, 0.0, Fo, n));
-    return 0;
-}
-
-// D = 2 X Do X^T
-static int to_ao_density(myqc_ctx *ctx, ScfBuffers &b, const double *Do, double *D, int n) {
-    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, n, 1.0, Do, n, b.X, n, 0.0, b.T1, n));
-    MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 2.0, b.X, n, b.T1, n, 0.0, D, n));
-    return 0;
-}
-
-// Orthonormal-basis density projector onto the nocc lowest levels of Fo by
-// purification; falls back to diagonalisation if it does not converge.
-static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int n, int nocc) {
-    if (nocc == 0) {
-        MYQC_HIP(ctx, hipMemsetAsync(P, 0, sizeof(double) * n * n, ctx->stream));
-        return 0;
-    }
-    int ok = 0;
-    MYQC_CHECK(sp2_purify(ctx, Fo, P, b.P2, b.lohi, b.dsc, n, nocc, &ok));
-    ++ctx->scf_purify;
-    if (!ok) {
-        // no clean gap at this step: occupy the nocc lowest eigenvectors
-        ++ctx->scf_diag;
-        MYQC_CHECK(diag_orth(ctx, b, Fo, n));
-        MYQC_CHECK(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, 1.0, b.T2, n, b.T2, n, 0.0,
-                        P, n));
-    }
-    return 0;
-}
-
-static int density(myqc_ctx *ctx, const double *C, int nocc, double *D, int n) {
-    if (nocc == 0) {
-        MYQC_HIP(ctx, hipMemsetAsync(D, 0, sizeof(double) * n * n, ctx->stream));
-        return 0;
-    }
-    return gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, 2.0, C, n, C, n, 0.0, D, n);
-}
-
-static double ddot(myqc_ctx *ctx, int cnt, const double *x, const double *y, int *rc) {
-    double r = 0.0;
-    rocblas_status s = rocblas_ddot(ctx->blas, cnt, x, 1, y, 1, &r);
-    if (s != rocblas_status_success) *rc = set_error(ctx, MYQC_E_HIP, "rocblas_ddot failed (%d)", (int)s);
-    return r;
-}
-
-static double maxabs_diff(myqc_ctx *ctx, ScfBuffers &b, const double *X, const double *Y, size_t cnt, int *rc) {
-    k_sub<<<256, 256, 0, ctx->stream>>>(X, Y, b.T1, cnt);
-    rocblas_int idx = 0;
-    rocblas_status s = rocblas_idamax(ctx->blas, (int)cnt, b.T1, 1, &idx);
-    if (s != rocblas_status_success) {
-        *rc = set_error(ctx, MYQC_E_HIP, "rocblas_idamax failed");
-        return 0.0;
-    }
-    double v = 0.0;
-    d2h(ctx, &v, b.T1 + (idx - 1), sizeof(double));
-    return fabs(v);
-}
-
-// solve the DIIS system on the host (m <= 8), Gaussian elimination with pivoting
-static bool diis_coeffs(const std::vector<double> &B, int m, std::vector<double> &c) {
-    const int N = m + 1;
-    std::vector<double> M((size_t)N * (N + 1), 0.0);
-    for (int i = 0; i < m; ++i) {
-        for (int j = 0; j < m; ++j) M[i * (N + 1) + j] = B[i * m + j];
-        M[i * (N + 1) + m] = -1.0;
-        M[i * (N + 1) + N] = 0.0;
-    }
-    for (int j = 0; j < m; ++j) M[m * (N + 1) + j] = -1.0;
-    M[m * (N + 1) + m] = 0.0;
-    M[m * (N + 1) + N] = -1.0;
-    for (int col = 0; col < N; ++col) {
-        int piv = col;
-        for (int r = col + 1; r < N; ++r)
-            if (fabs(M[r * (N + 1) + col]) > fabs(M[piv * (N + 1) + col])) piv = r;
-        if (fabs(M[piv * (N + 1) + col]) < 1e-300) return false;
-        if (piv != col)
-            for (int k = 0; k <= N; ++k) std::swap(M[col * (N + 1) + k], M[piv * (N + 1) + k]);
-        for (int r = 0; r < N; ++r) {
-            if (r == col) continue;
-            const double f = M[r * (N + 1) + col] / M[col * (N + 1) + col];
-            for (int k = col; k <= N; ++k) M[r * (N + 1) + k] -= f * M[col * (N + 1) + k];
-        }
-    }
-    c.resize(m);
-    for (int i = 0; i < m; ++i) c[i] = M[i * (N + 1) + N] / M[i * (N + 1) + i];
-    return true;
-}
-
-static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc, const double *C_guess,
-                   double e_tol, double d_tol, int max_iter, int precision, double *energy, double *C_out,
-                   double *eig_out, int *iters_out) {
-    const int n = ctx->n;
-    const size_t nn = (size_t)n * n;
-    const int MD = 8;
-    ScfBuffers b;
-    int rc = 0;
-    auto fail = [&](int code) {
-        b.release();
-        return code;
-    };
-#define TRY(x)                   \
-    do {                         \
-        rc = (x);                \
-        if (rc) return fail(rc); \
-    } while (0)
-    double **mats[] = {&b.H, &b.S, &b.X, &b.Xi, &b.F, &b.Fo, &b.Fx, &b.D, &b.Dn, &b.Do, &b.Don, &b.Dref, &b.G, &b.C,
-                       &b.T1, &b.T2, &b.P2};
-    for (auto m : mats) TRY(dalloc(ctx, (void **)m, sizeof(double) * nn));
-    TRY(dalloc(ctx, (void **)&b.eig, sizeof(double) * n));
-    TRY(dalloc(ctx, (void **)&b.work, sizeof(double) * n));
-    TRY(dalloc(ctx, (void **)&b.dsc, sizeof(double) * 8));
-    TRY(dalloc(ctx, (void **)&b.lohi, sizeof(double) * 2 * n));
-    TRY(dalloc(ctx, (void **)&b.hF, sizeof(double) * nn * MD));
-    TRY(dalloc(ctx, (void **)&b.hE, sizeof(double) * nn * MD));
-    TRY(dalloc(ctx, (void **)&b.info, sizeof(int)));
-    if (hipMemcpyAsync(b.H, H, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(b.S, S, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-        return fail(set_error(ctx, MYQC_E_HIP, "rhf: upload failed"));
-    // X = S^{-1/2}, Xi = S^{1/2} (symmetric orthogonalisation)
-    if (hipMemcpyAsync(b.T2, b.S, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
-        return fail(set_error(ctx, MYQC_E_HIP, "rhf: copy failed"));
-    TRY(syevd(ctx, b, b.T2, b.eig, n));
-    {
-        std::vector<double> s(n);
-        d2h(ctx, s.data(), b.eig, sizeof(double) * n);
-        if (s[0] <= 1e-10) return fail(set_error(ctx, MYQC_E_ARG, "rhf: overlap matrix singular (min eigenvalue %g)", s[0]));
-    }
-    hipMemcpyAsync(b.T1, b.T2, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
-    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n, -0.5);
-    TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, n, 1.0, b.T1, n, b.T2, n, 0.0, b.X, n));
-    hipMemcpyAsync(b.T1, b.T2, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
-    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n, 0.5);
-    TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_transpose, n, n, n, 1.0, b.T1, n, b.T2, n, 0.0, b.Xi, n));
-    // guess density
-    if (C_guess) {
-        if (hipMemcpyAsync(b.C, C_guess, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-            return fail(set_error(ctx, MYQC_E_HIP, "rhf: upload failed"));
-        TRY(density(ctx, b.C, nocc, b.D, n));
-        // Do = S^{1/2} D S^{1/2} / 2
-        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, b.D, n, b.Xi, n, 0.0, b.T1, n));
-        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 0.5, b.Xi, n, b.T1, n, 0.0, b.Do, n));
-    } else {
-        // core Hamiltonian guess (scf.f90:410-512)
-        TRY(to_orth(ctx, b, b.H, b.Fo, n));
-        TRY(purify(ctx, b, b.Fo, b.Do, n, nocc));
-        TRY(to_ao_density(ctx, b, b.Do, b.D, n));
-    }
-    std::vector<double> Bm((size_t)MD * MD, 0.0);
-    int hist = 0, head = 0;
-    double E = 0.0, E_old = 0.0;
-    int it = 0;
-    bool conv = false;
-    // mixed precision: incremental FP32 builds F_k = F_{k-1} + G32(D_k - D_{k-1})
-    // from an FP64 reference build; FP64 rebuild every 8 steps and at the end
-    const bool incr = (precision == MYQC_FP32);
-    int since_full = 0;
-    bool final_fp64 = false;
-    for (it = 1; it <= max_iter; ++it) {
-        if (!incr || final_fp64 || since_full == 0 || since_full >= 8) {
-            TRY(digest_G(ctx, b.D, b.G, MYQC_FP64));
-            since_full = 1;
-        } else {
-            k_sub<<<256, 256, 0, ctx->stream>>>(b.D, b.Dref, b.T1, nn);
-            TRY(digest_G(ctx, b.T1, b.G, MYQC_FP32));   // G += G32(dD)
-            ++since_full;
-        }
-        hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream);
-        k_add<<<256, 256, 0, ctx->stream>>>(b.H, b.G, b.F, nn);
-        E = 0.5 * (ddot(ctx, (int)nn, b.D, b.F, &rc) + ddot(ctx, (int)nn, b.D, b.H, &rc)) + e_nuc;
-        if (rc) return fail(rc);
-        // orthonormal basis: Fo = X^T F X, DIIS error Fo Do - Do Fo
-        double *fslot = b.hF + nn * head, *eslot = b.hE + nn * head;
-        TRY(to_orth(ctx, b, b.F, fslot, n));
-        TRY(gemm(ctx, rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, fslot, n, b.Do, n, 0.0, b.T2, n));
-        k_antisym<<<grid2(n), 128, 0, ctx->stream>>>(b.T2, eslot, n);
-        if (hist < MD) ++hist;
-        for (int j = 0; j < hist; ++j) {
-            const double v = ddot(ctx, (int)nn, eslot, b.hE + nn * j, &rc);
-            Bm[head * MD + j] = v;
-            Bm[j * MD + head] = v;
-        }
-        if (rc) return fail(rc);
-        const double err = sqrt(fabs(Bm[head * MD + head]));
-        const double *Fuse = fslot;
-        if (hist >= 2) {
-            std::vector<double> Bs((size_t)hist * hist), c;
-            for (int i = 0; i < hist; ++i)
-                for (int j = 0; j < hist; ++j) Bs[i * hist + j] = Bm[i * MD + j];
-            if (diis_coeffs(Bs, hist, c)) {
-                MYQC_HIP(ctx, hipMemsetAsync(b.Fx, 0, sizeof(double) * nn, ctx->stream));
-                for (int i = 0; i < hist; ++i)
-                    MYQC_BLAS(ctx, rocblas_daxpy(ctx->blas, (int)nn, &c[i], b.hF + nn * i, 1, b.Fx, 1));
-                Fuse = b.Fx;
-            }
-        }
-        head = (head + 1) % MD;
-        const double dE = fabs(E - E_old);
-        E_old = E;
-        TRY(purify(ctx, b, Fuse, b.Don, n, nocc));
-        TRY(to_ao_density(ctx, b, b.Don, b.Dn, n));
-        const double dD = maxabs_diff(ctx, b, b.Dn, b.D, nn, &rc);
-        if (rc) return fail(rc);
-        std::swap(b.D, b.Dn);
-        std::swap(b.Do, b.Don);
-        if (it > 1 && dE < e_tol && dD < d_tol && err < 100 * d_tol) {
-            if (incr && !final_fp64) {
-                final_fp64 = true;   // confirm with FP64 builds
-                continue;
-            }
-            conv = true;
-            break;
-        }
-    }
-    // report E[D] with the orbitals of its plain Fock matrix (the reference
-    // prints E of the density it diagonalises, scf.f90:775-848)
-    TRY(digest_G(ctx, b.D, b.G, MYQC_FP64));
-    k_add<<<256, 256, 0, ctx->stream>>>(b.H, b.G, b.F, nn);
-    E = 0.5 * (ddot(ctx, (int)nn, b.D, b.F, &rc) + ddot(ctx, (int)nn, b.D, b.H, &rc)) + e_nuc;
-    if (rc) return fail(rc);
-    TRY(to_orth(ctx, b, b.F, b.Fo, n));
-    TRY(diag_orth(ctx, b, b.Fo, n));
-    if (energy) *energy = E;
-    if (C_out) d2h(ctx, C_out, b.C, sizeof(double) * nn);
-    if (eig_out) d2h(ctx, eig_out, b.eig, sizeof(double) * n);
-    if (iters_out) *iters_out = it > max_iter ? max_iter : it;
-    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    b.release();
-#undef TRY
-    if (!conv) return set_error(ctx, MYQC_E_NOCONV, "RHF did not converge in %d iterations", max_iter);
-    return 0;
-}
-
 }  // namespace myqc
 
 using namespace myqc;
@@ -625,45 +327,4 @@ extern "C" int myqc_fock_G(myqc_ctx *ctx, const double *D, double *G, int precis
         rc = set_error(ctx, MYQC_E_HIP, "myqc_fock_G: copy failed");
     dfree((void **)&d_out);
     return rc;
-}
-
-extern "C" int myqc_density_rhf(myqc_ctx *ctx, const double *C, int nocc, double *D) {
-    if (!ctx || !C || !D) return MYQC_E_ARG;
-    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_density_rhf: no system set");
-    const int n = ctx->n;
-    if (nocc < 0 || nocc > n) return set_error(ctx, MYQC_E_ARG, "myqc_density_rhf: nocc=%d", nocc);
-    const size_t nn = (size_t)n * n;
-    double *dC = nullptr, *dD = nullptr;
-    MYQC_CHECK(dalloc(ctx, (void **)&dC, sizeof(double) * nn));
-    int rc = dalloc(ctx, (void **)&dD, sizeof(double) * nn);
-    if (rc == 0 && hipMemcpyAsync(dC, C, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-        rc = set_error(ctx, MYQC_E_HIP, "upload failed");
-    if (rc == 0) rc = density(ctx, dC, nocc, dD, n);
-    if (rc == 0 && d2h(ctx, D, dD, sizeof(double) * nn) != hipSuccess)
-        rc = set_error(ctx, MYQC_E_HIP, "download failed");
-    dfree((void **)&dC);
-    dfree((void **)&dD);
-    return rc;
-}
-
-extern "C" int myqc_scf_info(myqc_ctx *ctx, int *purifications, int *diag_fallbacks) {
-    if (!ctx) return MYQC_E_ARG;
-    if (purifications) *purifications = ctx->scf_purify;
-    if (diag_fallbacks) *diag_fallbacks = ctx->scf_diag;
-    return 0;
-}
-
-extern "C" int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
-                            const double *C_guess, double e_tol, double d_tol, int max_iter, int precision,
-                            double *energy, double *C, double *eig, int *iterations) {
-    if (!ctx || !H || !S) return MYQC_E_ARG;
-    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf: no system set");
-    if (nocc < 0 || nocc > ctx->n || max_iter < 1 || e_tol <= 0 || d_tol <= 0)
-        return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf: invalid arguments");
-    if (precision != MYQC_FP64 && precision != MYQC_FP32)
-        return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf: precision %d", precision);
-    if (!ctx->have_eri || (precision == MYQC_FP32 && !ctx->have_eri32))
-        return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf: call myqc_eri_build first");
-    ctx->scf_purify = ctx->scf_diag = 0;
-    return rhf_scf(ctx, H, S, nocc, e_nuc, C_guess, e_tol, d_tol, max_iter, precision, energy, C, eig, iterations);
 }

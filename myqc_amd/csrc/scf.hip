@@ -1,0 +1,515 @@
+// scf.hip -- the in-process RHF driver: the RHF loop of scf.f90
+// (src/scf/scf.f90:61-216,720-904) with the dens and RHFI2G program spawns
+// replaced by device-resident D, G and the stored integrals, plus DIIS.
+//
+// Everything n x n lives on the device in the context's SCF buffers, which
+// persist across calls of the same basis size.  Per iteration the host reads
+// one small vector of scalars (energy terms, the new row of the DIIS matrix
+// and the previous density change) produced by one fused reduction, and
+// decides convergence from it; the density itself comes from the MFMA
+// purification in sp2.hip.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include <rocsolver/rocsolver.h>
+
+#include "context.h"
+
+namespace myqc {
+
+int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, int n, int nocc,
+               int *ok, int *steps);
+int gershgorin_bounds(myqc_ctx *ctx, const double *A, int n, double *lohi, double *slots);
+
+static dim3 grid2(int n) { return dim3((n + 127) / 128, n); }
+
+// --- elementwise kernels -----------------------------------------------------
+
+// U(:, j) *= s_j^power
+__global__ void k_scale_cols(double *__restrict__ U, const double *__restrict__ s, int n, double power) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    U[i + (size_t)n * j] *= pow(s[j], power);
+}
+
+__global__ void k_antisym(const double *__restrict__ T, double *__restrict__ E, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    E[i + (size_t)n * j] = T[i + (size_t)n * j] - T[j + (size_t)n * i];
+}
+
+// Z = a X + b Y + c I  (Y may be null when b = 0)
+__global__ void k_axpbyc(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, double a,
+                         double b, double c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    const size_t e = i + (size_t)n * j;
+    double v = (X ? a * X[e] : 0.0) + (Y ? b * Y[e] : 0.0);
+    if (i == j) v += c;
+    Z[e] = v;
+}
+
+struct LinComb {
+    double c[8];
+    int m;
+};
+
+// Fx = sum_i c_i hF_i (DIIS extrapolation over the history slots)
+__global__ void k_lincomb(const double *__restrict__ hF, size_t nn, LinComb lc, double *__restrict__ Fx) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < lc.m; ++i) v += lc.c[i] * hF[nn * i + e];
+        Fx[e] = v;
+    }
+}
+
+// --- fused reductions ---------------------------------------------------------
+//
+// Job d: mode 0 sum x*y, mode 1 max |x - y|, mode 2 max |x - I|.  Stage one
+// writes one partial per (job, block); stage two folds them in a fixed order,
+// so results are bitwise reproducible.
+constexpr int kRedJobs = 12, kRedBlocks = 64;
+
+struct RedJobs {
+    const double *x[kRedJobs];
+    const double *y[kRedJobs];
+    int mode[kRedJobs];
+    int m;
+    int n;   // matrix order (mode 2)
+};
+
+__device__ __forceinline__ double wave_fold(double v, int mode) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = mode == 0 ? v + o : fmax(v, o);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_part(RedJobs J, double *__restrict__ part) {
+    __shared__ double s[4];
+    const int d = blockIdx.y, mode = J.mode[d];
+    const size_t nn = (size_t)J.n * J.n;
+    const double *x = J.x[d], *y = J.y[d];
+    double acc = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+        if (mode == 0)
+            acc += x[e] * y[e];
+        else if (mode == 1)
+            acc = fmax(acc, fabs(x[e] - y[e]));
+        else
+            acc = fmax(acc, fabs(x[e] - ((e % J.n) == (e / J.n) ? 1.0 : 0.0)));
+    }
+    acc = wave_fold(acc, mode);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = s[0];
+        for (int w = 1; w < 4; ++w) v = mode == 0 ? v + s[w] : fmax(v, s[w]);
+        part[d * kRedBlocks + blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_reduce_final(RedJobs J, const double *__restrict__ part,
+                                                     double *__restrict__ out) {
+    const int d = blockIdx.x;
+    const double v = wave_fold(part[d * kRedBlocks + threadIdx.x], J.mode[d]);
+    if (threadIdx.x == 0) out[d] = v;
+}
+
+// --- buffers -------------------------------------------------------------------
+
+struct ScfBuffers {
+    int n = -1;
+    double *H = nullptr, *S = nullptr, *X = nullptr, *Xi = nullptr, *F = nullptr, *Fx = nullptr, *D = nullptr,
+           *Dn = nullptr, *Do = nullptr, *Don = nullptr, *Dref = nullptr, *G = nullptr, *C = nullptr, *T1 = nullptr,
+           *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr, *lohi = nullptr,
+           *part = nullptr, *red = nullptr;
+    double *hF = nullptr, *hE = nullptr;   // DIIS history, kDiis x n^2 each
+    int *info = nullptr;
+    static constexpr int kDiis = 8;
+    void release() {
+        double **ps[] = {&H,  &S,  &X,   &Xi,  &F,   &Fx,   &D,    &Dn,   &Do,   &Don, &Dref, &G, &C,
+                         &T1, &T2, &P2, &eig, &work, &dsc, &lohi, &part, &red, &hF,  &hE};
+        for (auto p : ps) dfree((void **)p);
+        dfree((void **)&info);
+        n = -1;
+    }
+    int ensure(myqc_ctx *ctx, int nb) {
+        if (n == nb) return 0;
+        release();
+        const size_t nn = (size_t)nb * nb;
+        double **mats[] = {&H, &S, &X, &Xi, &F, &Fx, &D, &Dn, &Do, &Don, &Dref, &G, &C, &T1, &T2, &P2};
+        for (auto m : mats) MYQC_CHECK(dalloc(ctx, (void **)m, sizeof(double) * nn));
+        MYQC_CHECK(dalloc(ctx, (void **)&eig, sizeof(double) * nb));
+        MYQC_CHECK(dalloc(ctx, (void **)&work, sizeof(double) * nb));
+        MYQC_CHECK(dalloc(ctx, (void **)&dsc, sizeof(double) * 8));
+        MYQC_CHECK(dalloc(ctx, (void **)&lohi, sizeof(double) * 2 * nb));
+        MYQC_CHECK(dalloc(ctx, (void **)&part, sizeof(double) * kRedJobs * kRedBlocks));
+        MYQC_CHECK(dalloc(ctx, (void **)&red, sizeof(double) * kRedJobs));
+        MYQC_CHECK(dalloc(ctx, (void **)&hF, sizeof(double) * nn * kDiis));
+        MYQC_CHECK(dalloc(ctx, (void **)&hE, sizeof(double) * nn * kDiis));
+        MYQC_CHECK(dalloc(ctx, (void **)&info, sizeof(int)));
+        n = nb;
+        return 0;
+    }
+};
+
+void free_scf(myqc_ctx *ctx) {
+    if (ctx->scf) {
+        ctx->scf->release();
+        delete ctx->scf;
+        ctx->scf = nullptr;
+    }
+}
+
+// runs the jobs and copies their results to the host (one synchronisation)
+static int reduce(myqc_ctx *ctx, ScfBuffers &b, RedJobs &J, double *out) {
+    J.n = b.n;
+    k_reduce_part<<<dim3(kRedBlocks, J.m), 256, 0, ctx->stream>>>(J, b.part);
+    k_reduce_final<<<J.m, 64, 0, ctx->stream>>>(J, b.part, b.red);
+    MYQC_HIP(ctx, hipGetLastError());
+    MYQC_HIP(ctx, d2h(ctx, out, b.red, sizeof(double) * J.m));
+    return 0;
+}
+
+static int add_job(myqc_ctx *ctx, RedJobs &J, int mode, const double *x, const double *y) {
+    if (J.m >= kRedJobs) return set_error(ctx, MYQC_E_ARG, "too many reductions");
+    J.x[J.m] = x;
+    J.y[J.m] = y;
+    J.mode[J.m] = mode;
+    return J.m++;
+}
+
+static int gemm(myqc_ctx *ctx, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha,
+                const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc) {
+    MYQC_BLAS(ctx, rocblas_dgemm(ctx->blas, ta, tb, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc));
+    return 0;
+}
+static const rocblas_operation NN = rocblas_operation_none, TT = rocblas_operation_transpose;
+
+// eigen-decomposition of the symmetric n x n A (overwritten by eigenvectors)
+static int syevd(myqc_ctx *ctx, ScfBuffers &b, double *A, double *w, int n) {
+    MYQC_BLAS(ctx, rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower, n, A, n, w, b.work, b.info));
+    int info = 0;
+    MYQC_HIP(ctx, d2h(ctx, &info, b.info, sizeof(int)));
+    if (info != 0) return set_error(ctx, MYQC_E_HIP, "rocsolver_dsyevd info=%d", info);
+    return 0;
+}
+
+// orbitals: T2 = eigenvectors of Fo, eig, and C = X T2
+static int diag_orth(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, int n) {
+    MYQC_HIP(ctx, hipMemcpyAsync(b.T2, Fo, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    MYQC_CHECK(syevd(ctx, b, b.T2, b.eig, n));
+    MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, b.X, n, b.T2, n, 0.0, b.C, n));
+    return 0;
+}
+
+// Fo = X^T F X
+static int to_orth(myqc_ctx *ctx, ScfBuffers &b, const double *F, double *Fo, int n) {
+    MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, F, n, b.X, n, 0.0, b.T1, n));
+    MYQC_CHECK(gemm(ctx, TT, NN, n, n, n, 1.0, b.X, n, b.T1, n, 0.0, Fo, n));
+    return 0;
+}
+
+// D = 2 X Do X^T
+static int to_ao_density(myqc_ctx *ctx, ScfBuffers &b, const double *Do, double *D, int n) {
+    MYQC_CHECK(gemm(ctx, NN, TT, n, n, n, 1.0, Do, n, b.X, n, 0.0, b.T1, n));
+    MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 2.0, b.X, n, b.T1, n, 0.0, D, n));
+    return 0;
+}
+
+// X = S^{-1/2}, Xi = S^{1/2} by the coupled Newton-Schulz iteration
+// (Higham, Numer. Algorithms 15, 227 (1997)): with A = S/c, c >= lambda_max,
+//     T = (3 I - Z Y)/2,  Y <- Y T,  Z <- T Z,   Y -> A^{1/2}, Z -> A^{-1/2};
+// matrix products only.  Returns *ok = 0 when it has not converged in its
+// step budget (an ill-conditioned S), for the eigen-decomposition path.
+static int inv_sqrt_ns(myqc_ctx *ctx, ScfBuffers &b, int n, int *ok) {
+    *ok = 0;
+    MYQC_CHECK(gershgorin_bounds(ctx, b.S, n, b.lohi, b.dsc));
+    double bnd[2];
+    MYQC_HIP(ctx, d2h(ctx, bnd, b.dsc + 6, sizeof bnd));
+    const double c = bnd[1];
+    if (!(c > 0.0)) return 0;
+    double *Y = b.Xi, *Z = b.X, *Yn = b.T2, *Zn = b.P2;
+    k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(b.S, nullptr, Y, 1.0 / c, 0.0, 0.0, n);
+    k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(nullptr, nullptr, Z, 0.0, 0.0, 1.0, n);
+    int extra = -1;
+    for (int k = 0; k < 40; ++k) {
+        MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, Z, n, Y, n, 0.0, b.T1, n));
+        if (extra < 0 && k >= 6 && (k & 1) == 0) {
+            RedJobs J{};
+            add_job(ctx, J, 2, b.T1, nullptr);
+            double r = 0.0;
+            MYQC_CHECK(reduce(ctx, b, J, &r));
+            // quadratic convergence: one more step takes 1e-9 below round-off
+            if (r < 1e-9) extra = 1;
+        }
+        if (extra == 0) {
+            *ok = 1;
+            break;
+        }
+        k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, nullptr, b.T1, -0.5, 0.0, 1.5, n);
+        MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, Y, n, b.T1, n, 0.0, Yn, n));
+        MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, b.T1, n, Z, n, 0.0, Zn, n));
+        std::swap(Y, Yn);
+        std::swap(Z, Zn);
+        if (extra > 0) --extra;
+    }
+    if (!*ok) return 0;
+    const double rc = sqrt(c);
+    k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(Z, nullptr, b.X, 1.0 / rc, 0.0, 0.0, n);
+    k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(Y, nullptr, b.Xi, rc, 0.0, 0.0, n);
+    MYQC_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+// X = S^{-1/2}, Xi = S^{1/2} through the eigen-decomposition of S; rejects a
+// numerically singular S
+static int inv_sqrt_eig(myqc_ctx *ctx, ScfBuffers &b, int n) {
+    MYQC_HIP(ctx, hipMemcpyAsync(b.T2, b.S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    MYQC_CHECK(syevd(ctx, b, b.T2, b.eig, n));
+    double s0 = 0.0;
+    MYQC_HIP(ctx, d2h(ctx, &s0, b.eig, sizeof(double)));
+    if (s0 <= 1e-10) return set_error(ctx, MYQC_E_ARG, "rhf: overlap matrix singular (min eigenvalue %g)", s0);
+    MYQC_HIP(ctx, hipMemcpyAsync(b.T1, b.T2, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n, -0.5);
+    MYQC_CHECK(gemm(ctx, NN, TT, n, n, n, 1.0, b.T1, n, b.T2, n, 0.0, b.X, n));
+    MYQC_HIP(ctx, hipMemcpyAsync(b.T1, b.T2, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    k_scale_cols<<<grid2(n), 128, 0, ctx->stream>>>(b.T1, b.eig, n, 0.5);
+    MYQC_CHECK(gemm(ctx, NN, TT, n, n, n, 1.0, b.T1, n, b.T2, n, 0.0, b.Xi, n));
+    return 0;
+}
+
+// Orthonormal-basis density projector onto the nocc lowest levels of Fo by
+// purification; falls back to diagonalisation if it does not converge.
+static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int n, int nocc) {
+    if (nocc == 0) {
+        k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(nullptr, nullptr, P, 0.0, 0.0, 0.0, n);
+        return 0;
+    }
+    int ok = 0;
+    MYQC_CHECK(sp2_purify(ctx, Fo, P, b.P2, b.lohi, b.dsc, n, nocc, &ok, &ctx->scf_sp2_steps));
+    ++ctx->scf_purify;
+    if (!ok) {
+        // no clean gap at this step: occupy the nocc lowest eigenvectors
+        ++ctx->scf_diag;
+        MYQC_CHECK(diag_orth(ctx, b, Fo, n));
+        MYQC_CHECK(gemm(ctx, NN, TT, n, n, nocc, 1.0, b.T2, n, b.T2, n, 0.0, P, n));
+    }
+    return 0;
+}
+
+// D = 2 C_occ C_occ^T (src/dens/dens.f90:115-124)
+static int density(myqc_ctx *ctx, const double *C, int nocc, double *D, int n) {
+    if (nocc == 0) {
+        k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(nullptr, nullptr, D, 0.0, 0.0, 0.0, n);
+        return 0;
+    }
+    return gemm(ctx, NN, TT, n, n, nocc, 2.0, C, n, C, n, 0.0, D, n);
+}
+
+// solve the DIIS system on the host (m <= 8), Gaussian elimination with pivoting
+static bool diis_coeffs(const std::vector<double> &B, int m, std::vector<double> &c) {
+    const int N = m + 1;
+    std::vector<double> M((size_t)N * (N + 1), 0.0);
+    for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < m; ++j) M[i * (N + 1) + j] = B[i * m + j];
+        M[i * (N + 1) + m] = -1.0;
+        M[i * (N + 1) + N] = 0.0;
+    }
+    for (int j = 0; j < m; ++j) M[m * (N + 1) + j] = -1.0;
+    M[m * (N + 1) + m] = 0.0;
+    M[m * (N + 1) + N] = -1.0;
+    for (int col = 0; col < N; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < N; ++r)
+            if (fabs(M[r * (N + 1) + col]) > fabs(M[piv * (N + 1) + col])) piv = r;
+        if (fabs(M[piv * (N + 1) + col]) < 1e-300) return false;
+        if (piv != col)
+            for (int k = 0; k <= N; ++k) std::swap(M[col * (N + 1) + k], M[piv * (N + 1) + k]);
+        for (int r = 0; r < N; ++r) {
+            if (r == col) continue;
+            const double f = M[r * (N + 1) + col] / M[col * (N + 1) + col];
+            for (int k = col; k <= N; ++k) M[r * (N + 1) + k] -= f * M[col * (N + 1) + k];
+        }
+    }
+    c.resize(m);
+    for (int i = 0; i < m; ++i) c[i] = M[i * (N + 1) + N] / M[i * (N + 1) + i];
+    return true;
+}
+
+// The loop.  Iteration k builds F[D_k] and E_k = (D_k.F_k + D_k.H)/2 + E_nuc,
+// extrapolates the orthonormal-basis Fock matrix by DIIS and purifies it into
+// D_{k+1}.  Iteration k has converged when k > 1, |E_k - E_{k-1}| < e_tol,
+// max|D_{k+1} - D_k| < d_tol and the DIIS error norm is below 100 d_tol; the
+// density change is read with iteration k+1's scalars, whose F[D_{k+1}] and
+// E_{k+1} are then the reported plain Fock matrix and energy (the reference
+// prints E of the density it diagonalises, scf.f90:775-848).  With
+// precision FP32, F_k = F_{k-1} + G32(D_k - D_{k-1}) between FP64 builds every
+// 8 iterations, and convergence is only accepted on two FP64 builds in a row.
+static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc, const double *C_guess,
+                   double e_tol, double d_tol, int max_iter, int precision, double *energy, double *C_out,
+                   double *eig_out, int *iters_out) {
+    const int n = ctx->n;
+    const size_t nn = (size_t)n * n;
+    const int MD = ScfBuffers::kDiis;
+    if (!ctx->scf) ctx->scf = new ScfBuffers();
+    ScfBuffers &b = *ctx->scf;
+    MYQC_CHECK(b.ensure(ctx, n));
+    MYQC_HIP(ctx, hipMemcpyAsync(b.H, H, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(b.S, S, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+    int ok = 0;
+    MYQC_CHECK(inv_sqrt_ns(ctx, b, n, &ok));
+    if (!ok) MYQC_CHECK(inv_sqrt_eig(ctx, b, n));
+    // guess density
+    if (C_guess) {
+        MYQC_HIP(ctx, hipMemcpyAsync(b.C, C_guess, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+        MYQC_CHECK(density(ctx, b.C, nocc, b.D, n));
+        // Do = S^{1/2} D S^{1/2} / 2
+        MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, b.D, n, b.Xi, n, 0.0, b.T1, n));
+        MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 0.5, b.Xi, n, b.T1, n, 0.0, b.Do, n));
+    } else {
+        // core Hamiltonian guess (scf.f90:410-512)
+        MYQC_CHECK(to_orth(ctx, b, b.H, b.F, n));
+        MYQC_CHECK(purify(ctx, b, b.F, b.Do, n, nocc));
+        MYQC_CHECK(to_ao_density(ctx, b, b.Do, b.D, n));
+    }
+    std::vector<double> Bm((size_t)MD * MD, 0.0);
+    int hist = 0, head = 0;
+    double E = 0.0, E_old = 0.0;
+    // convergence data of the previous iteration
+    double dE_prev = 1e300, err_prev = 1e300;
+    bool prev64 = false;
+    const bool incr = (precision == MYQC_FP32);
+    int since_full = 0;
+    bool final_fp64 = false, conv = false;
+    double *fslot = nullptr;
+    int it = 1;
+    while (true) {
+        const bool full = !incr || final_fp64 || since_full == 0 || since_full >= 8;
+        if (full) {
+            MYQC_CHECK(digest_G(ctx, b.D, b.G, MYQC_FP64));
+            since_full = 1;
+        } else {
+            k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(b.D, b.Dref, b.T1, 1.0, -1.0, 0.0, n);
+            MYQC_CHECK(digest_G(ctx, b.T1, b.G, MYQC_FP32));   // G += G32(dD)
+            ++since_full;
+        }
+        if (incr) MYQC_HIP(ctx, hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
+        k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(b.H, b.G, b.F, 1.0, 1.0, 0.0, n);
+        // orthonormal basis: Fo = X^T F X, DIIS error Fo Do - Do Fo
+        fslot = b.hF + nn * head;
+        double *eslot = b.hE + nn * head;
+        MYQC_CHECK(to_orth(ctx, b, b.F, fslot, n));
+        MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, fslot, n, b.Do, n, 0.0, b.T2, n));
+        k_antisym<<<grid2(n), 128, 0, ctx->stream>>>(b.T2, eslot, n);
+        const int hn = std::min(hist + 1, MD);   // history after this step, slot `head` included
+        RedJobs J{};
+        add_job(ctx, J, 0, b.D, b.F);
+        add_job(ctx, J, 0, b.D, b.H);
+        const int jdd = it > 1 ? add_job(ctx, J, 1, b.D, b.Dn) : -1;   // Dn holds D_{k-1}
+        const int jb0 = J.m;
+        for (int j = 0; j < hn; ++j) add_job(ctx, J, 0, eslot, b.hE + nn * j);
+        double r[kRedJobs];
+        MYQC_CHECK(reduce(ctx, b, J, r));
+        E = 0.5 * (r[0] + r[1]) + e_nuc;
+        if (jdd >= 0 && it > 2 && dE_prev < e_tol && r[jdd] < d_tol && err_prev < 100 * d_tol) {
+            if (!incr || (prev64 && full)) {
+                conv = true;
+                break;
+            }
+            if (!final_fp64) {
+                // confirm with FP64 builds, starting with this iteration's
+                final_fp64 = true;
+                if (!full) continue;
+            }
+        }
+        if (it > max_iter) break;
+        for (int j = 0; j < hn; ++j) {
+            Bm[head * MD + j] = r[jb0 + j];
+            Bm[j * MD + head] = r[jb0 + j];
+        }
+        hist = hn;
+        const double *Fuse = fslot;
+        if (hist >= 2) {
+            std::vector<double> Bs((size_t)hist * hist), c;
+            for (int i = 0; i < hist; ++i)
+                for (int j = 0; j < hist; ++j) Bs[i * hist + j] = Bm[i * MD + j];
+            if (diis_coeffs(Bs, hist, c)) {
+                LinComb lc{};
+                lc.m = hist;
+                for (int i = 0; i < hist; ++i) lc.c[i] = c[i];
+                k_lincomb<<<256, 256, 0, ctx->stream>>>(b.hF, nn, lc, b.Fx);
+                Fuse = b.Fx;
+            }
+        }
+        err_prev = sqrt(fabs(Bm[head * MD + head]));
+        dE_prev = fabs(E - E_old);
+        E_old = E;
+        prev64 = full;
+        head = (head + 1) % MD;
+        MYQC_CHECK(purify(ctx, b, Fuse, b.Don, n, nocc));
+        std::swap(b.D, b.Dn);   // Dn keeps D_k for the change
+        std::swap(b.Do, b.Don);
+        MYQC_CHECK(to_ao_density(ctx, b, b.Do, b.D, n));
+        ++it;
+    }
+    // orbitals of the plain Fock matrix of the reported density (fslot holds
+    // its orthonormal-basis form)
+    MYQC_CHECK(diag_orth(ctx, b, fslot, n));
+    if (energy) *energy = E;
+    if (C_out) MYQC_HIP(ctx, d2h(ctx, C_out, b.C, sizeof(double) * nn));
+    if (eig_out) MYQC_HIP(ctx, d2h(ctx, eig_out, b.eig, sizeof(double) * n));
+    if (iters_out) *iters_out = it - 1;
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!conv) return set_error(ctx, MYQC_E_NOCONV, "RHF did not converge in %d iterations", max_iter);
+    return 0;
+}
+
+}  // namespace myqc
+
+using namespace myqc;
+
+extern "C" int myqc_density_rhf(myqc_ctx *ctx, const double *C, int nocc, double *D) {
+    if (!ctx || !C || !D) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_density_rhf: no system set");
+    const int n = ctx->n;
+    if (nocc < 0 || nocc > n) return set_error(ctx, MYQC_E_ARG, "myqc_density_rhf: nocc=%d", nocc);
+    const size_t nn = (size_t)n * n;
+    double *dC = nullptr, *dD = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&dC, sizeof(double) * nn));
+    int rc = dalloc(ctx, (void **)&dD, sizeof(double) * nn);
+    if (rc == 0 && hipMemcpyAsync(dC, C, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "upload failed");
+    if (rc == 0) rc = density(ctx, dC, nocc, dD, n);
+    if (rc == 0 && d2h(ctx, D, dD, sizeof(double) * nn) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "download failed");
+    dfree((void **)&dC);
+    dfree((void **)&dD);
+    return rc;
+}
+
+extern "C" int myqc_scf_info(myqc_ctx *ctx, int *purifications, int *diag_fallbacks) {
+    if (!ctx) return MYQC_E_ARG;
+    if (purifications) *purifications = ctx->scf_purify;
+    if (diag_fallbacks) *diag_fallbacks = ctx->scf_diag;
+    return 0;
+}
+
+extern "C" int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
+                            const double *C_guess, double e_tol, double d_tol, int max_iter, int precision,
+                            double *energy, double *C, double *eig, int *iterations) {
+    if (!ctx || !H || !S) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf: no system set");
+    if (nocc < 0 || nocc > ctx->n || max_iter < 1 || e_tol <= 0 || d_tol <= 0)
+        return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf: invalid arguments");
+    if (precision != MYQC_FP64 && precision != MYQC_FP32)
+        return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf: precision %d", precision);
+    if (!ctx->have_eri || (precision == MYQC_FP32 && !ctx->have_eri32))
+        return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf: call myqc_eri_build first");
+    ctx->scf_purify = ctx->scf_diag = ctx->scf_sp2_steps = 0;
+    return rhf_scf(ctx, H, S, nocc, e_nuc, C_guess, e_tol, d_tol, max_iter, precision, energy, C, eig, iterations);
+}

@@ -90,25 +90,42 @@ __global__ __launch_bounds__(64) void k_sp2_start(const double *__restrict__ F, 
 
 // One purification step k: Pn = choice(P^2, 2P - P^2) from the traces in
 // slot k%3; Pn's traces accumulate into slot (k+1)%3; slot (k+2)%3 (last read
-// by step k-1) is cleared for step k+1.  Grid: one 64-lane wavefront per
-// 16x16 output tile.  Lane l feeds A row l%16 and B column l%16 at k offset
-// l/16 and receives D rows l/16 + 4v, column l%16, v = 0..3 (the layout the
-// GPU test of the builtin checks).
-__global__ __launch_bounds__(64) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn, int n,
-                                                     double nocc, double *__restrict__ slots, int step) {
+// by step k-1) is cleared for step k+1.  Grid: one workgroup of KS wavefronts
+// per 16x16 output tile, wavefront w summing the k range [w*kc, (w+1)*kc) so
+// that each dependent MFMA chain is n/(4 KS) long; the partial tiles meet in
+// LDS.  Lane l feeds A row l%16 and B column l%16 at k offset l/16 and
+// receives D rows l/16 + 4v, column l%16, v = 0..3 (the layout the GPU test of
+// the builtin checks).
+constexpr int KS = 4;
+
+__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
+                                                          int n, double nocc, double *__restrict__ slots, int step) {
+    __shared__ double s_acc[KS - 1][4][64];
     const int tiles = (n + 15) >> 4;
     const int ti = blockIdx.x % tiles, tj = blockIdx.x / tiles;
-    const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
     const int ia = ti * 16 + r;     // row of P fed as A
     const int jb = tj * 16 + r;     // column of P fed as B (read as a row: P is symmetric)
+    const int kc = ((n + 4 * KS - 1) / (4 * KS)) * 4;
+    const int kb = w * kc, ke = min(n, kb + kc);
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int k0 = 0; k0 < n; k0 += 4) {
+#pragma unroll 3
+    for (int k0 = kb; k0 < ke; k0 += 4) {
         const int k = k0 + kq;
-        const double a = (ia < n && k < n) ? P[ia + (size_t)n * k] : 0.0;
-        const double b = (jb < n && k < n) ? P[jb + (size_t)n * k] : 0.0;
+        const double a = (ia < n && k < ke) ? P[ia + (size_t)n * k] : 0.0;
+        const double b = (jb < n && k < ke) ? P[jb + (size_t)n * k] : 0.0;
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
+    if (w > 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_acc[w - 1][v][lane] = acc[v];
+    }
+    __syncthreads();
+    if (w > 0) return;
+#pragma unroll
+    for (int q = 0; q < KS - 1; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] += s_acc[q][v][lane];
     const double *sl = slots + 2 * (step % 3);
     const double tr = sl[0], fro = sl[1];
     const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
@@ -139,21 +156,34 @@ __global__ __launch_bounds__(64) void k_sp2_step_mfma(const double *__restrict__
     }
 }
 
+// Gershgorin bounds of the symmetric A into slots[6..7]; clears slots[0..5]
+int gershgorin_bounds(myqc_ctx *ctx, const double *A, int n, double *lohi, double *slots) {
+    k_gersh_rows<<<n, 64, 0, ctx->stream>>>(A, n, lohi);
+    k_gersh_reduce<<<1, 1024, 0, ctx->stream>>>(lohi, n, slots);
+    MYQC_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
 // Projector onto the nocc lowest levels of the symmetric Fo (n x n) into P.
 // work: n x n scratch; lohi: 2n; slots: 8 doubles.  Returns 1 in *ok when
 // purification converged (idempotency below 1e-8, then two more steps).
+// *steps carries the step count of the previous call in and this call's out:
+// successive SCF iterations need nearly the same count, so the host checks
+// convergence (a synchronising read of two scalars) only from a few steps
+// before it.
 int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, int n, int nocc,
-               int *ok) {
+               int *ok, int *steps) {
     *ok = 0;
-    k_gersh_rows<<<n, 64, 0, ctx->stream>>>(Fo, n, lohi);
-    k_gersh_reduce<<<1, 1024, 0, ctx->stream>>>(lohi, n, slots);
+    MYQC_CHECK(gershgorin_bounds(ctx, Fo, n, lohi, slots));
     const size_t nn = (size_t)n * n;
     k_sp2_start<<<(unsigned)((nn + 63) / 64), 64, 0, ctx->stream>>>(Fo, P, n, slots);
     const int tiles = (n + 15) / 16;
     double *cur = P, *nxt = work;
     int extra = -1;
+    const int first = *steps > 16 ? *steps - 4 : 12;
+    *steps = 0;
     for (int step = 0; step < 120; ++step) {
-        if (extra < 0 && step >= 12 && (step & 3) == 0) {
+        if (extra < 0 && step >= first && ((step - first) & 1) == 0) {
             double sl[2];
             MYQC_HIP(ctx, d2h(ctx, sl, slots + 2 * (step % 3), sizeof sl));
             // tr P - ||P||^2 = sum l(1-l) over P's eigenvalues l; near
@@ -162,9 +192,10 @@ int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double 
         }
         if (extra == 0) {
             *ok = 1;
+            *steps = step;
             break;
         }
-        k_sp2_step_mfma<<<tiles * tiles, 64, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, slots, step);
+        k_sp2_step_mfma<<<tiles * tiles, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, slots, step);
         double *t = cur;
         cur = nxt;
         nxt = t;
