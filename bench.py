@@ -10,6 +10,13 @@ uses incremental FP32 dD Fock builds with FP64 rebuilds and an FP64 finish.
 form to 1e-8 Eh, else the plain rate.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+
+`roofline` is the J/K digestion kernel's: algorithmic bytes per pass
+(8 N_stored + 16 n^2) over its HIP-event time.  `roofline.traffic` is the HBM
+bytes per digestion launch that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+passes measured, scaled by the same counters' reading on a plain copy of
+known size (--copy-calibration runs one), as recorded in
+profiles/k_digest_40water_traffic.json; null when that file is absent.
 """
 from __future__ import annotations
 
@@ -25,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_digest_40water_traffic.json")
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -74,6 +82,25 @@ def cpu_baseline(Z, xyz, basis, ftab, n_iter, budget_s=20.0):
                        f"extrapolated {total:.0f} s per solve")
 
 
+def copy_calibration(mb):
+    """One device-to-device copy of `mb` MiB, for scaling HBM counters."""
+    import torch
+    src = torch.ones(mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {mb} MiB read + {mb} MiB written")
+
+
+def measured_traffic(waters):
+    if waters != 40 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +109,8 @@ def main():
     ap.add_argument("--waters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="first run one device copy of MB MiB (scale for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); the path is single-GPU by design",
@@ -99,6 +128,8 @@ def main():
     nocc = sum(Z) // 2
     e_nuc = enuc(Z, xyz)
     log(f"[bench] {args.waters} waters, n={basis.norb}, nocc={nocc}")
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     ctx = Context(0)
     res = {}
     forms = [f for f in (("fp64", FP64), ("mixed", FP32)) if f[0] in args.forms.split(",")]
@@ -151,7 +182,7 @@ def main():
                 "TFLOPs": eri_tflops},
         "jk": {"GB_per_s": jk_gbs, "ms_per_pass": jk_ms / max(jk_n, 1), "passes": jk_n},
         "roofline": {"bound": "hbm", "achieved": jk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": jk_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": jk_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.waters),
                      "kernel": "J/K digestion (k_digest)"},
         "roofline_eri": {"bound": "fp64-valu", "achieved": eri_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": eri_tflops / FP64_PEAK_TFLOPS},

@@ -114,8 +114,11 @@ int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, doub
 int myqc_scf_info(myqc_ctx *ctx, int *purifications, int *diag_fallbacks);
 
 /* Kernel timing accumulated on the context's stream with HIP events since
- * the last reset: which = 0 ERI, 1 J/K digestion, 2 ao2mo, 3 MP2.
- * Returns launches, total milliseconds and algorithmic work (flops or bytes). */
+ * the last reset: which = 0 ERI (the shell-quartet class kernels of a build),
+ * 1 J/K digestion (the digestion kernel alone; work = bytes s*N_stored + 16 n^2,
+ * s = 8 FP64 or 4 FP32), 2 ao2mo (work = flops of the four quarter steps),
+ * 3 MP2.  Returns launches, total milliseconds and algorithmic work.
+ * No counterpart in the reference (it prints CPU_TIME per program). */
 int myqc_timer_reset(myqc_ctx *ctx, int enable);
 int myqc_timer_get(myqc_ctx *ctx, int which, long long *launches, double *ms, double *work);
 

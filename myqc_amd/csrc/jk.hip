@@ -269,7 +269,12 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, TV *pa
         a.partial = partial;
         a.n = ctx->n;
         const size_t shm = sizeof(TV) * 6 * ctx->n;
+        // timed alone: algorithmic bytes per pass are one read of every
+        // stored unique integral plus the density in and G out (16 n^2)
+        hipEvent_t ev;
+        MYQC_CHECK(timer_begin(ctx, T_JK, &ev));
         k_digest<TV><<<ctx->n_wg, 256, shm, ctx->stream>>>(a);
+        MYQC_CHECK(timer_end(ctx, T_JK, ev, (double)sizeof(TV) * (double)ctx->n_stored + 16.0 * ctx->n * ctx->n));
     }
     k_gather_rows<TV><<<dim3((ctx->n + 127) / 128, ctx->n), 128, 0, ctx->stream>>>(partial, ctx->d_shell,
                                                                                   ctx->d_ao_shell, ctx->n, Graw);
@@ -286,8 +291,6 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     const size_t nn = (size_t)n * n;
     if ((size_t)6 * n * sizeof(double) > 64 * 1024)
         return set_error(ctx, MYQC_E_ARG, "digestion rows for n=%d exceed 64 KB of LDS", n);
-    hipEvent_t ev;
-    MYQC_CHECK(timer_begin(ctx, T_JK, &ev));
     if (precision == MYQC_FP64) {
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, ctx->d_D, nullptr, n);
         MYQC_CHECK(launch_digest<double>(ctx, ctx->d_D, ctx->d_x64, ctx->d_Dpair, ctx->d_partial, ctx->d_A));
@@ -300,8 +303,6 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
         k_add<<<256, 256, 0, ctx->stream>>>(d_Gout, ctx->d_G, d_Gout, nn);
     }
     MYQC_HIP(ctx, hipGetLastError());
-    const double bytes = (precision == MYQC_FP64 ? 8.0 : 4.0) * (double)ctx->n_expanded + 16.0 * nn;
-    MYQC_CHECK(timer_end(ctx, T_JK, ev, bytes));
     return 0;
 }
 

@@ -11,6 +11,18 @@ from myqc_amd.hip import FP32, FP64, enuc
 pytestmark = pytest.mark.gpu
 
 
+def _dense_eri(f):
+    """The reference's XX from a fixture's canonical (ij|gh) list, 8-fold filled."""
+    n = int(f["n"])
+    XX = np.zeros((n, n, n, n))
+    i, j, g, h = f["eri_idx"].T
+    v = f["eri_val"]
+    for a, b, c, d in ((i, j, g, h), (j, i, g, h), (i, j, h, g), (j, i, h, g),
+                       (g, h, i, j), (h, g, i, j), (g, h, j, i), (h, g, j, i)):
+        XX[a, b, c, d] = v
+    return XX
+
+
 def _setup(ctx, name):
     f = golden(name)
     Z = [int(z) for z in f["Z"]]
@@ -74,6 +86,29 @@ def test_rhf_energy_matches_reference(ctx, name):
     assert r["purifications"] >= r["iterations"] and r["diag_fallbacks"] == 0
     # orbital energies up to rounding of the converged Fock matrix
     assert np.max(np.abs(r["eig"] - f["eig"])) <= 1e-6
+
+
+@pytest.mark.parametrize("name", [m for m in MOLECULES if m not in CONVERGED])
+def test_rhf_converges_where_reference_loop_oscillates(ctx, name):
+    """The reference's plain Roothaan loop does not converge on these inputs
+    (src/scf/scf.f90:720-904); the DIIS driver must reach the stationary
+    point the oracle's DIIS loop reaches (on the reference's S, H and XX)."""
+    from oracle import oracle as O
+    f, Z = _setup(ctx, name)
+    assert not bool(f["converged"])
+    ctx.eri_build(1e-12)
+    S, H = ctx.int1e()
+    nocc = int(f["nocc"])
+    r = ctx.rhf(H, S, nocc, enuc(Z, f["xyz"]), e_tol=1e-10, d_tol=1e-8)
+    XX = _dense_eri(f)
+    ro = O.rhf(f["S"], f["H"], XX, nocc, O.enuc(Z, f["xyz"]), diis=True)
+    assert abs(r["E"] - ro["E"]) <= 1e-8, (r["E"], ro["E"])
+    # stationary: F D S - S D F = 0 for the returned orbitals
+    C = r["C"]
+    D = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+    F = f["H"] + O.fock_G(XX, D)
+    comm = F @ D @ f["S"] - f["S"] @ D @ F
+    assert np.max(np.abs(comm)) <= 1e-6
 
 
 @pytest.mark.parametrize("name", ["co2", "h2o_2", "h2o_4"])

@@ -1,0 +1,154 @@
+"""The Fortran host programs (int1e, int2e, scf, ao2mo, mp2) run end to end
+over the iso_c_binding C ABI, reading and writing the reference's files
+(envdat/nucpos/fmem, mybasis, Ftab -> Suv/Huv, XX, Cui/Da/eig, ijab_AA/AB),
+checked against the reference's own outputs for the same inputs (committed
+fixtures) and the oracle.
+
+Reference program contracts: src/integrals/int1e.f90, src/integrals/int2e.f90
+(XX: one unformatted record, :166,306), src/scf/scf.f90 (RHF, :61-216),
+src/ao2mo/ao2mo.f90 (:465-602, records :566-582), src/mp2/mp2.f90 (:79-144).
+"""
+from __future__ import annotations
+
+import os
+import re
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "myqc_amd", "bin")
+DATA = os.path.join(ROOT, "myqc_amd", "data")
+
+pytestmark = pytest.mark.gpu
+
+
+def _records(path):
+    """Fortran sequential unformatted records (4-byte markers, subrecords)."""
+    buf = open(path, "rb").read()
+    out, pos = [], 0
+    while pos < len(buf):
+        (m,) = struct.unpack("<i", buf[pos:pos + 4])
+        chunks = []
+        while True:
+            ln = abs(m)
+            chunks.append(buf[pos + 4:pos + 4 + ln])
+            pos += 8 + ln
+            if m >= 0:
+                break
+            (m,) = struct.unpack("<i", buf[pos:pos + 4])
+        out.append(np.frombuffer(b"".join(chunks), dtype="<f8"))
+    return out
+
+
+def _text_matrix(path, n):
+    return np.array(open(path).read().split(), dtype=np.float64)[: n * n].reshape(n, n, order="F")
+
+
+def _write_inputs(d, Z, xyz_bohr, scf_conv=11):
+    """The files the reference's parser writes (src/myQC/env.f90:16-73):
+    envdat = nnuc / nelcA nelcB / 17 / options, nucpos = Z x y z (bohr)."""
+    nelc = int(sum(Z))
+    opts = [1, 1, 0, 0, 0, 1, 1000000, 1, scf_conv, 0, 1, 1, 1, 0, 0, 0, 0]
+    with open(os.path.join(d, "envdat"), "w") as f:
+        f.write(f" {len(Z)}\n {nelc // 2} {nelc // 2}\n 17\n " + " ".join(map(str, opts)) + "\n")
+    with open(os.path.join(d, "nucpos"), "w") as f:
+        for z, r in zip(Z, xyz_bohr):
+            f.write(f" {int(z)} {float(r[0])!r} {float(r[1])!r} {float(r[2])!r}\n")
+    with open(os.path.join(d, "fmem"), "w") as f:
+        f.write(" 1000000.0\n")
+    shutil.copy(os.path.join(DATA, "Ftab"), os.path.join(d, "Ftab"))
+    shutil.copy(os.path.join(DATA, "mybasis"), os.path.join(d, "mybasis"))
+
+
+def _run(d, prog):
+    exe = os.path.join(BIN, prog)
+    print(f"[program] {prog} in {d}", flush=True)
+    r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and not os.path.exists(os.path.join(d, "error")), \
+        f"{prog} failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    return r.stdout
+
+
+def _number(text, label):
+    m = re.search(re.escape(label) + r"\s*=?\s*([-+0-9.EeDd]+)", text)
+    assert m, f"no '{label}' in output:\n{text}"
+    return float(m.group(1).replace("D", "E").replace("d", "e"))
+
+
+@pytest.mark.parametrize("mol", ["co2", "h2o"])
+def test_programs_end_to_end(mol, tmp_path):
+    if not all(os.path.exists(os.path.join(BIN, p)) for p in ("int1e", "int2e", "scf", "ao2mo", "mp2")):
+        pytest.fail("Fortran host programs not built (myqc_amd/build.py)")
+    from oracle import oracle as O
+
+    g = golden(mol)
+    n, nocc = int(g["n"]), int(g["nocc"])
+    d = str(tmp_path)
+    _write_inputs(d, g["Z"], g["xyz"])
+
+    _run(d, "int1e")
+    S = _text_matrix(os.path.join(d, "Suv"), n)
+    H = _text_matrix(os.path.join(d, "Huv"), n)
+    assert np.max(np.abs(S - g["S"])) <= 1e-10
+    assert np.max(np.abs(H - g["H"])) <= 1e-10
+
+    _run(d, "int2e")
+    recs = _records(os.path.join(d, "XX"))
+    assert len(recs) == 1 and recs[0].size == n ** 4
+    XX = recs[0].reshape(n, n, n, n, order="F")
+    assert np.max(np.abs(XX - g["XX_full"])) <= 1e-10
+    # int2e skips when XX exists (int2e.f90:58-67)
+    assert "Reading two electron integrals" in _run(d, "int2e")
+
+    out = _run(d, "scf")
+    E = _number(out, "Total SCF energy (a.u)")
+    assert abs(E - float(g["E_scf"])) <= 1e-8, (E, float(g["E_scf"]))
+    eig = np.array(open(os.path.join(d, "eig")).read().split(), dtype=np.float64)
+    assert np.max(np.abs(eig - g["eig"])) <= 1e-6
+    C = _text_matrix(os.path.join(d, "Cui"), n)
+    # orthonormal orbitals, and the density record is 2 C_occ C_occ^T
+    assert np.max(np.abs(C.T @ g["S"] @ C - np.eye(n))) <= 1e-9
+    Da = _records(os.path.join(d, "Da"))[0].reshape(n, n, order="F")
+    assert np.max(np.abs(Da - 2.0 * C[:, :nocc] @ C[:, :nocc].T)) <= 1e-12
+    assert abs(np.trace(Da @ g["S"]) - 2 * nocc) <= 1e-9
+
+    _run(d, "ao2mo")
+    nv = n - nocc
+    ab = _records(os.path.join(d, "ijab_AB"))
+    aa = _records(os.path.join(d, "ijab_AA"))
+    assert len(ab) == nocc * nocc and len(aa) == nocc * (nocc - 1) // 2
+    # (ia|jb) of this run's own orbitals, by the oracle's quarter transforms
+    iajb = O.ao2mo_ovov(g["XX_full"], C, nocc)
+    k = 0
+    for i in range(nocc):
+        for j in range(nocc):
+            rec = ab[k].reshape(nv, nv, order="F")
+            assert np.max(np.abs(rec - iajb[i, :, j, :])) <= 1e-10
+            k += 1
+    rec01 = aa[0].reshape(nv, nv, order="F")
+    assert np.max(np.abs(rec01 - iajb[0, :, 1, :])) <= 1e-10
+
+    out = _run(d, "mp2")
+    # the reference's mp2 prints the correlation energies (mp2.f90:136-141)
+    for label, key in (("E(AA) =", "E_aa"), ("E(AB) =", "E_ab"), ("E(MBPT2) =", "E_mp2")):
+        v = _number(out, label)
+        assert abs(v - float(g[key])) <= 1e-8, (label, v, float(g[key]))
+
+
+def test_program_error_sentinel(tmp_path):
+    """A failing program writes the `error` sentinel and stops nonzero, and
+    later programs stop at once when it exists (src/myQC/env.f90:36-41)."""
+    d = str(tmp_path)
+    g = golden("h2o")
+    _write_inputs(d, g["Z"], g["xyz"])
+    os.remove(os.path.join(d, "Ftab"))
+    r = subprocess.run([os.path.join(BIN, "int2e")], cwd=d, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and os.path.exists(os.path.join(d, "error"))
+    r = subprocess.run([os.path.join(BIN, "int1e")], cwd=d, capture_output=True, text=True, timeout=60)
+    assert not os.path.exists(os.path.join(d, "Suv"))
