@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_digest_40water_traffic.json")
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0   # dense MFMA
 
 
 def log(msg):
@@ -82,6 +83,40 @@ def cpu_baseline(Z, xyz, basis, ftab, n_iter, budget_s=20.0):
                        f"extrapolated {total:.0f} s per solve")
 
 
+def mp2_leg(ctx, waters, reps=2):
+    """RHF-MP2 of a seeded water cluster: the ao2mo quarter transforms in FP64
+    (rocBLAS DGEMM) and split-bf16 (MFMA, FP32 accumulation), each timed over
+    its four GEMM launches with HIP events; E(MBPT2) of both forms."""
+    from myqc_amd import molecules as M
+    from myqc_amd.basis import build_basis, read_ftab
+    from myqc_amd.hip import BF16X3, FP64, T_AO2MO, enuc
+    Z, x = M.water_cluster(waters, seed=waters)
+    xyz = M.to_bohr(Z, x)
+    basis = build_basis(Z)
+    nocc = sum(Z) // 2
+    ctx.set_system(Z, xyz, basis, read_ftab())
+    S, H = ctx.int1e()
+    ctx.eri_build(1e-12)
+    r = ctx.rhf(H, S, nocc, enuc(Z, xyz), e_tol=1e-10, d_tol=1e-8, max_iter=200)
+    n = basis.norb
+    out = {"workload": f"RHF-MP2/STO-3G {waters}-water cluster (seed {waters}), n={n}, o={nocc}, v={n - nocc}",
+           "E_scf": r["E"]}
+    for name, prec in (("fp64", FP64), ("bf16x3", BF16X3)):
+        ctx.ao2mo(r["C"], nocc, prec, download=False)   # warmup
+        ctx.timer_reset(True)
+        for _ in range(reps):
+            ctx.ao2mo(r["C"], nocc, prec, download=False)
+        cnt, ms, flops = ctx.timer_get(T_AO2MO)
+        ctx.timer_reset(False)
+        _, _, e2 = ctx.mp2(r["eig"], nocc)
+        tf = (flops / cnt) / (ms / cnt * 1e-3) / 1e12
+        peak = FP64_PEAK_TFLOPS if prec == FP64 else BF16_PEAK_TFLOPS
+        out[name] = {"E_mp2": e2, "ao2mo_ms": ms / cnt, "TFLOPs": tf, "peak": peak, "frac": tf / peak}
+        log(f"[bench] mp2 {name}: E(MBPT2)={e2:.10f} ao2mo {ms / cnt:.2f} ms {tf:.1f} TFLOP/s")
+    out["dE_mp2_bf16x3_vs_fp64"] = out["bf16x3"]["E_mp2"] - out["fp64"]["E_mp2"]
+    return out
+
+
 def copy_calibration(mb):
     """One device-to-device copy of `mb` MiB, for scaling HBM counters."""
     import torch
@@ -109,6 +144,7 @@ def main():
     ap.add_argument("--waters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--mp2-waters", type=int, default=24, help="water cluster of the MP2 leg (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run one device copy of MB MiB (scale for HBM counters)")
     args = ap.parse_args()
@@ -148,6 +184,7 @@ def main():
         ctx.timer_reset(False)
         res[name] = dict(E=r["E"], iters=r["iterations"], sec=dt / args.steps, eri=eri, jk=jk,
                          n_stored=r["n_stored"])
+    mp2 = mp2_leg(ctx, args.mp2_waters) if args.mp2_waters else None
     ctx.close()
     p = res.get("fp64") or res["mixed"]
     m = res.get("mixed") or p
@@ -187,6 +224,8 @@ def main():
         "roofline_eri": {"bound": "fp64-valu", "achieved": eri_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": eri_tflops / FP64_PEAK_TFLOPS},
     }
+    if mp2:
+        out["mp2"] = mp2
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle port, sampled)")
         out["cpu_baseline"] = cpu_baseline(Z, xyz, basis, ftab, p["iters"])
