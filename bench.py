@@ -117,6 +117,58 @@ def mp2_leg(ctx, waters, reps=2):
     return out
 
 
+def batch_leg(ctx, nmol, steps=3, plain_sample=64):
+    """The seeded 4096-molecule batch (2-8 atoms from H..Ne on random trees):
+    converged RHF solves/s of the batched persistent launch (graph replays,
+    integrals + SCF from the core-Hamiltonian guess) against the plain
+    single-molecule FP64 path timed on a sample of the same molecules."""
+    from myqc_amd import molecules as M
+    from myqc_amd.hip import MyqcError, enuc
+    rng = np.random.default_rng(nmol)
+    mols = []
+    for _ in range(nmol):
+        Z, x = M.random_molecule(rng)
+        mols.append((Z, M.to_bohr(Z, x)))
+    ctx.batch_set(mols)
+    t0 = time.perf_counter()
+    E, it, cv = ctx.batch_rhf(e_tol=1e-10, d_tol=1e-8, max_iter=100)   # tunes launch shapes, captures the graph
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E, it, cv = ctx.batch_rhf(e_tol=1e-10, d_tol=1e-8, max_iter=100)
+    dt = (time.perf_counter() - t0) / steps
+    conv = int(np.sum(cv == 1))
+    info = ctx.batch_info()
+    # plain path on a sample: set_system + int1e + screened ERIs + SCF per molecule
+    idx = np.random.default_rng(1).choice(nmol, size=min(plain_sample, nmol), replace=False)
+    t0 = time.perf_counter()
+    done, worst = 0, 0.0
+    for m in idx:
+        Z, xyz = mols[m]
+        ctx.set_system(Z, xyz)
+        S, H = ctx.int1e()
+        ctx.eri_build(1e-12)
+        try:
+            r = ctx.rhf(H, S, sum(Z) // 2, enuc(Z, xyz), e_tol=1e-10, d_tol=1e-8, max_iter=100)
+        except MyqcError:
+            continue
+        done += 1
+        if cv[m] == 1:
+            worst = max(worst, abs(r["E"] - E[m]))
+    t_plain = (time.perf_counter() - t0) / len(idx)
+    out = {"workload": f"{nmol} seeded molecules (seed {nmol}), 2-8 atoms H..Ne, n = {min(ctx.batch_norb)}.."
+                       f"{max(ctx.batch_norb)} (mean {np.mean(ctx.batch_norb):.1f})",
+           "batched_solves_per_s": conv / dt, "ms_per_batch": dt * 1e3, "converged": conv,
+           "mean_iterations": float(np.mean(it[cv == 1])), "first_call_ms": t_first * 1e3,
+           "plain_fp64_solves_per_s": 1.0 / t_plain, "plain_sample": int(len(idx)),
+           "speedup_vs_plain": (conv / dt) * t_plain, "max_dE_vs_plain": worst,
+           "eri_threads": info["eri_threads"], "scf_threads": info["scf_threads"],
+           "stored_integrals": info["n_integrals"]}
+    log(f"[bench] batch {nmol}: {out['batched_solves_per_s']:.0f} solves/s ({dt * 1e3:.1f} ms per batch, "
+        f"{conv} converged, {out['mean_iterations']:.1f} iterations); plain {out['plain_fp64_solves_per_s']:.1f}/s")
+    return out
+
+
 def copy_calibration(mb):
     """One device-to-device copy of `mb` MiB, for scaling HBM counters."""
     import torch
@@ -145,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--mp2-waters", type=int, default=24, help="water cluster of the MP2 leg (0: skip)")
+    ap.add_argument("--batch", type=int, default=4096, help="molecules in the batched leg (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run one device copy of MB MiB (scale for HBM counters)")
     args = ap.parse_args()
@@ -185,6 +238,7 @@ def main():
         res[name] = dict(E=r["E"], iters=r["iterations"], sec=dt / args.steps, eri=eri, jk=jk,
                          n_stored=r["n_stored"])
     mp2 = mp2_leg(ctx, args.mp2_waters) if args.mp2_waters else None
+    batch = batch_leg(ctx, args.batch) if args.batch else None
     ctx.close()
     p = res.get("fp64") or res["mixed"]
     m = res.get("mixed") or p
@@ -226,6 +280,8 @@ def main():
     }
     if mp2:
         out["mp2"] = mp2
+    if batch:
+        out["batch"] = batch
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle port, sampled)")
         out["cpu_baseline"] = cpu_baseline(Z, xyz, basis, ftab, p["iters"])

@@ -113,6 +113,32 @@ int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, doub
  * the reference (it diagonalises every iteration, src/scf/scf.f90:848). */
 int myqc_scf_info(myqc_ctx *ctx, int *purifications, int *diag_fallbacks);
 
+/* Batched RHF of many small molecules (1..40 functions each) in one
+ * persistent launch.  No counterpart in the reference, which solves one
+ * molecule per run of its programs; for callers that screen or train on
+ * thousands of molecules.  myqc_batch_set takes, per molecule, exactly the
+ * arrays myqc_set_system takes, concatenated in molecule order:
+ *   mol_nnuc[m], mol_norb[m], mol_nocc[m];
+ *   Z (sum nnuc) and xyz_bohr (each molecule an nnuc x 3 col-major block);
+ *   set_alpha, setinfo, bas, basinfo (buildBasis layouts) with
+ *   offsets[4 m + {0,1,2,3}] = the molecule's first element in each.
+ * myqc_batch_rhf runs integrals + SCF for the whole batch (DIIS, |dE| <
+ * e_tol and max|dD| < d_tol as myqc_rhf_scf) from the core-Hamiltonian guess
+ * or, with D_guess != NULL, from the given densities (n_m x n_m blocks,
+ * concatenated).  The first call tunes the launch shapes and captures the
+ * pipeline in a hipGraph; later calls replay it.  converged[m] is 1, 0 (not
+ * within max_iter) or -1 (singular overlap). */
+int myqc_batch_set(myqc_ctx *ctx, int nmol, const int *mol_nnuc, const int *mol_norb, const int *mol_nocc,
+                   const int *Z, const double *xyz_bohr, const double *set_alpha, const int *setinfo,
+                   const double *bas, const int *basinfo, const int *offsets, const double *Ftab);
+int myqc_batch_rhf(myqc_ctx *ctx, const double *D_guess, double e_tol, double d_tol, int max_iter, double *energy,
+                   int *iterations, int *converged);
+/* The converged densities of the last myqc_batch_rhf (n_m x n_m blocks). */
+int myqc_batch_density(myqc_ctx *ctx, double *D);
+/* Chosen launch shapes (threads per ERI class kernel, per SCF workgroup),
+ * graph replays so far and stored integrals of the batch. */
+int myqc_batch_info(myqc_ctx *ctx, int *eri_threads, int *scf_threads, int *graph_replays, long long *n_integrals);
+
 /* Kernel timing accumulated on the context's stream with HIP events since
  * the last reset: which = 0 ERI (the shell-quartet class kernels of a build),
  * 1 J/K digestion (the digestion kernel alone; work = bytes s*N_stored + 16 n^2,

@@ -26,6 +26,7 @@ struct QuartetClass {
 };
 
 struct ScfBuffers;
+struct BatchState;
 
 enum TimerCat { T_ERI = 0, T_JK = 1, T_AO2MO = 2, T_MP2 = 3, T_NCAT = 4 };
 
@@ -104,6 +105,7 @@ struct myqc_ctx {
 
     // last RHF run: purification steps and diagonalisation fallbacks
     myqc::ScfBuffers *scf = nullptr;       // RHF work arrays, kept across calls
+    myqc::BatchState *batch = nullptr;     // batched small-molecule RHF (batch.hip)
     int scf_purify = 0, scf_diag = 0;
     int scf_sp2_steps = 0;                // steps the last purification took
 
@@ -160,5 +162,6 @@ int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision);
 int ensure_work(myqc_ctx *ctx);
 void free_eri(myqc_ctx *ctx);
 void free_scf(myqc_ctx *ctx);
+void free_batch(myqc_ctx *ctx);
 
 }  // namespace myqc

@@ -41,6 +41,10 @@ EXPORTS = {
     "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
                       ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
     "myqc_scf_info": ([_vp, _ip, _ip], ctypes.c_int),
+    "myqc_batch_set": ([_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _ip, _dp, _ip, _ip, _dp], ctypes.c_int),
+    "myqc_batch_rhf": ([_vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip], ctypes.c_int),
+    "myqc_batch_density": ([_vp, _dp], ctypes.c_int),
+    "myqc_batch_info": ([_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "myqc_timer_reset": ([_vp, ctypes.c_int], ctypes.c_int),
     "myqc_timer_get": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp, _dp], ctypes.c_int),
 }
@@ -218,6 +222,74 @@ class Context:
                     diag_fallbacks=ndiag.value)
 
     # -- timing -------------------------------------------------------------
+    # -- batched small molecules -------------------------------------------
+    def batch_set(self, molecules, ftab=None):
+        """molecules: list of (Z, xyz_bohr[, charge]); each one's reference
+        basis arrays (buildBasis layouts) are concatenated with offsets."""
+        ftab = read_ftab() if ftab is None else np.asarray(ftab, dtype=np.float64)
+        nnuc, norb, nocc, Z, xyz, al, si, bs, bi, off = [], [], [], [], [], [], [], [], [], []
+        o = [0, 0, 0, 0]
+        for mol in molecules:
+            z, x = list(mol[0]), np.asarray(mol[1], dtype=np.float64).reshape(-1, 3)
+            charge = mol[2] if len(mol) > 2 else 0
+            b = build_basis(z)
+            nnuc.append(len(z))
+            norb.append(b.norb)
+            nocc.append((sum(z) - charge) // 2)
+            Z.extend(z)
+            xyz.append(x.ravel(order="F"))
+            off.extend(o)
+            for k, arr in enumerate((b.set_alpha, b.setinfo, b.bas, b.basinfo)):
+                (al, si, bs, bi)[k].append(np.asarray(arr))
+                o[k] += len(arr)
+        self._b = dict(
+            nnuc=np.array(nnuc, np.int32), norb=np.array(norb, np.int32), nocc=np.array(nocc, np.int32),
+            Z=np.array(Z, np.int32), xyz=np.concatenate(xyz), al=np.concatenate(al).astype(np.float64),
+            si=np.concatenate(si).astype(np.int32), bs=np.concatenate(bs).astype(np.float64),
+            bi=np.concatenate(bi).astype(np.int32), off=np.array(off, np.int32),
+            ftab=np.ascontiguousarray(ftab, dtype=np.float64))
+        d = self._b
+        self._chk(self.L.myqc_batch_set(self.h, len(molecules), _p(d["nnuc"], _ip), _p(d["norb"], _ip),
+                                        _p(d["nocc"], _ip), _p(d["Z"], _ip), _p(d["xyz"]), _p(d["al"]),
+                                        _p(d["si"], _ip), _p(d["bs"]), _p(d["bi"], _ip), _p(d["off"], _ip),
+                                        _p(d["ftab"])))
+        self.batch_norb = d["norb"].copy()
+        self.batch_nocc = d["nocc"].copy()
+
+    def batch_rhf(self, D_guess=None, e_tol=1e-10, d_tol=1e-8, max_iter=100):
+        """One persistent launch over the batch; D_guess: list of n_m x n_m
+        densities (the learned guess) or None for the core-Hamiltonian guess.
+        Returns (energies, iterations, converged)."""
+        nm = len(self.batch_norb)
+        E = np.zeros(nm)
+        it = np.zeros(nm, np.int32)
+        cv = np.zeros(nm, np.int32)
+        g = None
+        if D_guess is not None:
+            g = np.concatenate([np.asarray(D, dtype=np.float64).ravel(order="F") for D in D_guess])
+        self._chk(self.L.myqc_batch_rhf(self.h, None if g is None else _p(g), e_tol, d_tol, max_iter, _p(E),
+                                        _p(it, _ip), _p(cv, _ip)))
+        return E, it, cv
+
+    def batch_density(self):
+        """The densities of the last batch_rhf, one n_m x n_m array each."""
+        tot = int(np.sum(self.batch_norb.astype(np.int64) ** 2))
+        D = np.zeros(tot)
+        self._chk(self.L.myqc_batch_density(self.h, _p(D)))
+        out, o = [], 0
+        for n in self.batch_norb:
+            out.append(D[o:o + n * n].reshape(n, n, order="F"))
+            o += n * n
+        return out
+
+    def batch_info(self):
+        eri = np.zeros(6, np.int32)
+        scf, rep, nint = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        self._chk(self.L.myqc_batch_info(self.h, _p(eri, _ip), ctypes.byref(scf), ctypes.byref(rep),
+                                         ctypes.byref(nint)))
+        return dict(eri_threads=eri.tolist(), scf_threads=scf.value, graph_replays=rep.value,
+                    n_integrals=nint.value)
+
     def timer_reset(self, enable: bool = True):
         self._chk(self.L.myqc_timer_reset(self.h, 1 if enable else 0))
 
