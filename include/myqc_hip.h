@@ -108,6 +108,14 @@ int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, doub
                  double e_tol, double d_tol, int max_iter, int precision, double *energy, double *C, double *eig,
                  int *iterations);
 
+/* myqc_rhf_scf started from an AO density D_guess (n x n, the 2 C_occ C_occ^T
+ * convention of densRHF) instead of orbitals: the entry for the learned
+ * guess.  No counterpart in the reference (restarts read Cui,
+ * src/scf/scf.f90:148-156). */
+int myqc_rhf_scf_density(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
+                         const double *D_guess, double e_tol, double d_tol, int max_iter, int precision,
+                         double *energy, double *C, double *eig, int *iterations);
+
 /* Diagnostics of the last myqc_rhf_scf: densities obtained by purification
  * and the iterations that fell back to diagonalisation.  No counterpart in
  * the reference (it diagonalises every iteration, src/scf/scf.f90:848). */
@@ -138,6 +146,44 @@ int myqc_batch_density(myqc_ctx *ctx, double *D);
 /* Chosen launch shapes (threads per ERI class kernel, per SCF workgroup),
  * graph replays so far and stored integrals of the batch. */
 int myqc_batch_info(myqc_ctx *ctx, int *eri_threads, int *scf_threads, int *graph_replays, long long *n_integrals);
+
+/* Learned initial guess.  No counterpart in the reference (its SCF starts
+ * from the core Hamiltonian, src/scf/scf.f90:410-512, which stays the
+ * fallback); for callers that solve many similar molecules.  An MLP with
+ * three SiLU hidden layers of `width` maps n_in atom-pair features to n_out
+ * bond-frame density invariants (the batch guess uses n_in = 200,
+ * n_out = 16).  Arrays are row-major FP32 (rows x n_in, rows x n_out).
+ *   myqc_mlp_create    Xavier-uniform weights from `seed`, zero biases
+ *   myqc_mlp_params    copy the flat parameter vector out (get) and/or in
+ *                      (set); count = its length
+ *   myqc_mlp_forward   inference
+ *   myqc_mlp_gradient  masked mean-squared error sum(M (Y - T)^2) / sum(M)
+ *                      and its gradient over the parameters
+ *   myqc_mlp_train     Adam over `epochs` shuffled passes of minibatches,
+ *                      the data resident on the device; loss_per_epoch
+ *                      (may be NULL) gets each epoch's first-batch loss
+ *                      sum(M (Y - T)^2) / (batch n_out) */
+int myqc_mlp_create(myqc_ctx *ctx, int n_in, int width, int n_out, unsigned long long seed);
+int myqc_mlp_params(myqc_ctx *ctx, float *get, const float *set, long long *count);
+int myqc_mlp_forward(myqc_ctx *ctx, const float *X, int rows, float *Y);
+int myqc_mlp_gradient(myqc_ctx *ctx, const float *X, const float *T, const float *M, int rows, float *loss,
+                      float *grad);
+int myqc_mlp_train(myqc_ctx *ctx, const float *X, const float *T, const float *M, int rows, int epochs, int batch,
+                   float lr, unsigned long long seed, float *loss_per_epoch);
+/* Atom pairs (A <= B) of the batch molecules: their features, and the
+ * targets and masks taken from the densities of the last myqc_batch_rhf
+ * (masks zero for molecules that did not converge).  With all three
+ * pointers NULL only *npair is returned. */
+int myqc_batch_pair_data(myqc_ctx *ctx, int *npair, float *features, float *targets, float *mask);
+/* The network's AO density (n x n) for one molecule of H..Ne atoms, as the
+ * D_guess of myqc_rhf_scf_density; xyz_bohr is (nnuc, 3) column-major. */
+int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const double *xyz_bohr, double *D);
+/* myqc_batch_rhf started from the network's densities, computed on the
+ * device inside the same captured graph; molecules that do not converge from
+ * it are solved again from the core-Hamiltonian guess (fallback[m] = 1,
+ * iterations[m] counts both runs). */
+int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol, int max_iter, double *energy, int *iterations,
+                           int *converged, int *fallback);
 
 /* Kernel timing accumulated on the context's stream with HIP events since
  * the last reset: which = 0 ERI (the shell-quartet class kernels of a build),

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "context.h"
+#include "guess.h"
 #include "one_e_math.h"
 
 namespace myqc {
@@ -583,6 +584,15 @@ struct BatchState {
     double *d_eri = nullptr, *d_S = nullptr, *d_H = nullptr, *d_D = nullptr, *d_guess = nullptr, *d_scratch = nullptr,
            *d_hist = nullptr, *d_energy = nullptr;
     int *d_iters = nullptr, *d_conv = nullptr, *d_counter = nullptr;
+    // atom pairs of the learned guess
+    bool guess_ok = false;
+    int npair = 0;
+    PairRec *d_pairrec = nullptr;
+    int *d_mol_atoms = nullptr, *d_fail = nullptr;
+    std::vector<int> pair_mol_of;   // molecule of each pair record
+    hipGraphExec_t graph_learned = nullptr;
+    double gl_etol = 0, gl_dtol = 0;
+    int gl_maxit = 0, gl_gen = -1;
     // launch shapes (autotuned) and the captured graph
     int eri_threads[6] = {256, 256, 256, 256, 256, 256};
     int scf_threads = 256;
@@ -598,8 +608,10 @@ struct BatchState {
                        (void **)&d_atoms,  (void **)&d_ftT,    (void **)&d_eri,      (void **)&d_S,
                        (void **)&d_H,      (void **)&d_D,      (void **)&d_guess,    (void **)&d_scratch,
                        (void **)&d_hist,   (void **)&d_energy, (void **)&d_iters,    (void **)&d_conv,
-                       (void **)&d_counter};
+                       (void **)&d_counter, (void **)&d_pairrec, (void **)&d_mol_atoms, (void **)&d_fail};
         for (auto p : ps) dfree(p);
+        if (graph_learned) (void)hipGraphExecDestroy(graph_learned);
+        graph_learned = nullptr;
         for (auto &q : d_qcum) dfree((void **)&q);
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -649,11 +661,11 @@ static int scf_grid(myqc_ctx *ctx, const BatchState &b) {
 }
 
 static void launch_batch_scf(myqc_ctx *ctx, BatchState &b, bool guess, double e_tol, double d_tol, int max_iter,
-                             int threads) {
+                             int threads, const int *order = nullptr, int count = -1) {
     BScfArgs a;
     a.mols = b.d_mols;
-    a.order = b.d_order;
-    a.nmol = b.nmol;
+    a.order = order ? order : b.d_order;
+    a.nmol = count >= 0 ? count : b.nmol;
     a.counter = b.d_counter;
     a.eri = b.d_eri;
     a.S = b.d_S;
@@ -669,7 +681,8 @@ static void launch_batch_scf(myqc_ctx *ctx, BatchState &b, bool guess, double e_
     a.d_tol = d_tol;
     a.max_iter = max_iter;
     k_zero_int<<<1, 64, 0, ctx->stream>>>(b.d_counter, 1);
-    k_batch_scf<<<scf_grid(ctx, b), threads, scf_lds_bytes(b.nmax, threads), ctx->stream>>>(a, b.nmax);
+    const int grid = std::max(1, std::min(scf_grid(ctx, b), a.nmol));
+    k_batch_scf<<<grid, threads, scf_lds_bytes(b.nmax, threads), ctx->stream>>>(a, b.nmax);
 }
 
 // the whole pipeline: integrals, then the SCF
@@ -810,6 +823,45 @@ extern "C" int myqc_batch_set(myqc_ctx *ctx, int nmol, const int *mol_nnuc, cons
         mat0 += (long long)norb * norb;
         b.nmax = std::max(b.nmax, norb);
     }
+    // atom pairs A <= B for the learned guess (H, He: 1 function; Li..Ne: 5,
+    // atoms' functions contiguous in atom order)
+    std::vector<PairRec> prs;
+    std::vector<int> mol_atoms(2 * nmol);
+    b.guess_ok = true;
+    for (int m = 0; m < nmol; ++m) {
+        const BMol &mol = b.mols[m];
+        mol_atoms[2 * m] = mol.atom0;
+        mol_atoms[2 * m + 1] = mol.nnuc;
+        std::vector<int> ao(mol.nnuc), nf(mol.nnuc);
+        int a0 = 0;
+        for (int a = 0; a < mol.nnuc; ++a) {
+            const int z = Z[mol.atom0 + a];
+            if (z < 1 || z > 10) b.guess_ok = false;
+            nf[a] = z <= 2 ? 1 : 5;
+            ao[a] = a0;
+            a0 += nf[a];
+        }
+        if (a0 != mol.n) b.guess_ok = false;
+        for (int a = 0; a < mol.nnuc; ++a)
+            for (int c = a; c < mol.nnuc; ++c) {
+                PairRec r;
+                r.mol = m;
+                r.a = mol.atom0 + a;
+                r.b = mol.atom0 + c;
+                r.aoA = ao[a];
+                r.aoB = ao[c];
+                r.nA = nf[a];
+                r.nB = nf[c];
+                r.n = mol.n;
+                r.mat0 = mol.mat0;
+                prs.push_back(r);
+                b.pair_mol_of.push_back(m);
+            }
+    }
+    b.npair = (int)prs.size();
+    MYQC_CHECK(upload(ctx, &b.d_pairrec, prs));
+    MYQC_CHECK(upload(ctx, &b.d_mol_atoms, mol_atoms));
+    MYQC_CHECK(dalloc(ctx, (void **)&b.d_fail, sizeof(int) * nmol));
     b.nmol = nmol;
     b.npairs = (int)pairs.size();
     b.total_eri = eri0;
@@ -913,5 +965,90 @@ extern "C" int myqc_batch_info(myqc_ctx *ctx, int *eri_threads, int *scf_threads
     if (scf_threads) *scf_threads = b.scf_threads;
     if (graph_replays) *graph_replays = b.replays;
     if (n_integrals) *n_integrals = b.total_eri;
+    return 0;
+}
+
+extern "C" int myqc_batch_pair_data(myqc_ctx *ctx, int *npair, float *features, float *targets, float *mask) {
+    if (!ctx || !npair) return MYQC_E_ARG;
+    if (!ctx->batch || ctx->batch->nmol == 0) return set_error(ctx, MYQC_E_STATE, "myqc_batch_pair_data: no batch set");
+    BatchState &b = *ctx->batch;
+    if (!b.guess_ok)
+        return set_error(ctx, MYQC_E_ARG, "myqc_batch_pair_data: the learned guess covers H..Ne with STO-3G only");
+    *npair = b.npair;
+    if (!features && !targets && !mask) return 0;
+    MYQC_CHECK(guess_export(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.d_D, features, targets, mask));
+    if (mask) {
+        // pairs of molecules that did not converge carry no target
+        std::vector<int> conv(b.nmol);
+        MYQC_HIP(ctx, d2h(ctx, conv.data(), b.d_conv, sizeof(int) * b.nmol));
+        for (int p = 0; p < b.npair; ++p)
+            if (conv[b.pair_mol_of[p]] != 1)
+                for (int c = 0; c < kGuessOut; ++c) mask[(size_t)p * kGuessOut + c] = 0.f;
+    }
+    return 0;
+}
+
+extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol, int max_iter, double *energy,
+                                      int *iterations, int *converged, int *fallback) {
+    if (!ctx || !energy || e_tol <= 0 || d_tol <= 0 || max_iter < 1) return MYQC_E_ARG;
+    if (!ctx->batch || ctx->batch->nmol == 0)
+        return set_error(ctx, MYQC_E_STATE, "myqc_batch_rhf_learned: no batch set");
+    BatchState &b = *ctx->batch;
+    if (!b.guess_ok)
+        return set_error(ctx, MYQC_E_ARG, "myqc_batch_rhf_learned: the learned guess covers H..Ne with STO-3G only");
+    MYQC_CHECK(guess_reserve(ctx, b.npair));
+    if (!b.tuned) MYQC_CHECK(batch_tune(ctx, b, false, e_tol, d_tol, max_iter));
+    if (!b.graph_learned || b.gl_etol != e_tol || b.gl_dtol != d_tol || b.gl_maxit != max_iter ||
+        b.gl_gen != ctx->mlp_gen) {
+        if (b.graph_learned) (void)hipGraphExecDestroy(b.graph_learned);
+        b.graph_learned = nullptr;
+        hipGraph_t g;
+        MYQC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        k_batch_one_e<<<(b.npairs + 127) / 128, 128, 0, ctx->stream>>>(b.d_pairs, b.d_prims, b.d_ftT, b.d_atoms,
+                                                                       b.d_pair_mol, b.d_mols, b.npairs, b.d_S, b.d_H);
+        for (int c = 0; c < 6; ++c)
+            if (b.nq[c] > 0) launch_batch_eri(ctx, b, c, b.eri_threads[c]);
+        guess_launch(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.d_guess);
+        launch_batch_scf(ctx, b, true, e_tol, d_tol, max_iter, b.scf_threads);
+        MYQC_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
+        hipError_t e = hipGraphInstantiate(&b.graph_learned, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b.graph_learned = nullptr;
+            return set_error(ctx, MYQC_E_HIP, "myqc_batch_rhf_learned: graph instantiation failed: %s",
+                             hipGetErrorString(e));
+        }
+        b.gl_etol = e_tol;
+        b.gl_dtol = d_tol;
+        b.gl_maxit = max_iter;
+        b.gl_gen = ctx->mlp_gen;
+    }
+    MYQC_HIP(ctx, hipGraphLaunch(b.graph_learned, ctx->stream));
+    ++b.replays;
+    std::vector<int> conv(b.nmol), it1(b.nmol);
+    MYQC_HIP(ctx, d2h(ctx, conv.data(), b.d_conv, sizeof(int) * b.nmol));
+    MYQC_HIP(ctx, d2h(ctx, it1.data(), b.d_iters, sizeof(int) * b.nmol));
+    // fallback: molecules the learned start did not bring to convergence
+    // restart from the core-Hamiltonian guess
+    std::vector<int> fail;
+    for (int m = 0; m < b.nmol; ++m)
+        if (conv[m] == 0) fail.push_back(m);
+    if (!fail.empty()) {
+        MYQC_HIP(ctx, hipMemcpyAsync(b.d_fail, fail.data(), sizeof(int) * fail.size(), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        launch_batch_scf(ctx, b, false, e_tol, d_tol, max_iter, b.scf_threads, b.d_fail, (int)fail.size());
+        MYQC_HIP(ctx, hipGetLastError());
+        MYQC_HIP(ctx, d2h(ctx, conv.data(), b.d_conv, sizeof(int) * b.nmol));
+    }
+    std::vector<int> it2(b.nmol);
+    MYQC_HIP(ctx, d2h(ctx, it2.data(), b.d_iters, sizeof(int) * b.nmol));
+    MYQC_HIP(ctx, d2h(ctx, energy, b.d_energy, sizeof(double) * b.nmol));
+    std::vector<int> fb(b.nmol, 0);
+    for (int m : fail) fb[m] = 1;
+    for (int m = 0; m < b.nmol; ++m) {
+        if (iterations) iterations[m] = fb[m] ? it1[m] + it2[m] : it1[m];
+        if (converged) converged[m] = conv[m];
+        if (fallback) fallback[m] = fb[m];
+    }
     return 0;
 }

@@ -40,6 +40,7 @@ extern "C" int myqc_finalize(myqc_ctx *ctx) {
     free_eri(ctx);
     free_scf(ctx);
     free_batch(ctx);
+    free_mlp(ctx);
     void **ptrs[] = {(void **)&ctx->d_pairs, (void **)&ctx->d_prims, (void **)&ctx->d_ftT, (void **)&ctx->d_schwarz,
                      (void **)&ctx->d_D,     (void **)&ctx->d_A,     (void **)&ctx->d_G,   (void **)&ctx->d_D32,
                      (void **)&ctx->d_A32,   (void **)&ctx->d_iajb};

@@ -27,6 +27,7 @@ struct QuartetClass {
 
 struct ScfBuffers;
 struct BatchState;
+struct MlpState;
 
 enum TimerCat { T_ERI = 0, T_JK = 1, T_AO2MO = 2, T_MP2 = 3, T_NCAT = 4 };
 
@@ -106,6 +107,8 @@ struct myqc_ctx {
     // last RHF run: purification steps and diagonalisation fallbacks
     myqc::ScfBuffers *scf = nullptr;       // RHF work arrays, kept across calls
     myqc::BatchState *batch = nullptr;     // batched small-molecule RHF (batch.hip)
+    myqc::MlpState *mlp = nullptr;         // learned-guess network (mlp.hip)
+    int mlp_gen = 0;                      // bumped whenever the network's buffers move
     int scf_purify = 0, scf_diag = 0;
     int scf_sp2_steps = 0;                // steps the last purification took
 
@@ -163,5 +166,6 @@ int ensure_work(myqc_ctx *ctx);
 void free_eri(myqc_ctx *ctx);
 void free_scf(myqc_ctx *ctx);
 void free_batch(myqc_ctx *ctx);
+void free_mlp(myqc_ctx *ctx);
 
 }  // namespace myqc

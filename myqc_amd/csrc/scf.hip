@@ -351,8 +351,8 @@ static bool diis_coeffs(const std::vector<double> &B, int m, std::vector<double>
 // precision FP32, F_k = F_{k-1} + G32(D_k - D_{k-1}) between FP64 builds every
 // 8 iterations, and convergence is only accepted on two FP64 builds in a row.
 static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc, const double *C_guess,
-                   double e_tol, double d_tol, int max_iter, int precision, double *energy, double *C_out,
-                   double *eig_out, int *iters_out) {
+                   const double *D_guess, double e_tol, double d_tol, int max_iter, int precision, double *energy,
+                   double *C_out, double *eig_out, int *iters_out) {
     const int n = ctx->n;
     const size_t nn = (size_t)n * n;
     const int MD = ScfBuffers::kDiis;
@@ -365,9 +365,13 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
     MYQC_CHECK(inv_sqrt_ns(ctx, b, n, &ok));
     if (!ok) MYQC_CHECK(inv_sqrt_eig(ctx, b, n));
     // guess density
-    if (C_guess) {
-        MYQC_HIP(ctx, hipMemcpyAsync(b.C, C_guess, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
-        MYQC_CHECK(density(ctx, b.C, nocc, b.D, n));
+    if (C_guess || D_guess) {
+        if (C_guess) {
+            MYQC_HIP(ctx, hipMemcpyAsync(b.C, C_guess, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+            MYQC_CHECK(density(ctx, b.C, nocc, b.D, n));
+        } else {
+            MYQC_HIP(ctx, hipMemcpyAsync(b.D, D_guess, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+        }
         // Do = S^{1/2} D S^{1/2} / 2
         MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, b.D, n, b.Xi, n, 0.0, b.T1, n));
         MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 0.5, b.Xi, n, b.T1, n, 0.0, b.Do, n));
@@ -511,5 +515,22 @@ extern "C" int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int
     if (!ctx->have_eri || (precision == MYQC_FP32 && !ctx->have_eri32))
         return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf: call myqc_eri_build first");
     ctx->scf_purify = ctx->scf_diag = ctx->scf_sp2_steps = 0;
-    return rhf_scf(ctx, H, S, nocc, e_nuc, C_guess, e_tol, d_tol, max_iter, precision, energy, C, eig, iterations);
+    return rhf_scf(ctx, H, S, nocc, e_nuc, C_guess, nullptr, e_tol, d_tol, max_iter, precision, energy, C, eig,
+                   iterations);
+}
+
+extern "C" int myqc_rhf_scf_density(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
+                                    const double *D_guess, double e_tol, double d_tol, int max_iter, int precision,
+                                    double *energy, double *C, double *eig, int *iterations) {
+    if (!ctx || !H || !S || !D_guess) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf_density: no system set");
+    if (nocc < 0 || nocc > ctx->n || max_iter < 1 || e_tol <= 0 || d_tol <= 0)
+        return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf_density: invalid arguments");
+    if (precision != MYQC_FP64 && precision != MYQC_FP32)
+        return set_error(ctx, MYQC_E_ARG, "myqc_rhf_scf_density: precision %d", precision);
+    if (!ctx->have_eri || (precision == MYQC_FP32 && !ctx->have_eri32))
+        return set_error(ctx, MYQC_E_STATE, "myqc_rhf_scf_density: call myqc_eri_build first");
+    ctx->scf_purify = ctx->scf_diag = ctx->scf_sp2_steps = 0;
+    return rhf_scf(ctx, H, S, nocc, e_nuc, nullptr, D_guess, e_tol, d_tol, max_iter, precision, energy, C, eig,
+                   iterations);
 }

@@ -21,6 +21,7 @@ FP64, FP32, BF16X3 = 0, 1, 2
 T_ERI, T_JK, T_AO2MO, T_MP2 = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
@@ -40,11 +41,23 @@ EXPORTS = {
     "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
     "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
                       ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
+    "myqc_rhf_scf_density": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
     "myqc_scf_info": ([_vp, _ip, _ip], ctypes.c_int),
     "myqc_batch_set": ([_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _ip, _dp, _ip, _ip, _dp], ctypes.c_int),
     "myqc_batch_rhf": ([_vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip], ctypes.c_int),
     "myqc_batch_density": ([_vp, _dp], ctypes.c_int),
     "myqc_batch_info": ([_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "myqc_mlp_create": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong], ctypes.c_int),
+    "myqc_mlp_params": ([_vp, _fp, _fp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "myqc_mlp_forward": ([_vp, _fp, ctypes.c_int, _fp], ctypes.c_int),
+    "myqc_mlp_gradient": ([_vp, _fp, _fp, _fp, ctypes.c_int, _fp, _fp], ctypes.c_int),
+    "myqc_mlp_train": ([_vp, _fp, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                        ctypes.c_ulonglong, _fp], ctypes.c_int),
+    "myqc_batch_pair_data": ([_vp, _ip, _fp, _fp, _fp], ctypes.c_int),
+    "myqc_guess_density": ([_vp, ctypes.c_int, _ip, _dp, _dp], ctypes.c_int),
+    "myqc_batch_rhf_learned": ([_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip, _ip],
+                               ctypes.c_int),
     "myqc_timer_reset": ([_vp, ctypes.c_int], ctypes.c_int),
     "myqc_timer_get": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp, _dp], ctypes.c_int),
 }
@@ -204,24 +217,30 @@ class Context:
         return a.value, b.value, t.value
 
     def rhf(self, H, S, nocc: int, e_nuc: float, C_guess=None, e_tol=1e-10, d_tol=1e-8, max_iter=200,
-            precision: int = FP64):
-        """In-process RHF; returns dict(E, C, eig, iterations)."""
+            precision: int = FP64, D_guess=None):
+        """In-process RHF from the core-Hamiltonian guess, orbitals C_guess or
+        an AO density D_guess; returns dict(E, C, eig, iterations, ...)."""
         n = self.n
         Hf, Sf = _f(H), _f(S)
-        Cg = _f(C_guess) if C_guess is not None else None
         C = np.zeros((n, n), order="F")
         eig = np.zeros(n)
         E = ctypes.c_double()
         it = ctypes.c_int()
-        self._chk(self.L.myqc_rhf_scf(self.h, _p(Hf), _p(Sf), nocc, e_nuc, _p(Cg) if Cg is not None else None,
-                                      e_tol, d_tol, max_iter, precision, ctypes.byref(E), _p(C), _p(eig),
-                                      ctypes.byref(it)))
+        if D_guess is not None:
+            Dg = _f(D_guess)
+            self._chk(self.L.myqc_rhf_scf_density(self.h, _p(Hf), _p(Sf), nocc, e_nuc, _p(Dg), e_tol, d_tol,
+                                                  max_iter, precision, ctypes.byref(E), _p(C), _p(eig),
+                                                  ctypes.byref(it)))
+        else:
+            Cg = _f(C_guess) if C_guess is not None else None
+            self._chk(self.L.myqc_rhf_scf(self.h, _p(Hf), _p(Sf), nocc, e_nuc, _p(Cg) if Cg is not None else None,
+                                          e_tol, d_tol, max_iter, precision, ctypes.byref(E), _p(C), _p(eig),
+                                          ctypes.byref(it)))
         npur, ndiag = ctypes.c_int(), ctypes.c_int()
         self._chk(self.L.myqc_scf_info(self.h, ctypes.byref(npur), ctypes.byref(ndiag)))
         return dict(E=E.value, C=C, eig=eig, iterations=it.value, purifications=npur.value,
                     diag_fallbacks=ndiag.value)
 
-    # -- timing -------------------------------------------------------------
     # -- batched small molecules -------------------------------------------
     def batch_set(self, molecules, ftab=None):
         """molecules: list of (Z, xyz_bohr[, charge]); each one's reference
@@ -289,6 +308,72 @@ class Context:
                                          ctypes.byref(nint)))
         return dict(eri_threads=eri.tolist(), scf_threads=scf.value, graph_replays=rep.value,
                     n_integrals=nint.value)
+
+    # -- learned guess --------------------------------------------------------
+    def mlp_create(self, n_in: int, width: int, n_out: int, seed: int = 0):
+        self._chk(self.L.myqc_mlp_create(self.h, n_in, width, n_out, seed))
+
+    def mlp_params(self, set_to=None):
+        cnt = ctypes.c_longlong()
+        self._chk(self.L.myqc_mlp_params(self.h, None, None, ctypes.byref(cnt)))
+        if set_to is not None:
+            p = np.ascontiguousarray(set_to, dtype=np.float32)
+            assert p.size == cnt.value, (p.size, cnt.value)
+            self._chk(self.L.myqc_mlp_params(self.h, None, _p(p, _fp), None))
+        out = np.zeros(cnt.value, np.float32)
+        self._chk(self.L.myqc_mlp_params(self.h, _p(out, _fp), None, None))
+        return out
+
+    def mlp_forward(self, X, n_out: int):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], n_out), np.float32)
+        self._chk(self.L.myqc_mlp_forward(self.h, _p(X, _fp), X.shape[0], _p(Y, _fp)))
+        return Y
+
+    def mlp_gradient(self, X, T, M):
+        X, T, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, T, M))
+        cnt = ctypes.c_longlong()
+        self._chk(self.L.myqc_mlp_params(self.h, None, None, ctypes.byref(cnt)))
+        g = np.zeros(cnt.value, np.float32)
+        loss = ctypes.c_float()
+        self._chk(self.L.myqc_mlp_gradient(self.h, _p(X, _fp), _p(T, _fp), _p(M, _fp), X.shape[0],
+                                           ctypes.byref(loss), _p(g, _fp)))
+        return loss.value, g
+
+    def mlp_train(self, X, T, M, epochs: int, batch: int, lr: float, seed: int = 0):
+        X, T, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, T, M))
+        hist = np.zeros(epochs, np.float32)
+        self._chk(self.L.myqc_mlp_train(self.h, _p(X, _fp), _p(T, _fp), _p(M, _fp), X.shape[0], epochs, batch, lr,
+                                        seed, _p(hist, _fp)))
+        return hist
+
+    def guess_density(self, Z, xyz_bohr):
+        """The network's density for one molecule (computed on the device)."""
+        Zc = np.ascontiguousarray(Z, dtype=np.int32)
+        x = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3).ravel(order="F").copy()
+        n = int(sum(1 if z <= 2 else 5 for z in Zc))
+        D = np.zeros((n, n), order="F")
+        self._chk(self.L.myqc_guess_density(self.h, len(Zc), _p(Zc, _ip), _p(x), _p(D)))
+        return D
+
+    def batch_pair_data(self, n_feat: int = 200, n_out: int = 16):
+        npair = ctypes.c_int()
+        self._chk(self.L.myqc_batch_pair_data(self.h, ctypes.byref(npair), None, None, None))
+        X = np.zeros((npair.value, n_feat), np.float32)
+        T = np.zeros((npair.value, n_out), np.float32)
+        M = np.zeros((npair.value, n_out), np.float32)
+        self._chk(self.L.myqc_batch_pair_data(self.h, ctypes.byref(npair), _p(X, _fp), _p(T, _fp), _p(M, _fp)))
+        return X, T, M
+
+    def batch_rhf_learned(self, e_tol=1e-10, d_tol=1e-8, max_iter=100):
+        """Batch SCF from the network's densities, with core-H fallback.
+        Returns (energies, iterations, converged, fallback)."""
+        nm = len(self.batch_norb)
+        E = np.zeros(nm)
+        it, cv, fb = (np.zeros(nm, np.int32) for _ in range(3))
+        self._chk(self.L.myqc_batch_rhf_learned(self.h, e_tol, d_tol, max_iter, _p(E), _p(it, _ip), _p(cv, _ip),
+                                                _p(fb, _ip)))
+        return E, it, cv, fb
 
     def timer_reset(self, enable: bool = True):
         self._chk(self.L.myqc_timer_reset(self.h, 1 if enable else 0))

@@ -176,3 +176,26 @@ def test_p_shell_rotation_covariance():
     U[2:4, 2:4] = np.array([[0, 1], [-1, 0.0]])
     X1r = np.einsum("pi,qj,rk,sl,pqrs->ijkl", U, U, U, U, X1)
     assert np.max(np.abs(X1r - X2)) < 1e-6   # Boys table anchors move with T, so ~1e-7
+
+
+def test_mlp_reference_gradient_matches_finite_differences():
+    """The numpy network used to check the GPU training kernels, against
+    central differences of its own loss (float64)."""
+    from oracle import mlp
+    rng = np.random.default_rng(5)
+    n_in, width, n_out = 7, 5, 3
+    npar = (n_in * width + width) + 2 * (width * width + width) + (width * n_out + n_out)
+    p = rng.standard_normal(npar) * 0.5
+    X = rng.standard_normal((11, n_in))
+    T = rng.standard_normal((11, n_out))
+    Mk = (rng.uniform(size=(11, n_out)) < 0.7).astype(float)
+    L, g = mlp.loss_grad(p, X, T, Mk, n_in, width, n_out)
+    assert abs(L - np.sum(Mk * (mlp.forward(p, X, n_in, width, n_out) - T) ** 2) / Mk.sum()) <= 1e-14
+    h = 1e-6
+    fd = np.zeros(npar)
+    for k in range(npar):
+        e = np.zeros(npar)
+        e[k] = h
+        fd[k] = (mlp.loss_grad(p + e, X, T, Mk, n_in, width, n_out)[0]
+                 - mlp.loss_grad(p - e, X, T, Mk, n_in, width, n_out)[0]) / (2 * h)
+    assert np.max(np.abs(fd - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
