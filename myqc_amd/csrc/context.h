@@ -7,6 +7,7 @@
 
 #include <cstdio>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/myqc_hip.h"
@@ -112,6 +113,9 @@ struct myqc_ctx {
     int scf_purify = 0, scf_diag = 0;
     int scf_sp2_steps = 0;                // steps the last purification took
 
+    // capacity of each device allocation made through dalloc
+    std::unordered_map<void *, size_t> dcap;
+
     // timing
     bool timing = false;
     std::vector<myqc::EventRec> events;
@@ -163,7 +167,7 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32);
 int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX);
 int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision);
 int ensure_work(myqc_ctx *ctx);
-void free_eri(myqc_ctx *ctx);
+void free_eri(myqc_ctx *ctx, bool keep_memory = false);
 void free_scf(myqc_ctx *ctx);
 void free_batch(myqc_ctx *ctx);
 void free_mlp(myqc_ctx *ctx);

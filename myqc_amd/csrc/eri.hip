@@ -32,9 +32,16 @@ int set_error(myqc_ctx *ctx, int code, const char *fmt, ...) {
     return code;
 }
 
+// Device allocations keep their capacity: a buffer is only replaced when a
+// request outgrows it, so repeated solves of same-sized systems reuse their
+// memory instead of paying hipFree/hipMalloc (and the 1-2 GB integral store)
+// every time.
 int dalloc(myqc_ctx *ctx, void **p, size_t bytes) {
     if (*p) {
-        hipFree(*p);
+        auto it = ctx->dcap.find(*p);
+        if (bytes > 0 && it != ctx->dcap.end() && it->second >= bytes) return 0;
+        if (it != ctx->dcap.end()) ctx->dcap.erase(it);
+        (void)hipFree(*p);
         *p = nullptr;
     }
     if (bytes == 0) return 0;
@@ -43,6 +50,7 @@ int dalloc(myqc_ctx *ctx, void **p, size_t bytes) {
         *p = nullptr;
         return set_error(ctx, MYQC_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     }
+    ctx->dcap[*p] = bytes;
     return 0;
 }
 
@@ -53,7 +61,7 @@ hipError_t d2h(myqc_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 
 void dfree(void **p) {
-    if (*p) hipFree(*p);
+    if (*p) (void)hipFree(*p);
     *p = nullptr;
 }
 
@@ -241,9 +249,11 @@ static const int kCB[6] = {0, 1, 1, 2, 2, 2};
 static const int kCK[6] = {0, 0, 1, 0, 1, 2};
 static const int kBLK[6] = {1, 3, 9, 9, 27, 81};
 
-void free_eri(myqc_ctx *ctx) {
+// Drops the stored integrals; with keep_memory the device buffers stay
+// allocated (their capacity is reused by the next build).
+void free_eri(myqc_ctx *ctx, bool keep_memory) {
     for (auto &c : ctx->cls) {
-        dfree((void **)&c.d_cum);
+        if (!keep_memory) dfree((void **)&c.d_cum);
         c.nq = 0;
     }
     void **ptrs[] = {(void **)&ctx->d_segoff, (void **)&ctx->d_rank,     (void **)&ctx->d_x64,
@@ -251,8 +261,10 @@ void free_eri(myqc_ctx *ctx) {
                      (void **)&ctx->d_unit_off, (void **)&ctx->d_shell, (void **)&ctx->d_ao_shell,
                      (void **)&ctx->d_partial, (void **)&ctx->d_partial32, (void **)&ctx->d_Dpair,
                      (void **)&ctx->d_Dpair32};
-    for (auto p : ptrs) dfree(p);
-    for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
+    if (!keep_memory) {
+        for (auto p : ptrs) dfree(p);
+        for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
+    }
     ctx->n_wg = ctx->n_units = 0;
     ctx->n_expanded = 0;
     ctx->have_eri = false;
@@ -526,12 +538,13 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
 
 int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     MYQC_CHECK(eri_schwarz(ctx));
-    free_eri(ctx);
+    free_eri(ctx, true);
     MYQC_CHECK(build_classes(ctx, tol));
     MYQC_CHECK(build_expanded(ctx, tol));
     const size_t nx = (size_t)std::max<long long>(ctx->n_expanded, 1);
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x64, sizeof(double) * nx));
     if (want32) MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x32, sizeof(float) * nx));
+    else dfree((void **)&ctx->d_x32);   // a kept buffer may be smaller than this build
     const size_t part = (size_t)std::max(ctx->n_wg, 1) * 3 * ctx->n;
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * part));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial32, sizeof(float) * part));
@@ -595,7 +608,7 @@ extern "C" int myqc_set_system(myqc_ctx *ctx, int nnuc, const int *Z, const doub
     if (nnuc <= 0 || nset <= 0 || norb <= 0 || !Z || !xyz_bohr || !set_alpha || !setinfo || !bas || !basinfo || !Ftab)
         return set_error(ctx, MYQC_E_ARG, "myqc_set_system: invalid arguments");
     if (flags != MYQC_REF_COMPAT) return set_error(ctx, MYQC_E_ARG, "myqc_set_system: unknown flags %d", flags);
-    free_eri(ctx);
+    free_eri(ctx, true);
     ctx->have_system = false;
     ctx->have_schwarz = false;
     ctx->n = norb;

@@ -145,7 +145,7 @@ struct ScfBuffers {
         for (auto m : mats) MYQC_CHECK(dalloc(ctx, (void **)m, sizeof(double) * nn));
         MYQC_CHECK(dalloc(ctx, (void **)&eig, sizeof(double) * nb));
         MYQC_CHECK(dalloc(ctx, (void **)&work, sizeof(double) * nb));
-        MYQC_CHECK(dalloc(ctx, (void **)&dsc, sizeof(double) * 8));
+        MYQC_CHECK(dalloc(ctx, (void **)&dsc, sizeof(double) * 16));
         MYQC_CHECK(dalloc(ctx, (void **)&lohi, sizeof(double) * 2 * nb));
         MYQC_CHECK(dalloc(ctx, (void **)&part, sizeof(double) * kRedJobs * kRedBlocks));
         MYQC_CHECK(dalloc(ctx, (void **)&red, sizeof(double) * kRedJobs));

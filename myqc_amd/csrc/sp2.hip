@@ -12,9 +12,11 @@
 // in its epilogue, writes the next P and accumulates that P's trace and
 // Frobenius norm for the following step.  Three rotating scalar slots carry
 // the traces between launches without host synchronisation.
+#include <algorithm>
 #include <cmath>
 
 #include "context.h"
+
 
 namespace myqc {
 
@@ -57,6 +59,7 @@ __global__ __launch_bounds__(1024) void k_gersh_reduce(const double *__restrict_
         slots[6] = smin[0];
         slots[7] = smax[0];
         for (int s = 0; s < 6; ++s) slots[s] = 0.0;
+        reinterpret_cast<int *>(slots + 8)[0] = -1;   // purification control word
     }
 }
 
@@ -96,20 +99,20 @@ __global__ __launch_bounds__(64) void k_sp2_start(const double *__restrict__ F, 
 // LDS.  Lane l feeds A row l%16 and B column l%16 at k offset l/16 and
 // receives D rows l/16 + 4v, column l%16, v = 0..3 (the layout the GPU test of
 // the builtin checks).
-constexpr int KS = 4;
+constexpr int KS = 8;
 
-__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
-                                                          int n, double nocc, double *__restrict__ slots, int step) {
-    __shared__ double s_acc[KS - 1][4][64];
+// Tile `tile` of step k (see k_sp2_step_mfma); s_acc: (KS - 1) x 4 x 64 doubles of LDS
+__device__ __forceinline__ void sp2_tile(const double *__restrict__ P, double *__restrict__ Pn, int n, double nocc,
+                                         double *__restrict__ slots, int step, int tile, double (*s_acc)[4][64]) {
     const int tiles = (n + 15) >> 4;
-    const int ti = blockIdx.x % tiles, tj = blockIdx.x / tiles;
+    const int ti = tile % tiles, tj = tile / tiles;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
     const int ia = ti * 16 + r;     // row of P fed as A
     const int jb = tj * 16 + r;     // column of P fed as B (read as a row: P is symmetric)
     const int kc = ((n + 4 * KS - 1) / (4 * KS)) * 4;
     const int kb = w * kc, ke = min(n, kb + kc);
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 3
+#pragma unroll 9
     for (int k0 = kb; k0 < ke; k0 += 4) {
         const int k = k0 + kq;
         const double a = (ia < n && k < ke) ? P[ia + (size_t)n * k] : 0.0;
@@ -121,38 +124,74 @@ __global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restr
         for (int v = 0; v < 4; ++v) s_acc[w - 1][v][lane] = acc[v];
     }
     __syncthreads();
-    if (w > 0) return;
+    if (w == 0) {
 #pragma unroll
-    for (int q = 0; q < KS - 1; ++q)
+        for (int q = 0; q < KS - 1; ++q)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[v] += s_acc[q][v][lane];
-    const double *sl = slots + 2 * (step % 3);
-    const double tr = sl[0], fro = sl[1];
-    const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
-    double t = 0.0, f = 0.0;
+            for (int v = 0; v < 4; ++v) acc[v] += s_acc[q][v][lane];
+        const double *sl = slots + 2 * (step % 3);
+        const double tr = sl[0], fro = sl[1];
+        const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
+        double t = 0.0, f = 0.0;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int row = ti * 16 + kq + 4 * v, col = tj * 16 + r;
-        if (row < n && col < n) {
-            const size_t e = row + (size_t)n * col;
-            const double p2 = acc[v];
-            const double nw = square ? p2 : 2.0 * P[e] - p2;
-            Pn[e] = nw;
-            f += nw * nw;
-            if (row == col) t += nw;
+        for (int v = 0; v < 4; ++v) {
+            const int row = ti * 16 + kq + 4 * v, col = tj * 16 + r;
+            if (row < n && col < n) {
+                const size_t e = row + (size_t)n * col;
+                const double p2 = acc[v];
+                const double nw = square ? p2 : 2.0 * P[e] - p2;
+                Pn[e] = nw;
+                f += nw * nw;
+                if (row == col) t += nw;
+            }
+        }
+        t = wsum(t);
+        f = wsum(f);
+        if (lane == 0) {
+            double *nx = slots + 2 * ((step + 1) % 3);
+            unsafeAtomicAdd(&nx[0], t);
+            unsafeAtomicAdd(&nx[1], f);
+            if (tile == 0) {
+                double *cl = slots + 2 * ((step + 2) % 3);
+                cl[0] = 0.0;
+                cl[1] = 0.0;
+            }
         }
     }
-    t = wsum(t);
-    f = wsum(f);
-    if (lane == 0) {
-        double *nx = slots + 2 * ((step + 1) % 3);
-        unsafeAtomicAdd(&nx[0], t);
-        unsafeAtomicAdd(&nx[1], f);
-        if (blockIdx.x == 0) {
-            double *cl = slots + 2 * ((step + 2) % 3);
-            cl[0] = 0.0;
-            cl[1] = 0.0;
-        }
+    __syncthreads();   // s_acc is reused by the workgroup's next tile
+}
+
+// Convergence is decided on the device: step k meets the criterion when k >=
+// first and P_k is idempotent to 1e-8 with trace nocc to 1e-6 (tr P -
+// ||P||^2 = sum l(1-l) over P's eigenvalues; near convergence each step
+// squares it).  The first such step is recorded in ctrl[0]; two steps later
+// the projector is final and the remaining launched steps return at once.
+// All workgroups of a step read the same traces, so they agree.
+__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
+                                                          int n, double nocc, double *__restrict__ slots, int step,
+                                                          int first, int *__restrict__ ctrl) {
+    __shared__ double s_acc[KS - 1][4][64];
+    const double *sl = slots + 2 * (step % 3);
+    const double tr = sl[0], fro = sl[1];
+    const bool crit = step >= first && fabs(tr - fro) < 1e-8 && fabs(tr - nocc) < 1e-6;
+    const int seen = __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int cs = (seen >= 0 && seen < step) ? seen : (crit ? step : -1);
+    if (cs >= 0 && step >= cs + 2) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && seen < 0 && crit) ctrl[0] = step;
+    sp2_tile(P, Pn, n, nocc, slots, step, blockIdx.x, s_acc);
+}
+
+// After the launched steps: status = {converged, final step}; the final
+// projector (the input of step ctrl[0] + 2) is moved into P when it sits in W.
+__global__ void k_sp2_finish(double *__restrict__ P, const double *__restrict__ W, int n, int *__restrict__ ctrl) {
+    const int cs = ctrl[0];
+    const size_t nn = (size_t)n * n;
+    if (cs >= 0 && ((cs + 2) & 1))
+        for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x)
+            P[e] = W[e];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl[1] = cs >= 0 ? 1 : 0;
+        ctrl[2] = cs >= 0 ? cs + 2 : -1;
     }
 }
 
@@ -165,7 +204,8 @@ int gershgorin_bounds(myqc_ctx *ctx, const double *A, int n, double *lohi, doubl
 }
 
 // Projector onto the nocc lowest levels of the symmetric Fo (n x n) into P.
-// work: n x n scratch; lohi: 2n; slots: 8 doubles.  Returns 1 in *ok when
+// work: n x n scratch; lohi: 2n; slots: 16 doubles (traces, bounds, and the
+// control words at slots + 8).  Returns 1 in *ok when
 // purification converged (idempotency below 1e-8, then two more steps).
 // *steps carries the step count of the previous call in and this call's out:
 // successive SCF iterations need nearly the same count, so the host checks
@@ -178,30 +218,32 @@ int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double 
     const size_t nn = (size_t)n * n;
     k_sp2_start<<<(unsigned)((nn + 63) / 64), 64, 0, ctx->stream>>>(Fo, P, n, slots);
     const int tiles = (n + 15) / 16;
-    double *cur = P, *nxt = work;
-    int extra = -1;
-    const int first = *steps > 16 ? *steps - 4 : 12;
+    // launch about as many steps as the previous purification took (plus a
+    // margin); steps past convergence cost an empty launch, and only the
+    // status read at the end waits for the device
+    int *ctrl = reinterpret_cast<int *>(slots + 8);
+    const int first = 8, max_steps = 120;
+    int launched = 0, target = *steps > 0 ? std::min(*steps + 6, max_steps) : 48;
     *steps = 0;
-    for (int step = 0; step < 120; ++step) {
-        if (extra < 0 && step >= first && ((step - first) & 1) == 0) {
-            double sl[2];
-            MYQC_HIP(ctx, d2h(ctx, sl, slots + 2 * (step % 3), sizeof sl));
-            // tr P - ||P||^2 = sum l(1-l) over P's eigenvalues l; near
-            // convergence each step squares it, so two more steps finish
-            if (fabs(sl[0] - sl[1]) < 1e-8 && fabs(sl[0] - nocc) < 1e-6) extra = 2;
+    while (true) {
+        for (; launched < target; ++launched) {
+            const double *cur = (launched & 1) ? work : P;
+            double *nxt = (launched & 1) ? P : work;
+            k_sp2_step_mfma<<<tiles * tiles, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, slots, launched,
+                                                                       first, ctrl);
         }
-        if (extra == 0) {
+        k_sp2_finish<<<64, 256, 0, ctx->stream>>>(P, work, n, ctrl);
+        MYQC_HIP(ctx, hipGetLastError());
+        int st[2] = {0, 0};
+        MYQC_HIP(ctx, d2h(ctx, st, ctrl + 1, sizeof st));
+        if (st[0]) {
             *ok = 1;
-            *steps = step;
+            *steps = st[1];
             break;
         }
-        k_sp2_step_mfma<<<tiles * tiles, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, slots, step);
-        double *t = cur;
-        cur = nxt;
-        nxt = t;
-        if (extra > 0) --extra;
+        if (launched >= max_steps) break;
+        target = std::min(launched + 8, max_steps);
     }
-    if (cur != P) MYQC_HIP(ctx, hipMemcpyAsync(P, cur, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
 }

@@ -7,8 +7,9 @@ csrc/mlp.hip).  This module holds what surrounds them:
   * training data: seeded config-4 molecules (2-8 atoms from H..Ne on random
     trees) and perturbations of them (every atom displaced uniformly within
     0.1 A, i.e. bond changes up to +-0.2 A and angle changes of a few
-    degrees), solved by the batched GPU SCF, with the atom-pair features,
-    bond-frame density invariants and masks the library extracts;
+    degrees), plus seeded clusters of 2-5 waters at the density of the
+    40-water workload, solved by the batched GPU SCF, with the atom-pair
+    features, bond-frame density invariants and masks the library extracts;
   * training and saving/loading of the parameters (myqc_amd/data/guess_mlp.npz);
   * a numpy statement of the pair features, invariants and density
     reconstruction (the device kernels are checked against it; the product
@@ -47,6 +48,20 @@ def molecules(n_base, n_perturb, seed):
         Z, x = M.random_molecule(rng)
         for _ in range(n_perturb):
             out.append((Z, M.to_bohr(Z, perturb(rng, x))))
+    return out
+
+
+def water_clusters(count, seed, n_min=2, n_max=5):
+    """Seeded clusters of n_min..n_max waters at the density of the 40-water
+    workload (40 waters in a 12 A cube), each perturbed like the rest of the
+    training set (bohr)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nw = int(rng.integers(n_min, n_max + 1))
+        box = 12.0 * (nw / 40.0) ** (1.0 / 3.0)
+        Z, x = M.water_cluster(nw, seed=int(rng.integers(1 << 31)), box=box)
+        out.append((Z, M.to_bohr(Z, perturb(rng, x))))
     return out
 
 

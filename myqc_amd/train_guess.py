@@ -4,7 +4,8 @@
 
 Data: `base` seeded config-4 molecules (seed 20000) with `perturb` perturbed
 copies each (every atom displaced within 0.1 A): 20k geometries by default,
-solved by the batched GPU SCF from the core-Hamiltonian guess.  Training:
+plus `waters` seeded clusters of 2-5 waters (seed 20002), solved by the
+batched GPU SCF from the core-Hamiltonian guess.  Training:
 Adam on the library's own forward/backward kernels, half the epochs at
 lr 1e-3 and half at 2e-4.  Held-out check: 512 fresh molecules (seed 20001)
 solved from the core-Hamiltonian guess and from the network's densities.
@@ -41,13 +42,14 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--base", type=int, default=1000)
     ap.add_argument("--perturb", type=int, default=20)
+    ap.add_argument("--waters", type=int, default=4000, help="small water clusters added to the data")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--out", default=G.DATA)
     args = ap.parse_args(argv)
     with Context(0) as ctx:
         t0 = time.perf_counter()
-        mols = G.molecules(args.base, args.perturb, seed=20000)
+        mols = G.molecules(args.base, args.perturb, seed=20000) + G.water_clusters(args.waters, seed=20002)
         X, T, Mk, it, cf = G.dataset(ctx, mols)
         t_data = time.perf_counter() - t0
         print(f"[train] {len(mols)} geometries solved ({cf:.3f} converged, {it:.1f} iterations) -> {len(X)} pairs "
@@ -60,8 +62,10 @@ def main(argv=None):
         print(f"[train] loss {hist[0]:.3e} -> {hist[-1]:.3e} in {t_train:.1f} s", flush=True)
         ev = evaluate(ctx, G.molecules(512, 1, seed=20001))
         print("[train] held-out " + json.dumps(ev), flush=True)
+        evw = evaluate(ctx, G.water_clusters(256, seed=20003))
+        print("[train] held-out water clusters " + json.dumps(evw), flush=True)
         G.save(ctx, args.out, loss=hist, geometries=len(mols), pairs=len(X), train_seconds=t_train,
-               data_seconds=t_data, heldout=json.dumps(ev))
+               data_seconds=t_data, heldout=json.dumps(ev), heldout_waters=json.dumps(evw))
         print(f"[train] saved {args.out}", flush=True)
     return 0
 
