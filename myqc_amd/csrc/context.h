@@ -6,6 +6,8 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdio>
+#include <cstdlib>
+#include <ctime>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -160,6 +162,25 @@ void dfree(void **p);
 // timing helpers: bracket a launch sequence on ctx->stream
 int timer_begin(myqc_ctx *ctx, int cat, hipEvent_t *ev);
 int timer_end(myqc_ctx *ctx, int cat, hipEvent_t ev, double work);
+
+// Host-side phase timing to stderr when MYQC_HOST_TIMING is set (profiling
+// of the setup work around the kernels).
+struct HostClock {
+    bool on;
+    double t0;
+    static double now() {
+        timespec ts;
+        clock_gettime(CLOCK_MONOTONIC, &ts);
+        return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+    }
+    explicit HostClock(myqc_ctx *) : on(getenv("MYQC_HOST_TIMING") != nullptr), t0(now()) {}
+    void mark(const char *what) {
+        if (!on) return;
+        const double t = now();
+        fprintf(stderr, "[host] %-40s %8.3f ms\n", what, t - t0);
+        t0 = t;
+    }
+};
 
 // internals shared between translation units
 int eri_schwarz(myqc_ctx *ctx);

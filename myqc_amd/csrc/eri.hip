@@ -537,10 +537,14 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
 }
 
 int eri_store(myqc_ctx *ctx, double tol, bool want32) {
+    HostClock hc(ctx);
     MYQC_CHECK(eri_schwarz(ctx));
+    hc.mark("eri_store: Schwarz factors");
     free_eri(ctx, true);
     MYQC_CHECK(build_classes(ctx, tol));
+    hc.mark("eri_store: quartet classes");
     MYQC_CHECK(build_expanded(ctx, tol));
+    hc.mark("eri_store: segments and schedule");
     const size_t nx = (size_t)std::max<long long>(ctx->n_expanded, 1);
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x64, sizeof(double) * nx));
     if (want32) MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x32, sizeof(float) * nx));
@@ -558,7 +562,9 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
         MYQC_HIP(ctx, hipGetLastError());
     }
     MYQC_CHECK(timer_end(ctx, T_ERI, ev, ctx->eri_flops));
+    hc.mark("eri_store: allocations and launches");
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hc.mark("eri_store: kernels");
     ctx->have_eri = true;
     ctx->have_eri32 = want32;
     ctx->eri_tol = tol;
@@ -608,6 +614,7 @@ extern "C" int myqc_set_system(myqc_ctx *ctx, int nnuc, const int *Z, const doub
     if (nnuc <= 0 || nset <= 0 || norb <= 0 || !Z || !xyz_bohr || !set_alpha || !setinfo || !bas || !basinfo || !Ftab)
         return set_error(ctx, MYQC_E_ARG, "myqc_set_system: invalid arguments");
     if (flags != MYQC_REF_COMPAT) return set_error(ctx, MYQC_E_ARG, "myqc_set_system: unknown flags %d", flags);
+    HostClock hc(ctx);
     free_eri(ctx, true);
     ctx->have_system = false;
     ctx->have_schwarz = false;
@@ -620,6 +627,7 @@ extern "C" int myqc_set_system(myqc_ctx *ctx, int nnuc, const int *Z, const doub
     const std::string why = build_system(nnuc, ctx->xyz.data(), nset, set_alpha, setinfo, bas, norb, basinfo,
                                          ctx->shells, ctx->pairs, ctx->prims, ctx->pair_ids);
     if (!why.empty()) return set_error(ctx, MYQC_E_ARG, "myqc_set_system: %s", why.c_str());
+    hc.mark("set_system: shells and pairs");
     // Boys table transposed to [Tk][m] so the 7 Taylor rows are contiguous
     ctx->ftT.assign(FT_NT * FT_NM, 0.0);
     for (int tk = 0; tk < FT_NT; ++tk)
@@ -634,6 +642,7 @@ extern "C" int myqc_set_system(myqc_ctx *ctx, int nnuc, const int *Z, const doub
     MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_ftT, ctx->ftT.data(), sizeof(double) * ctx->ftT.size(), hipMemcpyHostToDevice,
                                  ctx->stream));
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hc.mark("set_system: uploads");
     ctx->have_system = true;
     ctx->work_n = -1;
     return 0;

@@ -197,3 +197,57 @@ extern "C" int hc_fock_G_rows(int nnuc, const double *xyz, int nset, const doubl
         for (size_t j = 0; j < n; ++j) G[i + n * j] = 0.5 * (Gr[i + n * j] + Gr[j + n * i]);
     return 0;
 }
+
+// The pair-owned digestion of the stored integrals, restated on the host:
+// every shell pair P owns a copy of (P|Q) for each ket Q (degeneracy-weighted
+// values); the copy adds J to P's block, and the four K terms only when P is
+// the bra of the unique quartet (higher class, or same class and rank(P) <=
+// rank(Q)); the diagonal quartet (P|P) also adds its ket-side J; G = A + A^T.
+extern "C" int hc_fock_G_pairs(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                               const double *bas, int norb, const int *basinfo, const double *Ftab, const double *D,
+                               double *G) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    const size_t n = norb;
+    std::vector<double> XX(n * n * n * n);
+    if (hc_eri_full(nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab, XX.data()) != 0) return -1;
+    std::vector<double> Ds(n * n), A(n * n, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) Ds[i + n * j] = 0.5 * (D[i + n * j] + D[j + n * i]);
+    auto X = [&](size_t i, size_t j, size_t k, size_t l) { return XX[i + n * (j + n * (k + n * l))]; };
+    std::vector<int> rank(s.pairs.size());
+    for (int c = 0; c < 3; ++c)
+        for (size_t r = 0; r < s.ids[c].size(); ++r) rank[s.ids[c][r]] = (int)r;
+    for (int P = 0; P < (int)s.pairs.size(); ++P) {
+        const ShellPair &p = s.pairs[P];
+        const int cp = pclass(p);
+        for (int ck = 0; ck < 3; ++ck)
+            for (int Q : s.ids[ck]) {
+                const ShellPair &q = s.pairs[Q];
+                const bool bra = cp > ck || (cp == ck && rank[P] <= rank[Q]);
+                const bool diag = P == Q;
+                const int na = p.la ? 3 : 1, nb = p.lb ? 3 : 1, nc = q.la ? 3 : 1, nd = q.lb ? 3 : 1;
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < nb; ++b)
+                        for (int c = 0; c < nc; ++c)
+                            for (int d = 0; d < nd; ++d) {
+                                const size_t i = p.ao_a + a, j = p.ao_b + b, k = q.ao_a + c, l = q.ao_b + d;
+                                const double w = block_weight(a, b, c, d, p.same, q.same, diag, nb, nd, (int)i,
+                                                              (int)j, (int)k, (int)l);
+                                if (w == 0.0) continue;
+                                const double x = X(i, j, k, l) * w;
+                                A[i + n * j] += 2.0 * Ds[k + n * l] * x;
+                                if (bra) {
+                                    A[i + n * k] -= 0.5 * Ds[j + n * l] * x;
+                                    A[j + n * k] -= 0.5 * Ds[i + n * l] * x;
+                                    A[i + n * l] -= 0.5 * Ds[j + n * k] * x;
+                                    A[j + n * l] -= 0.5 * Ds[i + n * k] * x;
+                                }
+                                if (diag) A[k + n * l] += 2.0 * Ds[i + n * j] * x;
+                            }
+            }
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) G[i + n * j] = A[i + n * j] + A[j + n * i];
+    return 0;
+}

@@ -107,3 +107,22 @@ def test_row_owned_digestion_formula(hc, name):
         G = np.zeros(n * n)
         assert hc.hc_fock_G_rows(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
         assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
+
+
+@pytest.mark.parametrize("name", ["co2", "h2o_2", "mol3"])
+def test_pair_owned_digestion_formula(hc, name):
+    """The pair-owned J/K split the digestion kernel uses (each shell pair's
+    copy adds J, and K only where the pair is the unique quartet's bra;
+    G = A + A^T) reproduces RHFI2G's G for the reference's density and for a
+    nonsymmetric one."""
+    f, Z, s = _sys(name)
+    n = s.n
+    rng = np.random.default_rng(11)
+    for D, ref in ((f["D_guess"], f["G_guess"]), (None, None)):
+        if D is None:
+            D = rng.standard_normal((n, n))
+            ref = O.fock_G(O.eri_full(s), D)
+        Df = np.asfortranarray(D).ravel(order="F").copy()
+        G = np.zeros(n * n)
+        assert hc.hc_fock_G_pairs(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
+        assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
