@@ -180,8 +180,8 @@ int myqc_batch_pair_data(myqc_ctx *ctx, int *npair, float *features, float *targ
 int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const double *xyz_bohr, double *D);
 /* myqc_batch_rhf started from the network's densities, computed on the
  * device inside the same captured graph; molecules that do not converge from
- * it are solved again from the core-Hamiltonian guess (fallback[m] = 1,
- * iterations[m] counts both runs). */
+ * it within min(max_iter, 48) iterations are solved again from the
+ * core-Hamiltonian guess (fallback[m] = 1, iterations[m] counts both runs). */
 int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol, int max_iter, double *energy, int *iterations,
                            int *converged, int *fallback);
 

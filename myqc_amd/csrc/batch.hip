@@ -998,7 +998,10 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
         return set_error(ctx, MYQC_E_ARG, "myqc_batch_rhf_learned: the learned guess covers H..Ne with STO-3G only");
     MYQC_CHECK(guess_reserve(ctx, b.npair));
     if (!b.tuned) MYQC_CHECK(batch_tune(ctx, b, false, e_tol, d_tol, max_iter));
-    if (!b.graph_learned || b.gl_etol != e_tol || b.gl_dtol != d_tol || b.gl_maxit != max_iter ||
+    // the learned start gets a shorter budget (typical solves take about 25
+    // iterations); what it leaves unconverged restarts from the core guess
+    const int learned_iter = std::min(max_iter, 48);
+    if (!b.graph_learned || b.gl_etol != e_tol || b.gl_dtol != d_tol || b.gl_maxit != learned_iter ||
         b.gl_gen != ctx->mlp_gen) {
         if (b.graph_learned) (void)hipGraphExecDestroy(b.graph_learned);
         b.graph_learned = nullptr;
@@ -1009,7 +1012,7 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
         for (int c = 0; c < 6; ++c)
             if (b.nq[c] > 0) launch_batch_eri(ctx, b, c, b.eri_threads[c]);
         guess_launch(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.d_guess);
-        launch_batch_scf(ctx, b, true, e_tol, d_tol, max_iter, b.scf_threads);
+        launch_batch_scf(ctx, b, true, e_tol, d_tol, learned_iter, b.scf_threads);
         MYQC_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
         hipError_t e = hipGraphInstantiate(&b.graph_learned, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -1020,7 +1023,7 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
         }
         b.gl_etol = e_tol;
         b.gl_dtol = d_tol;
-        b.gl_maxit = max_iter;
+        b.gl_maxit = learned_iter;
         b.gl_gen = ctx->mlp_gen;
     }
     MYQC_HIP(ctx, hipGraphLaunch(b.graph_learned, ctx->stream));

@@ -300,7 +300,10 @@ int eri_schwarz(myqc_ctx *ctx) {
     // sort each class by Schwarz factor, largest first (ties by pair id)
     for (int c = 0; c < 3; ++c) {
         auto &ids = ctx->pair_ids[c];
-        std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return ctx->schwarz[x] > ctx->schwarz[y]; });
+        std::vector<std::pair<double, int>> key(ids.size());
+        for (size_t r = 0; r < ids.size(); ++r) key[r] = {-ctx->schwarz[ids[r]], ids[r]};
+        std::sort(key.begin(), key.end());   // descending factor, ties by pair id
+        for (size_t r = 0; r < ids.size(); ++r) ids[r] = key[r].second;
         if (!ids.empty())
             MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_sorted[c], ids.data(), sizeof(int) * ids.size(),
                                          hipMemcpyHostToDevice, ctx->stream));
@@ -428,22 +431,31 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
             qsorted[c][r] = ctx->schwarz[ids[r]];
         }
     }
+    // screened ket count of every pair in each ket class: kets with
+    // q_P q_Q >= tol form a prefix of the Schwarz-sorted class list
+    std::vector<int> pcnt((size_t)np * 3);
+    for (int p = 0; p < np; ++p) {
+        const double qp = ctx->schwarz[p];
+        for (int ck = 0; ck < 3; ++ck) {
+            const auto &qk = qsorted[ck];
+            long long cnt;
+            if (tol <= 0.0) cnt = (long long)qk.size();
+            else if (qp <= 0.0) cnt = 0;
+            else
+                cnt = std::lower_bound(qk.begin(), qk.end(), tol / qp, [](double a, double b) { return a >= b; }) -
+                      qk.begin();
+            pcnt[(size_t)p * 3 + ck] = (int)cnt;
+        }
+    }
     ctx->segoff.assign((size_t)ns * ns * 3, 0);
     ctx->segcnt.assign((size_t)ns * ns * 3, 0);
     long long off = 0;
     for (int X = 0; X < ns; ++X)
         for (int Y = 0; Y < ns; ++Y) {
             const int p = pair_of[(size_t)X * ns + Y];
-            const double qp = ctx->schwarz[p];
             const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
             for (int ck = 0; ck < 3; ++ck) {
-                const auto &qk = qsorted[ck];
-                long long cnt;
-                if (tol <= 0.0) cnt = (long long)qk.size();
-                else if (qp <= 0.0) cnt = 0;
-                else
-                    cnt = std::lower_bound(qk.begin(), qk.end(), tol / qp, [](double a, double b) { return a >= b; }) -
-                          qk.begin();
+                const long long cnt = pcnt[(size_t)p * 3 + ck];
                 const size_t seg = ((size_t)X * ns + Y) * 3 + ck;
                 ctx->segoff[seg] = off;
                 ctx->segcnt[seg] = (int)cnt;

@@ -90,8 +90,10 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
         double t2 = 2.0 * T;
         // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160),
         // with 1/(2j+1) as constants; the reference divides (rounding-level difference)
+        // Q <= 12: sets have max L <= 1 (checked when the system is built), so
+        // Q = 3 (sum of four set L's) <= 12
 #pragma unroll
-        for (int j = 15; j >= 0; --j)
+        for (int j = 11; j >= 0; --j)
             if (j >= M && j < Q) fq = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
         // fq now holds F_top, top = min(Q, M); the reference never forms F_j for
         // j > Q, and callers only read F_j for j <= Q

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -96,6 +97,8 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
     prims.clear();
     for (int c = 0; c < 3; ++c) pair_ids[c].clear();
     const int ns = (int)shells.size();
+    // pairs first (cheap), then their primitive pairs in parallel: each pair
+    // owns the slice [pp0, pp0 + npp) of prims
     for (int A = 0; A < ns; ++A)
         for (int B = A; B < ns; ++B) {
             int f = A, g = B;
@@ -113,11 +116,22 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
             sp.same = (A == B);
             sp.sa = f;
             sp.sb = g;
-            const double ab2 = pow(sp.AB[0], 2.0) + pow(sp.AB[1], 2.0) + pow(sp.AB[2], 2.0);
+            sp.npp = (int)(sa.alpha.size() * sb.alpha.size());
+            prims.resize(prims.size() + sp.npp);
+            pair_ids[sa.L + sb.L].push_back((int)pairs.size());
+            pairs.push_back(sp);
+        }
+    auto fill = [&](size_t p0, size_t p1) {
+        for (size_t q = p0; q < p1; ++q) {
+            const ShellPair &sp = pairs[q];
+            const Shell &sa = shells[sp.sa], &sb = shells[sp.sb];
+            const double *RA = xyz + 3 * sa.center, *RB = xyz + 3 * sb.center;
+            const double ab2 = sp.AB[0] * sp.AB[0] + sp.AB[1] * sp.AB[1] + sp.AB[2] * sp.AB[2];
+            PrimPair *out = prims.data() + sp.pp0;
             for (size_t i = 0; i < sa.alpha.size(); ++i)
                 for (size_t j = 0; j < sb.alpha.size(); ++j) {
                     const double aa = sa.alpha[i], bb = sb.alpha[j], p = aa + bb;
-                    PrimPair pp;
+                    PrimPair &pp = *out++;
                     pp.p = p;
                     for (int d = 0; d < 3; ++d) {
                         pp.P[d] = (aa * RA[d] + bb * RB[d]) / p;
@@ -127,12 +141,16 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
                     pp.K = pp.EIJ * sa.coef[i] * sb.coef[j];
                     pp.beta = bb;
                     pp.ip = 1.0 / p;
-                    prims.push_back(pp);
                 }
-            sp.npp = (int)(sa.alpha.size() * sb.alpha.size());
-            pair_ids[sa.L + sb.L].push_back((int)pairs.size());
-            pairs.push_back(sp);
         }
+    };
+    const size_t np = pairs.size();
+    const unsigned nt = (unsigned)std::max<size_t>(1, std::min<size_t>({16, std::thread::hardware_concurrency(),
+                                                                         np / 2048 + 1}));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(fill, np * t / nt, np * (t + 1) / nt);
+    fill(0, np / nt);
+    for (auto &th : pool) th.join();
     return "";
 }
 
