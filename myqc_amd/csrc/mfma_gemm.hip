@@ -131,8 +131,118 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(int m, int n, int k, const 
             }
 }
 
+// hi/lo planes of op(A) = A^T (A column-major, lda): rows i < m of length
+// kp (k rounded up to GK), zero beyond k and beyond m (up to mp rows)
+__global__ void k_split_rows_t(const double *__restrict__ A, long long lda, int m, int k, int mp, int kp,
+                               uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+    const long long cnt = (long long)mp * kp;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / kp), kk = (int)(e - (long long)i * kp);
+        const double x = (i < m && kk < k) ? A[kk + lda * i] : 0.0;
+        split(x, hi[e], lo[e]);
+    }
+}
+
+// C = A^T B with A's hi/lo planes pre-split (k_split_rows_t) and B (k x n,
+// k-contiguous columns) split on its way into LDS with 16-byte vector loads;
+// the next k-step's B values are loaded into registers while this step's
+// MFMAs run.  Same tiling and accumulation as k_gemm_bf16x3.
+__global__ __launch_bounds__(256) void k_gemm_bf16x3_tn(int m, int n, int k, int kp, const uint16_t *__restrict__ Ahi,
+                                                        const uint16_t *__restrict__ Alo, const double *__restrict__ B,
+                                                        long long ldb, double *__restrict__ C, long long ldc,
+                                                        int vec_ok) {
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[GM][GKP], sAl[GM][GKP], sBh[GN][GKP], sBl[GN][GKP];
+    const int m0 = blockIdx.y * GM;
+    const long long n0 = (long long)blockIdx.x * GN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w & 1) * 64, wn = (w >> 1) * 32;
+    const int r16 = lane & 15, q = lane >> 4;
+    // B loader: column j, 8 consecutive k
+    const int bj = t >> 2, bk = (t & 3) * 8;
+    const long long gj = n0 + bj;
+    double nb[8];
+    auto load_b = [&](int k0) {
+        const int gk = k0 + bk;
+        if (vec_ok && gj < n && gk + 8 <= k) {
+            const double2 *p = reinterpret_cast<const double2 *>(B + gk + ldb * gj);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double2 d = p[v];
+                nb[2 * v] = d.x;
+                nb[2 * v + 1] = d.y;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) nb[e] = (gj < n && gk + e < k) ? B[gk + e + ldb * gj] : 0.0;
+        }
+    };
+    f32x4 acc[4][2], cor[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            cor[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        }
+    load_b(0);
+    for (int k0 = 0; k0 < k; k0 += GK) {
+        // A planes: 128 rows x 32 k, 16-byte chunks (two per thread per plane)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int id = t + 256 * c, row = id >> 2, ch = (id & 3) * 8;
+            const size_t src = (size_t)(m0 + row) * kp + k0 + ch;
+            *reinterpret_cast<uint4 *>(&sAh[row][ch]) = *reinterpret_cast<const uint4 *>(Ahi + src);
+            *reinterpret_cast<uint4 *>(&sAl[row][ch]) = *reinterpret_cast<const uint4 *>(Alo + src);
+        }
+        // B: split the prefetched values, then start loading the next step's
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint16_t h0, l0, h1, l1;
+            split(nb[2 * e], h0, l0);
+            split(nb[2 * e + 1], h1, l1);
+            h[e] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            l[e] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+        }
+        *reinterpret_cast<uint4 *>(&sBh[bj][bk]) = make_uint4(h[0], h[1], h[2], h[3]);
+        *reinterpret_cast<uint4 *>(&sBl[bj][bk]) = make_uint4(l[0], l[1], l[2], l[3]);
+        if (k0 + GK < k) load_b(k0 + GK);
+        __syncthreads();
+        bf16x8 bh[2], bl[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            bh[j] = *reinterpret_cast<const bf16x8 *>(&sBh[wn + 16 * j + r16][8 * q]);
+            bl[j] = *reinterpret_cast<const bf16x8 *>(&sBl[wn + 16 * j + r16][8 * q]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&sAh[wm + 16 * i + r16][8 * q]);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&sAl[wm + 16 * i + r16][8 * q]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+                cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], cor[i][j], 0, 0, 0);
+                cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], cor[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = m0 + wm + 16 * i + 4 * q + v;
+                const long long gjj = n0 + wn + 16 * j + r16;
+                if (gi < m && gjj < n) C[gi + ldc * gjj] = (double)acc[i][j][v] + (double)cor[i][j][v];
+            }
+}
+
 // C (m x n, ldc) = op(A) (m x k) * B (k x n, ldb) for `batch` strided
-// problems; FP64 in and out, split-bf16 MFMA products inside.
+// problems; FP64 in and out, split-bf16 MFMA products inside.  The transposed
+// single problem (the first quarter transform: a small A against a large,
+// k-contiguous B) pre-splits A once.
 int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A, int lda, long long strideA,
                 const double *B, int ldb, long long strideB, double *C, int ldc, long long strideC, int batch) {
     if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: empty problem");
@@ -140,6 +250,20 @@ int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A
     const long long gx = tilesN * batch;
     const int gy = (m + GM - 1) / GM;
     if (gx > 0x7fffffffLL || gy > 65535) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: grid too large");
+    if (transA && batch == 1) {
+        const int mp = gy * GM, kp = ((k + GK - 1) / GK) * GK;
+        uint16_t *planes = nullptr;
+        MYQC_CHECK(dalloc(ctx, (void **)&planes, sizeof(uint16_t) * 2 * (size_t)mp * kp));
+        uint16_t *hi = planes, *lo = planes + (size_t)mp * kp;
+        k_split_rows_t<<<256, 256, 0, ctx->stream>>>(A, lda, m, k, mp, kp, hi, lo);
+        const int vec_ok = (ldb % 2 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
+        k_gemm_bf16x3_tn<<<dim3((unsigned)tilesN, (unsigned)gy), 256, 0, ctx->stream>>>(m, n, k, kp, hi, lo, B, ldb, C,
+                                                                                      ldc, vec_ok);
+        MYQC_HIP(ctx, hipGetLastError());
+        MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        dfree((void **)&planes);
+        return 0;
+    }
     const dim3 grid((unsigned)gx, (unsigned)gy);
     if (transA)
         k_gemm_bf16x3<true><<<grid, 256, 0, ctx->stream>>>(m, n, k, A, lda, strideA, B, ldb, strideB, C, ldc, strideC,
