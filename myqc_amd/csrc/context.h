@@ -84,16 +84,18 @@ struct myqc_ctx {
     int *d_rank = nullptr;                // rank of each pair within its class's sorted list
     double *d_x64 = nullptr;              // expanded values
     float *d_x32 = nullptr;
-    // digestion schedule: work units (A, B, ket class) grouped into workgroups of one row shell A
-    int n_wg = 0, n_units = 0;
-    int *d_wg = nullptr;                  // [w] = {A, u0, u1}
+    // digestion schedule: work units (A, B) grouped into workgroups of one
+    // row shell A, each unit's kets cut into chunks of at most 64 of one class
+    int n_wg = 0, n_units = 0, n_chunks = 0;
+    int dig_rows = 0;                     // most D rows a workgroup stages in LDS
+    int *d_wg = nullptr;                  // [w] = {A, chunk0, chunk1, first D row, D rows, partial slot}, longest first
     int *d_unit = nullptr;                // [u] = {B, count ck=0, count ck=1, count ck=2}
+    int *d_chunk = nullptr;               // [c] = {offset lo, hi, aoB | LB << 15 | class << 16 | kets << 18, first rank}
     long long *d_unit_off = nullptr;      // [u][3] segment offsets
     int *d_meta[3] = {nullptr, nullptr, nullptr};   // per class, by Schwarz rank: aoC | aoD << 15 | same << 30
     int *d_shell = nullptr;               // [s] = {ao, L, wg0, wg1}
     int *d_ao_shell = nullptr;            // shell of each AO
-    double *d_partial = nullptr;          // n_wg x 3 x n
-    float *d_partial32 = nullptr;
+    double *d_partial = nullptr;          // n_wg x 3 x n (both precisions accumulate in FP64)
     double *d_Dpair = nullptr;            // density blocks of the kets, by class and rank (see jk.hip)
     float *d_Dpair32 = nullptr;
 

@@ -259,13 +259,13 @@ void free_eri(myqc_ctx *ctx, bool keep_memory) {
     void **ptrs[] = {(void **)&ctx->d_segoff, (void **)&ctx->d_rank,     (void **)&ctx->d_x64,
                      (void **)&ctx->d_x32,    (void **)&ctx->d_wg,       (void **)&ctx->d_unit,
                      (void **)&ctx->d_unit_off, (void **)&ctx->d_shell, (void **)&ctx->d_ao_shell,
-                     (void **)&ctx->d_partial, (void **)&ctx->d_partial32, (void **)&ctx->d_Dpair,
-                     (void **)&ctx->d_Dpair32};
+                     (void **)&ctx->d_partial, (void **)&ctx->d_Dpair,
+                     (void **)&ctx->d_Dpair32, (void **)&ctx->d_chunk};
     if (!keep_memory) {
         for (auto p : ptrs) dfree(p);
         for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
     }
-    ctx->n_wg = ctx->n_units = 0;
+    ctx->n_wg = ctx->n_units = ctx->n_chunks = 0;
     ctx->n_expanded = 0;
     ctx->have_eri = false;
     ctx->have_eri32 = false;
@@ -411,6 +411,9 @@ static int build_classes(myqc_ctx *ctx, double tol) {
 // ordered shell pair (X,Y) and ket class ck, the kets of class ck whose
 // Schwarz product with (X,Y) passes `tol` form a prefix of that class's
 // sorted pair list; their blocks (X Y|ket) are stored contiguously.
+// kets per lane of a digestion chunk (jk.hip)
+__host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 : row == 3 ? 4 : row == 9 ? 2 : 1; }
+
 static int build_expanded(myqc_ctx *ctx, double tol) {
     const int ns = (int)ctx->shells.size();
     ctx->ns = ns;
@@ -464,45 +467,84 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
         }
     ctx->n_expanded = off;
     // digestion schedule: per row shell X, its units Y (all three ket
-    // classes) packed into workgroups of about kTarget values
+    // classes) packed into workgroups of about kTarget values whose Y shells
+    // span at most dig_rows AOs (the D rows a workgroup stages in LDS: with
+    // the 3 owned rows of G, (3 + dig_rows) n doubles stay within
+    // kLdsBudget so that three workgroups fit a CU); each unit's kets are
+    // cut into chunks of at most 64 of one class, one wavefront's work
     const long long kTarget = 1 << 15;
-    std::vector<int> wg, unit, shell(4 * ns);
-    std::vector<long long> unit_off;
+    const size_t kLdsBudget = 52 * 1024;
+    const int n = ctx->n;
+    ctx->dig_rows = std::max(3, std::min(64, (int)(kLdsBudget / (sizeof(double) * (size_t)n)) - 3));
+    std::vector<int> wg, unit, chunk, shell(4 * ns);
+    std::vector<long long> unit_off, wg_cost;
     for (int X = 0; X < ns; ++X) {
         shell[4 * X + 0] = ctx->shells[X].ao;
         shell[4 * X + 1] = ctx->shells[X].L;
-        shell[4 * X + 2] = (int)(wg.size() / 3);
+        shell[4 * X + 2] = (int)(wg.size() / 6);
+        const int nX = ctx->shells[X].L ? 3 : 1;
         long long acc = 0;
-        int u0 = (int)(unit.size() / 4);
+        int ch0 = (int)(chunk.size() / 4), row0 = -1, row1 = -1;
+        auto close = [&]() {
+            const int ch1 = (int)(chunk.size() / 4);
+            if (ch1 > ch0) {
+                wg.push_back(X);
+                wg.push_back(ch0);
+                wg.push_back(ch1);
+                wg.push_back(row0);
+                wg.push_back(row1 - row0);
+                wg.push_back((int)(wg.size() / 6));   // partial slot: the workgroups of X stay contiguous
+                wg_cost.push_back(acc);
+            }
+            ch0 = ch1;
+            row0 = row1 = -1;
+            acc = 0;
+        };
         for (int Y = 0; Y < ns; ++Y) {
             const size_t seg = ((size_t)X * ns + Y) * 3;
-            const int c0 = ctx->segcnt[seg], c1 = ctx->segcnt[seg + 1], c2 = ctx->segcnt[seg + 2];
-            if (c0 + c1 + c2 == 0) continue;
-            const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
+            const int cnt[3] = {ctx->segcnt[seg], ctx->segcnt[seg + 1], ctx->segcnt[seg + 2]};
+            if (cnt[0] + cnt[1] + cnt[2] == 0) continue;
+            const int aoY = ctx->shells[Y].ao, nY = ctx->shells[Y].L ? 3 : 1;
+            if (row0 >= 0 && aoY + nY - row0 > ctx->dig_rows) close();
             unit.push_back(Y);
-            unit.push_back(c0);
-            unit.push_back(c1);
-            unit.push_back(c2);
-            for (int ck = 0; ck < 3; ++ck) unit_off.push_back(ctx->segoff[seg + ck]);
-            acc += (long long)nxy * (c0 + 3LL * c1 + 9LL * c2);
-            if (acc >= kTarget) {
-                const int u1 = (int)(unit.size() / 4);
-                wg.push_back(X);
-                wg.push_back(u0);
-                wg.push_back(u1);
-                u0 = u1;
-                acc = 0;
+            for (int ck = 0; ck < 3; ++ck) unit.push_back(cnt[ck]);
+            for (int ck = 0; ck < 3; ++ck) {
+                const long long off = ctx->segoff[seg + ck];
+                unit_off.push_back(off);
+                const int row = nY * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
+                const int step = 64 * kets_per_lane(row);
+                for (int r0 = 0; r0 < cnt[ck]; r0 += step) {
+                    // a workgroup closes after about kTarget values, also
+                    // inside a unit: chunks are independent
+                    if (row0 < 0) row0 = aoY;
+                    row1 = aoY + nY;
+                    const int nk = std::min(step, cnt[ck] - r0);
+                    chunk.push_back((int)(unsigned)(off & 0xffffffffLL));
+                    chunk.push_back((int)(off >> 32));
+                    chunk.push_back(aoY | (ctx->shells[Y].L << 15) | (ck << 16) | (nk << 18));
+                    chunk.push_back(r0);
+                    acc += (long long)nX * row * nk;
+                    if (acc >= kTarget) close();
+                }
             }
         }
-        const int u1 = (int)(unit.size() / 4);
-        if (u1 > u0) {
-            wg.push_back(X);
-            wg.push_back(u0);
-            wg.push_back(u1);
-        }
-        shell[4 * X + 3] = (int)(wg.size() / 3);
+        close();
+        shell[4 * X + 3] = (int)(wg.size() / 6);
     }
-    ctx->n_wg = (int)(wg.size() / 3);
+    // launch order: most values first, so that the long workgroups do not
+    // form the tail of the pass
+    {
+        const size_t nw = wg_cost.size();
+        std::vector<int> order(nw);
+        for (size_t i = 0; i < nw; ++i) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return wg_cost[p] > wg_cost[q]; });
+        std::vector<int> sorted(wg.size());
+        for (size_t i = 0; i < nw; ++i)
+            for (int k = 0; k < 6; ++k) sorted[6 * i + k] = wg[6 * (size_t)order[i] + k];
+        wg.swap(sorted);
+    }
+    ctx->n_wg = (int)(wg.size() / 6);
+    ctx->n_chunks = (int)(chunk.size() / 4);
     ctx->n_units = (int)(unit.size() / 4);
     // ket metadata in Schwarz-rank order per class
     for (int c = 0; c < 3; ++c) {
@@ -527,6 +569,7 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_wg, sizeof(int) * std::max<size_t>(wg.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit, sizeof(int) * std::max<size_t>(unit.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit_off, sizeof(long long) * std::max<size_t>(unit_off.size(), 1)));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_chunk, sizeof(int) * std::max<size_t>(chunk.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_shell, sizeof(int) * shell.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_ao_shell, sizeof(int) * ao_shell.size()));
     MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_segoff, ctx->segoff.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice,
@@ -539,6 +582,8 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
                                      ctx->stream));
         MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_unit_off, unit_off.data(), sizeof(long long) * unit_off.size(),
                                      hipMemcpyHostToDevice, ctx->stream));
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_chunk, chunk.data(), sizeof(int) * chunk.size(), hipMemcpyHostToDevice,
+                                     ctx->stream));
     }
     MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_shell, shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice,
                                  ctx->stream));
@@ -563,7 +608,6 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     else dfree((void **)&ctx->d_x32);   // a kept buffer may be smaller than this build
     const size_t part = (size_t)std::max(ctx->n_wg, 1) * 3 * ctx->n;
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * part));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial32, sizeof(float) * part));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair, sizeof(double) * 9 * ctx->pairs.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair32, sizeof(float) * 9 * ctx->pairs.size()));
     hipEvent_t ev;

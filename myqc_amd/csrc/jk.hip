@@ -9,13 +9,18 @@
 // turns every update into a row-local one:
 //     J: G(i,j) += f D(k,l) (ij|kl),              f = 2 unless C == D
 //     K: G(i,k) -= D(j,l) (ij|kl) / 2,  and if C != D  G(i,l) -= D(j,k) (ij|kl) / 2
-// for i in A, j in B.  One workgroup owns the rows of one shell A for a run
-// of units (A, B, ket class): it stages D's rows of B in LDS, streams the
-// unit's integrals once from HBM (the dominant traffic), reduces J across the
-// wavefront and adds K into LDS rows of A, then writes those rows to its own
-// partial slot.  A gather sums the partial rows of each shell in a fixed
-// order and G = (A + A^T)/2 restores symmetry: no global atomics, and
-// bitwise-reproducible results.
+// for i in A, j in B.  A workgroup owns the rows of one shell A for a run of
+// about 2^15 stored values: it stages in LDS the rows of D of the B shells it
+// meets (at most dig_rows AOs), then each wavefront takes chunks of kets of
+// one (A, B, ket class) segment in turn, with no barrier between chunks.  A
+// lane takes 1-8 kets of a chunk (more for the small classes) and issues all
+// their loads of one bra row before their arithmetic.  J is reduced across
+// the wavefront and K is added per ket, both into FP64 LDS rows of A (FP32
+// values are accumulated in FP64 too); the rows go to the workgroup's own
+// partial slot.  Workgroups launch longest first.  A gather sums the partial
+// rows of each shell in a fixed order and G = (A + A^T)/2 restores symmetry:
+// no global atomics; the LDS atomics make the summation order, and so the
+// last bits, vary from run to run.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -35,13 +40,14 @@ template <typename TV>
 struct DigArgs {
     const int *meta0, *meta1, *meta2;         // ket AOs by class and Schwarz rank
     const TV *dk0, *dk1, *dk2;                // ket density blocks by class and rank (1, 3, 9 values)
-    const int *wg;                            // [w] = {A, u0, u1}
+    const int *wg;                            // [w] = {A, chunk0, chunk1, first D row, D rows, partial slot}
     const int *unit;                          // [u] = {B, c0, c1, c2}
+    const int4 *chunk;                        // [c] = {offset lo, offset hi, aoB | LB << 15 | class << 16 | kets << 18, first rank}
     const long long *unit_off;                // [u][3]
     const int *shell;                         // [s] = {ao, L, wg0, wg1}
     const TV *x;                              // expanded values
     const TV *D;                              // symmetric density (or change), n x n
-    TV *partial;                              // [w][3][n]
+    double *partial;                          // [w][3][n]
     int n;
 };
 
@@ -56,142 +62,168 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// One ket of rank r in class CK against the bra (A, B): J into the thread's
-// accumulator, K into the workgroup's LDS rows of A.  s_drow holds D's rows
-// of shell B.
+// Kets per lane of a chunk: enough that every lane has several loads of one
+// bra row in flight (nb*nc*nd values per ket and row).
+__host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 : row == 3 ? 4 : row == 9 ? 2 : 1; }
+
+// One chunk of kets of class CK against the bra (A, B): lane l takes kets
+// r0 + l + 64 j, j < KPL.  J is reduced across the wavefront into the LDS
+// rows of A, K added per ket into them.  s_drow holds D's rows of shell B.
+// The bra rows are streamed one at a time, with the loads of all the lane's
+// kets issued before their arithmetic.
 template <int LA, int LB, int CK, typename TV>
-__device__ __forceinline__ void digest_ket(const DigArgs<TV> &a, TV *s_rows, const TV *s_drow, int r, long long off,
-                                           TV (&J)[3][3]) {
+__device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int r0, int nk,
+                                             long long off, int aoB) {
     constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
-    constexpr int BLK = na * nb * nc * nd;
+    constexpr int ROW = nb * nc * nd, BLK = na * ROW, KPL = kets_per_lane(ROW);
     const int n = a.n;
-    const int m = (CK == 0 ? a.meta0 : CK == 1 ? a.meta1 : a.meta2)[r];
-    const int aoC = m & 0x7fff, aoD = (m >> 15) & 0x7fff, same = m >> 30;
-    const TV *__restrict__ v = a.x + off + (long long)r * BLK;
-    TV x[BLK];
+    const int lane = threadIdx.x & 63;
+    const int *meta = CK == 0 ? a.meta0 : CK == 1 ? a.meta1 : a.meta2;
+    const TV *dkc = CK == 0 ? a.dk0 : CK == 1 ? a.dk1 : a.dk2;
+    bool act[KPL];
+    int aoC[KPL], aoD[KPL], same[KPL];
+    TV dq[KPL][nc * nd], dbd[KPL][nb * nd], dbc[KPL][nb * nc];
+    const TV *v[KPL];
 #pragma unroll
-    for (int e = 0; e < BLK; ++e) x[e] = v[e];
-    const TV *dk = (CK == 0 ? a.dk0 : CK == 1 ? a.dk1 : a.dk2) + (size_t)r * (nc * nd);
-    TV dq[nc][nd];
+    for (int j = 0; j < KPL; ++j) {
+        const int k = lane + 64 * j;
+        act[j] = k < nk;
+        const int r = r0 + (act[j] ? k : 0);
+        const int m = meta[r];
+        aoC[j] = m & 0x7fff;
+        aoD[j] = (m >> 15) & 0x7fff;
+        same[j] = m >> 30;
+        v[j] = a.x + off + (long long)r * BLK;
 #pragma unroll
-    for (int c = 0; c < nc; ++c)
-#pragma unroll
-        for (int d = 0; d < nd; ++d) dq[c][d] = dk[c * nd + d];
-    const TV fj = same ? (TV)1 : (TV)2;
-    TV dbd[nb][nd], dbc[nb][nc];
-#pragma unroll
-    for (int b = 0; b < nb; ++b) {
-#pragma unroll
-        for (int d = 0; d < nd; ++d) dbd[b][d] = s_drow[b * n + aoD + d];
-#pragma unroll
-        for (int c = 0; c < nc; ++c) dbc[b][c] = s_drow[b * n + aoC + c];
+        for (int e = 0; e < nc * nd; ++e) dq[j][e] = act[j] ? dkc[(size_t)r * (nc * nd) + e] : (TV)0;
     }
 #pragma unroll
+    for (int j = 0; j < KPL; ++j)
+#pragma unroll
+        for (int b = 0; b < nb; ++b) {
+#pragma unroll
+            for (int d = 0; d < nd; ++d) dbd[j][b * nd + d] = s_drow[b * n + aoD[j] + d];
+#pragma unroll
+            for (int c = 0; c < nc; ++c) dbc[j][b * nc + c] = s_drow[b * n + aoC[j] + c];
+        }
+#pragma unroll 1
     for (int ia = 0; ia < na; ++ia) {
+        TV x[KPL][ROW];
 #pragma unroll
-        for (int ib = 0; ib < nb; ++ib) {
-            TV js = 0;
+        for (int j = 0; j < KPL; ++j)
 #pragma unroll
-            for (int c = 0; c < nc; ++c)
+            for (int e = 0; e < ROW; ++e) x[j][e] = act[j] ? v[j][ia * ROW + e] : (TV)0;
+        TV jr[nb];
 #pragma unroll
-                for (int d = 0; d < nd; ++d) js += dq[c][d] * x[((ia * nb + ib) * nc + c) * nd + d];
-            J[ia][ib] += fj * js;
-        }
-        // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
+        for (int ib = 0; ib < nb; ++ib) jr[ib] = 0;
 #pragma unroll
-        for (int c = 0; c < nc; ++c) {
-            TV ks = 0;
+        for (int j = 0; j < KPL; ++j) {
+            const TV fj = same[j] ? (TV)1 : (TV)2;
 #pragma unroll
-            for (int ib = 0; ib < nb; ++ib)
+            for (int ib = 0; ib < nb; ++ib) {
+                TV js = 0;
 #pragma unroll
-                for (int d = 0; d < nd; ++d) ks += dbd[ib][d] * x[((ia * nb + ib) * nc + c) * nd + d];
-            atomicAdd(&s_rows[ia * n + aoC + c], (TV)(-0.5) * ks);
-        }
-        // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
-        if (!same) {
+                for (int c = 0; c < nc; ++c)
 #pragma unroll
-            for (int d = 0; d < nd; ++d) {
+                    for (int d = 0; d < nd; ++d) js += dq[j][c * nd + d] * x[j][(ib * nc + c) * nd + d];
+                jr[ib] += fj * js;
+            }
+            if (!act[j]) continue;
+            // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
+#pragma unroll
+            for (int c = 0; c < nc; ++c) {
                 TV ks = 0;
 #pragma unroll
                 for (int ib = 0; ib < nb; ++ib)
 #pragma unroll
-                    for (int c = 0; c < nc; ++c) ks += dbc[ib][c] * x[((ia * nb + ib) * nc + c) * nd + d];
-                atomicAdd(&s_rows[ia * n + aoD + d], (TV)(-0.5) * ks);
+                    for (int d = 0; d < nd; ++d) ks += dbd[j][ib * nd + d] * x[j][(ib * nc + c) * nd + d];
+                atomicAdd(&s_rows[ia * n + aoC[j] + c], -0.5 * (double)ks);
+            }
+            // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
+            if (!same[j]) {
+#pragma unroll
+                for (int d = 0; d < nd; ++d) {
+                    TV ks = 0;
+#pragma unroll
+                    for (int ib = 0; ib < nb; ++ib)
+#pragma unroll
+                        for (int c = 0; c < nc; ++c) ks += dbc[j][ib * nc + c] * x[j][(ib * nc + c) * nd + d];
+                    atomicAdd(&s_rows[ia * n + aoD[j] + d], -0.5 * (double)ks);
+                }
             }
         }
-    }
-}
-
-// All kets of the unit (A, B): the workgroup's threads stride over the
-// concatenated ket-class segments.
-template <int LA, int LB, typename TV>
-__device__ __forceinline__ void digest_unit(const DigArgs<TV> &a, TV *s_rows, const TV *s_drow, int u, int aoB) {
-    constexpr int na = ncart(LA), nb = ncart(LB);
-    const int c0 = a.unit[4 * u + 1], c1 = a.unit[4 * u + 2], c2 = a.unit[4 * u + 3];
-    const long long o0 = a.unit_off[3 * u], o1 = a.unit_off[3 * u + 1], o2 = a.unit_off[3 * u + 2];
-    TV J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    const int total = c0 + c1 + c2;
-    for (int k = threadIdx.x; k < total; k += blockDim.x) {
-        if (k < c0) digest_ket<LA, LB, 0>(a, s_rows, s_drow, k, o0, J);
-        else if (k < c0 + c1) digest_ket<LA, LB, 1>(a, s_rows, s_drow, k - c0, o1, J);
-        else digest_ket<LA, LB, 2>(a, s_rows, s_drow, k - c0 - c1, o2, J);
-    }
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int ia = 0; ia < na; ++ia)
 #pragma unroll
         for (int ib = 0; ib < nb; ++ib) {
-            const TV t = wave_sum(J[ia][ib]);
-            if (lane == 0) atomicAdd(&s_rows[ia * a.n + aoB + ib], t);
+            const TV t = wave_sum(jr[ib]);
+            if (lane == 0) atomicAdd(&s_rows[ia * n + aoB + ib], (double)t);
         }
+    }
 }
 
-// One workgroup per block of units of one row shell A; writes the rows of A
-// it accumulated to its partial slot.
+template <int LA, typename TV>
+__device__ __forceinline__ void digest_chunk_la(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int LB, int ck,
+                                                int r0, int nk, long long off, int aoB) {
+    if (LB == 0) {
+        if (ck == 0) digest_chunk<LA, 0, 0>(a, s_rows, s_drow, r0, nk, off, aoB);
+        else if (ck == 1) digest_chunk<LA, 0, 1>(a, s_rows, s_drow, r0, nk, off, aoB);
+        else digest_chunk<LA, 0, 2>(a, s_rows, s_drow, r0, nk, off, aoB);
+    } else {
+        if (ck == 0) digest_chunk<LA, 1, 0>(a, s_rows, s_drow, r0, nk, off, aoB);
+        else if (ck == 1) digest_chunk<LA, 1, 1>(a, s_rows, s_drow, r0, nk, off, aoB);
+        else digest_chunk<LA, 1, 2>(a, s_rows, s_drow, r0, nk, off, aoB);
+    }
+}
+
+// One workgroup per run of units of one row shell A: it stages the D rows of
+// the units' B shells once, then each wavefront takes chunks in turn (no
+// barrier between them; the next chunk's descriptor is loaded ahead), and
+// the rows of A it accumulated go to its partial slot.
 template <typename TV>
-__global__ __launch_bounds__(256) void k_digest(DigArgs<TV> a) {
+__global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
     extern __shared__ unsigned char smem[];
     const int n = a.n;
-    TV *s_rows = reinterpret_cast<TV *>(smem);
-    TV *s_drow = s_rows + 3 * n;
+    double *s_rows = reinterpret_cast<double *>(smem);
+    TV *s_drow = reinterpret_cast<TV *>(s_rows + 3 * n);
     const int w = blockIdx.x;
-    const int A = a.wg[3 * w], u0 = a.wg[3 * w + 1], u1 = a.wg[3 * w + 2];
+    const int A = a.wg[6 * w], ch0 = a.wg[6 * w + 1], ch1 = a.wg[6 * w + 2];
+    const int row0 = a.wg[6 * w + 3], nrows = a.wg[6 * w + 4], slot = a.wg[6 * w + 5];
     const int LA = a.shell[4 * A + 1];
     const int na = LA ? 3 : 1;
     for (int i = threadIdx.x; i < na * n; i += blockDim.x) s_rows[i] = 0;
-    for (int u = u0; u < u1; ++u) {
-        const int B = a.unit[4 * u];
-        const int aoB = a.shell[4 * B], LB = a.shell[4 * B + 1];
-        const int nb = LB ? 3 : 1;
-        __syncthreads();   // previous unit done with s_drow
-        for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
-            const int b = i / n, j = i - b * n;
-            s_drow[i] = a.D[(aoB + b) + (size_t)n * j];
-        }
-        __syncthreads();
-        if (LA == 0) {
-            if (LB == 0) digest_unit<0, 0>(a, s_rows, s_drow, u, aoB);
-            else digest_unit<0, 1>(a, s_rows, s_drow, u, aoB);
-        } else {
-            if (LB == 0) digest_unit<1, 0>(a, s_rows, s_drow, u, aoB);
-            else digest_unit<1, 1>(a, s_rows, s_drow, u, aoB);
-        }
+    // D(row0 + b, j) = D(j, row0 + b): read columns, coalesced
+    for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) {
+        const int b = i / n, j = i - b * n;
+        s_drow[i] = a.D[j + (size_t)n * (row0 + b)];
     }
     __syncthreads();
-    TV *dst = a.partial + (size_t)w * 3 * n;
+    const int wave = threadIdx.x >> 6;
+    int c = ch0 + wave;
+    int4 d = c < ch1 ? a.chunk[c] : make_int4(0, 0, 0, 0);
+    while (c < ch1) {
+        const int cn = c + 4;
+        const int4 dn = cn < ch1 ? a.chunk[cn] : make_int4(0, 0, 0, 0);
+        const long long off = (long long)(unsigned)d.x | ((long long)d.y << 32);
+        const int aoB = d.z & 0x7fff, LB = (d.z >> 15) & 1, ck = (d.z >> 16) & 3, nk = d.z >> 18;
+        const TV *drow = s_drow + (size_t)(aoB - row0) * n;
+        if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB);
+        else digest_chunk_la<1>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB);
+        c = cn;
+        d = dn;
+    }
+    __syncthreads();
+    double *dst = a.partial + (size_t)slot * 3 * n;
     for (int i = threadIdx.x; i < na * n; i += blockDim.x) dst[i] = s_rows[i];
 }
 
 // G(i,j) = sum over the workgroups of shell(i) of their partial rows
-template <typename TV>
-__global__ void k_gather_rows(const TV *__restrict__ partial, const int *__restrict__ shell,
+__global__ void k_gather_rows(const double *__restrict__ partial, const int *__restrict__ shell,
                               const int *__restrict__ ao_shell, int n, double *__restrict__ Graw) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (j >= n) return;
     const int A = ao_shell[i];
     const int ia = i - shell[4 * A], w0 = shell[4 * A + 2], w1 = shell[4 * A + 3];
     double s = 0.0;
-    for (int w = w0; w < w1; ++w) s += (double)partial[((size_t)w * 3 + ia) * n + j];
+    for (int w = w0; w < w1; ++w) s += partial[((size_t)w * 3 + ia) * n + j];
     Graw[i + (size_t)n * j] = s;
 }
 
@@ -241,7 +273,7 @@ int ensure_work(myqc_ctx *ctx) {
 }
 
 template <typename TV>
-static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, TV *partial, double *Graw) {
+static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double *partial, double *Graw) {
     const int np0 = (int)ctx->pair_ids[0].size(), np1 = (int)ctx->pair_ids[1].size(),
               np2 = (int)ctx->pair_ids[2].size();
     TV *dk0 = dk, *dk1 = dk + np0, *dk2 = dk + np0 + 3 * (size_t)np1;
@@ -262,13 +294,14 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, TV *pa
         a.dk2 = dk2;
         a.wg = ctx->d_wg;
         a.unit = ctx->d_unit;
+        a.chunk = reinterpret_cast<const int4 *>(ctx->d_chunk);
         a.unit_off = ctx->d_unit_off;
         a.shell = ctx->d_shell;
         a.x = x;
         a.D = D;
         a.partial = partial;
         a.n = ctx->n;
-        const size_t shm = sizeof(TV) * 6 * ctx->n;
+        const size_t shm = (3 * sizeof(double) + ctx->dig_rows * sizeof(TV)) * (size_t)ctx->n;
         // timed alone: algorithmic bytes per pass are one read of every
         // stored unique integral plus the density in and G out (16 n^2)
         hipEvent_t ev;
@@ -276,7 +309,7 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, TV *pa
         k_digest<TV><<<ctx->n_wg, 256, shm, ctx->stream>>>(a);
         MYQC_CHECK(timer_end(ctx, T_JK, ev, (double)sizeof(TV) * (double)ctx->n_stored + 16.0 * ctx->n * ctx->n));
     }
-    k_gather_rows<TV><<<dim3((ctx->n + 127) / 128, ctx->n), 128, 0, ctx->stream>>>(partial, ctx->d_shell,
+    k_gather_rows<<<dim3((ctx->n + 127) / 128, ctx->n), 128, 0, ctx->stream>>>(partial, ctx->d_shell,
                                                                                   ctx->d_ao_shell, ctx->n, Graw);
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
@@ -289,7 +322,7 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     MYQC_CHECK(ensure_work(ctx));
     const int n = ctx->n;
     const size_t nn = (size_t)n * n;
-    if ((size_t)6 * n * sizeof(double) > 64 * 1024)
+    if ((size_t)(3 + ctx->dig_rows) * n * sizeof(double) > 64 * 1024)
         return set_error(ctx, MYQC_E_ARG, "digestion rows for n=%d exceed 64 KB of LDS", n);
     if (precision == MYQC_FP64) {
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, ctx->d_D, nullptr, n);
@@ -298,7 +331,7 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     } else {
         if (!ctx->have_eri32) return set_error(ctx, MYQC_E_STATE, "no FP32 integrals stored");
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n);
-        MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial32, ctx->d_A));
+        MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial, ctx->d_A));
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n);
         k_add<<<256, 256, 0, ctx->stream>>>(d_Gout, ctx->d_G, d_Gout, nn);
     }
