@@ -18,7 +18,8 @@
 
 namespace myqc {
 
-int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, int n, int nocc,
+int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, double *tpart,
+               int n, int nocc,
                int *ok, int *steps);
 int gershgorin_bounds(myqc_ctx *ctx, const double *A, int n, double *lohi, double *slots);
 
@@ -126,13 +127,13 @@ struct ScfBuffers {
     double *H = nullptr, *S = nullptr, *X = nullptr, *Xi = nullptr, *F = nullptr, *Fx = nullptr, *D = nullptr,
            *Dn = nullptr, *Do = nullptr, *Don = nullptr, *Dref = nullptr, *G = nullptr, *C = nullptr, *T1 = nullptr,
            *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr, *lohi = nullptr,
-           *part = nullptr, *red = nullptr;
+           *part = nullptr, *red = nullptr, *sp2t = nullptr;
     double *hF = nullptr, *hE = nullptr;   // DIIS history, kDiis x n^2 each
     int *info = nullptr;
     static constexpr int kDiis = 8;
     void release() {
         double **ps[] = {&H,  &S,  &X,   &Xi,  &F,   &Fx,   &D,    &Dn,   &Do,   &Don, &Dref, &G, &C,
-                         &T1, &T2, &P2, &eig, &work, &dsc, &lohi, &part, &red, &hF,  &hE};
+                         &T1, &T2, &P2, &eig, &work, &dsc, &lohi, &part, &red, &hF,  &hE, &sp2t};
         for (auto p : ps) dfree((void **)p);
         dfree((void **)&info);
         n = -1;
@@ -147,6 +148,8 @@ struct ScfBuffers {
         MYQC_CHECK(dalloc(ctx, (void **)&work, sizeof(double) * nb));
         MYQC_CHECK(dalloc(ctx, (void **)&dsc, sizeof(double) * 16));
         MYQC_CHECK(dalloc(ctx, (void **)&lohi, sizeof(double) * 2 * nb));
+        const size_t tiles = (size_t)(nb + 15) / 16;   // purification trace partials: 2 steps x tiles^2 x 2
+        MYQC_CHECK(dalloc(ctx, (void **)&sp2t, sizeof(double) * 4 * tiles * tiles));
         MYQC_CHECK(dalloc(ctx, (void **)&part, sizeof(double) * kRedJobs * kRedBlocks));
         MYQC_CHECK(dalloc(ctx, (void **)&red, sizeof(double) * kRedJobs));
         MYQC_CHECK(dalloc(ctx, (void **)&hF, sizeof(double) * nn * kDiis));
@@ -291,7 +294,7 @@ static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int
         return 0;
     }
     int ok = 0;
-    MYQC_CHECK(sp2_purify(ctx, Fo, P, b.P2, b.lohi, b.dsc, n, nocc, &ok, &ctx->scf_sp2_steps));
+    MYQC_CHECK(sp2_purify(ctx, Fo, P, b.P2, b.lohi, b.dsc, b.sp2t, n, nocc, &ok, &ctx->scf_sp2_steps));
     ++ctx->scf_purify;
     if (!ok) {
         // no clean gap at this step: occupy the nocc lowest eigenvectors

@@ -9,9 +9,9 @@
 // For symmetric P, tr(P^2) = ||P||_F^2, so both traces the choice needs are
 // known before the product; one kernel per step computes P^2 with
 // v_mfma_f64_16x16x4f64 (one wavefront per 16x16 tile), applies the choice
-// in its epilogue, writes the next P and accumulates that P's trace and
-// Frobenius norm for the following step.  Three rotating scalar slots carry
-// the traces between launches without host synchronisation.
+// in its epilogue, writes the next P and that P's per-tile trace and
+// Frobenius-norm partials for the following step, so the traces pass between
+// launches without host synchronisation.
 #include <algorithm>
 #include <cmath>
 
@@ -22,7 +22,7 @@ namespace myqc {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-// slots: 3 x {tr P, ||P||_F^2}; bounds at slots[6..7]
+// slots: {tr P0, ||P0||_F^2} at slots[0..1]; bounds at slots[6..7]; control words at slots + 8
 __global__ __launch_bounds__(64) void k_gersh_rows(const double *__restrict__ F, int n, double *__restrict__ lohi) {
     const int i = blockIdx.x, lane = threadIdx.x;
     double r = 0.0;
@@ -91,94 +91,112 @@ __global__ __launch_bounds__(64) void k_sp2_start(const double *__restrict__ F, 
     }
 }
 
-// One purification step k: Pn = choice(P^2, 2P - P^2) from the traces in
-// slot k%3; Pn's traces accumulate into slot (k+1)%3; slot (k+2)%3 (last read
-// by step k-1) is cleared for step k+1.  Grid: one workgroup of KS wavefronts
-// per 16x16 output tile, wavefront w summing the k range [w*kc, (w+1)*kc) so
-// that each dependent MFMA chain is n/(4 KS) long; the partial tiles meet in
-// LDS.  Lane l feeds A row l%16 and B column l%16 at k offset l/16 and
-// receives D rows l/16 + 4v, column l%16, v = 0..3 (the layout the GPU test of
-// the builtin checks).
-constexpr int KS = 8;
+// One purification step k: Pn = choice(P^2, 2P - P^2).  Grid: one workgroup
+// of KS wavefronts per 16x16 output tile, wavefront w summing the k range
+// [w*kc, (w+1)*kc) so that each dependent MFMA chain is n/(4 KS) long; the
+// partial tiles meet in LDS.  Lane l feeds A row l%16 and B column l%16 at k
+// offset l/16 and receives D rows l/16 + 4v, column l%16, v = 0..3 (the layout
+// the GPU test of the builtin checks).
+//
+// The traces of P that the choice needs arrive as per-tile partials written
+// by step k-1 (tin, nin of them; step 0 reads the one pair k_sp2_start
+// summed), and this step writes Pn's per-tile partials to tout: no atomics,
+// and every workgroup sums the same partials in the same order, so all of
+// them make the same choice.  Every load a step needs (the partials, the
+// control word, its MFMA operands and its own tile of P) is issued before
+// any of them is used, so a step costs about one memory round trip.
+//
+// Convergence is decided on the device: step k meets the criterion when k >=
+// first and P_k is idempotent to 1e-8 with trace nocc to 1e-6 (tr P -
+// ||P||^2 = sum l(1-l) over P's eigenvalues; near convergence each step
+// squares it).  The first such step is recorded in ctrl[0]; two steps later
+// the projector is final and the remaining launched steps return at once.
+constexpr int KS = 8, KU = 9;
 
-// Tile `tile` of step k (see k_sp2_step_mfma); s_acc: (KS - 1) x 4 x 64 doubles of LDS
-__device__ __forceinline__ void sp2_tile(const double *__restrict__ P, double *__restrict__ Pn, int n, double nocc,
-                                         double *__restrict__ slots, int step, int tile, double (*s_acc)[4][64]) {
-    const int tiles = (n + 15) >> 4;
+__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
+                                                          int n, double nocc, const double *__restrict__ tin,
+                                                          int nin, double *__restrict__ tout, int step, int first,
+                                                          int *__restrict__ ctrl) {
+    __shared__ double s_acc[KS - 1][4][64];
+    __shared__ double s_t[KS], s_f[KS];
+    const int tiles = (n + 15) >> 4, tile = blockIdx.x;
     const int ti = tile % tiles, tj = tile / tiles;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
     const int ia = ti * 16 + r;     // row of P fed as A
     const int jb = tj * 16 + r;     // column of P fed as B (read as a row: P is symmetric)
     const int kc = ((n + 4 * KS - 1) / (4 * KS)) * 4;
     const int kb = w * kc, ke = min(n, kb + kc);
+    double t_in = 0.0, f_in = 0.0;
+    for (int i = threadIdx.x; i < nin; i += blockDim.x) {
+        t_in += tin[2 * i];
+        f_in += tin[2 * i + 1];
+    }
+    const int seen = __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double own[4] = {0.0, 0.0, 0.0, 0.0};
+    if (w == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = ti * 16 + kq + 4 * v, col = tj * 16 + r;
+            if (row < n && col < n) own[v] = P[row + (size_t)n * col];
+        }
+    }
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 9
-    for (int k0 = kb; k0 < ke; k0 += 4) {
-        const int k = k0 + kq;
-        const double a = (ia < n && k < ke) ? P[ia + (size_t)n * k] : 0.0;
-        const double b = (jb < n && k < ke) ? P[jb + (size_t)n * k] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int k00 = kb; k00 < ke; k00 += 4 * KU) {
+        double a[KU], b[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int k = k00 + 4 * u + kq;
+            a[u] = (ia < n && k < ke) ? P[ia + (size_t)n * k] : 0.0;
+            b[u] = (jb < n && k < ke) ? P[jb + (size_t)n * k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+    }
+    t_in = wsum(t_in);
+    f_in = wsum(f_in);
+    if (lane == 0) {
+        s_t[w] = t_in;
+        s_f[w] = f_in;
     }
     if (w > 0) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) s_acc[w - 1][v][lane] = acc[v];
     }
     __syncthreads();
-    if (w == 0) {
+    double tr = 0.0, fro = 0.0;
 #pragma unroll
-        for (int q = 0; q < KS - 1; ++q)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] += s_acc[q][v][lane];
-        const double *sl = slots + 2 * (step % 3);
-        const double tr = sl[0], fro = sl[1];
-        const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
-        double t = 0.0, f = 0.0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = ti * 16 + kq + 4 * v, col = tj * 16 + r;
-            if (row < n && col < n) {
-                const size_t e = row + (size_t)n * col;
-                const double p2 = acc[v];
-                const double nw = square ? p2 : 2.0 * P[e] - p2;
-                Pn[e] = nw;
-                f += nw * nw;
-                if (row == col) t += nw;
-            }
-        }
-        t = wsum(t);
-        f = wsum(f);
-        if (lane == 0) {
-            double *nx = slots + 2 * ((step + 1) % 3);
-            unsafeAtomicAdd(&nx[0], t);
-            unsafeAtomicAdd(&nx[1], f);
-            if (tile == 0) {
-                double *cl = slots + 2 * ((step + 2) % 3);
-                cl[0] = 0.0;
-                cl[1] = 0.0;
-            }
-        }
+    for (int q = 0; q < KS; ++q) {
+        tr += s_t[q];
+        fro += s_f[q];
     }
-    __syncthreads();   // s_acc is reused by the workgroup's next tile
-}
-
-// Convergence is decided on the device: step k meets the criterion when k >=
-// first and P_k is idempotent to 1e-8 with trace nocc to 1e-6 (tr P -
-// ||P||^2 = sum l(1-l) over P's eigenvalues; near convergence each step
-// squares it).  The first such step is recorded in ctrl[0]; two steps later
-// the projector is final and the remaining launched steps return at once.
-// All workgroups of a step read the same traces, so they agree.
-__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
-                                                          int n, double nocc, double *__restrict__ slots, int step,
-                                                          int first, int *__restrict__ ctrl) {
-    __shared__ double s_acc[KS - 1][4][64];
-    const double *sl = slots + 2 * (step % 3);
-    const double tr = sl[0], fro = sl[1];
     const bool crit = step >= first && fabs(tr - fro) < 1e-8 && fabs(tr - nocc) < 1e-6;
-    const int seen = __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int cs = (seen >= 0 && seen < step) ? seen : (crit ? step : -1);
     if (cs >= 0 && step >= cs + 2) return;
     if (blockIdx.x == 0 && threadIdx.x == 0 && seen < 0 && crit) ctrl[0] = step;
-    sp2_tile(P, Pn, n, nocc, slots, step, blockIdx.x, s_acc);
+    if (w != 0) return;
+#pragma unroll
+    for (int q = 0; q < KS - 1; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] += s_acc[q][v][lane];
+    const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
+    double t = 0.0, f = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = ti * 16 + kq + 4 * v, col = tj * 16 + r;
+        if (row < n && col < n) {
+            const double p2 = acc[v];
+            const double nw = square ? p2 : 2.0 * own[v] - p2;
+            Pn[row + (size_t)n * col] = nw;
+            f += nw * nw;
+            if (row == col) t += nw;
+        }
+    }
+    t = wsum(t);
+    f = wsum(f);
+    if (lane == 0) {
+        tout[2 * tile] = t;
+        tout[2 * tile + 1] = f;
+    }
 }
 
 // After the launched steps: status = {converged, final step}; the final
@@ -205,14 +223,15 @@ int gershgorin_bounds(myqc_ctx *ctx, const double *A, int n, double *lohi, doubl
 
 // Projector onto the nocc lowest levels of the symmetric Fo (n x n) into P.
 // work: n x n scratch; lohi: 2n; slots: 16 doubles (traces, bounds, and the
-// control words at slots + 8).  Returns 1 in *ok when
+// control words at slots + 8); tpart: 4 ceil(n/16)^2 doubles (per-tile trace
+// partials of two consecutive steps).  Returns 1 in *ok when
 // purification converged (idempotency below 1e-8, then two more steps).
 // *steps carries the step count of the previous call in and this call's out:
 // successive SCF iterations need nearly the same count, so the host checks
 // convergence (a synchronising read of two scalars) only from a few steps
 // before it.
-int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, int n, int nocc,
-               int *ok, int *steps) {
+int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double *lohi, double *slots, double *tpart,
+               int n, int nocc, int *ok, int *steps) {
     *ok = 0;
     MYQC_CHECK(gershgorin_bounds(ctx, Fo, n, lohi, slots));
     const size_t nn = (size_t)n * n;
@@ -229,8 +248,13 @@ int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double 
         for (; launched < target; ++launched) {
             const double *cur = (launched & 1) ? work : P;
             double *nxt = (launched & 1) ? P : work;
-            k_sp2_step_mfma<<<tiles * tiles, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, slots, launched,
-                                                                       first, ctrl);
+            // traces in: slots[0..1] (k_sp2_start) for step 0, else the
+            // previous step's per-tile partials; out: this step's partials
+            const double *tin = launched == 0 ? slots : tpart + 2 * (size_t)tiles * tiles * ((launched - 1) & 1);
+            double *tout = tpart + 2 * (size_t)tiles * tiles * (launched & 1);
+            k_sp2_step_mfma<<<tiles * tiles, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, tin,
+                                                                       launched == 0 ? 1 : tiles * tiles, tout,
+                                                                       launched, first, ctrl);
         }
         k_sp2_finish<<<64, 256, 0, ctx->stream>>>(P, work, n, ctrl);
         MYQC_HIP(ctx, hipGetLastError());
