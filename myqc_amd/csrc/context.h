@@ -63,6 +63,9 @@ struct myqc_ctx {
     myqc::PrimPair *d_prims = nullptr;
     double *d_ftT = nullptr;
     int *d_sorted[3] = {nullptr, nullptr, nullptr};
+    // per class, the primitive pairs of the sorted pairs as structure of
+    // arrays: field f of primitive j of rank r at [(j * kKetFields + f) * np + r]
+    double *d_ksoa[3] = {nullptr, nullptr, nullptr};
     double *d_schwarz = nullptr;
     bool have_schwarz = false;
 

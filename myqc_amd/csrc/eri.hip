@@ -91,9 +91,50 @@ template <> struct PairL<0> { static constexpr int a = 0, b = 0; };
 template <> struct PairL<1> { static constexpr int a = 1, b = 0; };
 template <> struct PairL<2> { static constexpr int a = 1, b = 1; };
 
+// Ket primitive pairs as structure of arrays in Schwarz-rank order, so that
+// the lanes of a wavefront (consecutive ket ranks) read each field with one
+// coalesced load: p, P[3], PA[3], K, EIJ, 1/p.  STO-3G pairs have 3 x 3
+// primitive pairs; more are refused when the arrays are built.
+constexpr int kKetFields = 10, kKetPrims = 9;
+
+__global__ void k_ket_soa(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
+                          const int *__restrict__ sorted, int np, double *__restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (r >= np) return;
+    const ShellPair sp = pairs[sorted[r]];
+    PrimPair q{};
+    if (j < sp.npp) q = prims[sp.pp0 + j];
+    const double f[kKetFields] = {q.p, q.P[0], q.P[1], q.P[2], q.PA[0], q.PA[1], q.PA[2], q.K, q.EIJ, q.ip};
+#pragma unroll
+    for (int k = 0; k < kKetFields; ++k) out[((size_t)j * kKetFields + k) * np + r] = f[k];
+}
+
+struct KetSoa {
+    const double *__restrict__ s;
+    long long np, r;
+    __device__ __forceinline__ PrimPair operator()(int j) const {
+        const double *b = s + (size_t)j * kKetFields * np + r;
+        PrimPair q;
+        q.p = b[0];
+        q.P[0] = b[np];
+        q.P[1] = b[2 * np];
+        q.P[2] = b[3 * np];
+        q.PA[0] = b[4 * np];
+        q.PA[1] = b[5 * np];
+        q.PA[2] = b[6 * np];
+        q.K = b[7 * np];
+        q.EIJ = b[8 * np];
+        q.beta = 0.0;
+        q.ip = b[9 * np];
+        return q;
+    }
+};
+
 struct EriArgs {
     const ShellPair *pairs;
     const PrimPair *prims;
+    const double *ksoa;        // ket primitives of the ket class (k_ket_soa)
+    long long nket;            // pairs in the ket class
     const double *ftT;
     const int *sorted_b;
     const int *sorted_k;
@@ -165,7 +206,7 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     const int pb = a.sorted_b[rb], pk = a.sorted_k[rk];
     const ShellPair bra = a.pairs[pb], ket = a.pairs[pk];
     QuartetAcc<LA, LB, LC, LD> acc;
-    quartet_prims<LA, LB, LC, LD>(bra, ket, a.prims, a.prims, s_ft, acc);
+    quartet_prims_f<LA, LB, LC, LD>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, rk}, s_ft, acc);
     double CD[3] = {ket.AB[0], ket.AB[1], ket.AB[2]};
     double out[BLK];
     quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, CD, out);
@@ -297,6 +338,10 @@ int eri_schwarz(myqc_ctx *ctx) {
     MYQC_HIP(ctx, hipMemcpyAsync(ctx->schwarz.data(), ctx->d_schwarz, sizeof(double) * np, hipMemcpyDeviceToHost,
                                  ctx->stream));
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (const ShellPair &sp : ctx->pairs)
+        if (sp.npp > kKetPrims)
+            return set_error(ctx, MYQC_E_ARG, "a shell pair has %d primitive pairs (at most %d: STO-3G shells)",
+                             sp.npp, kKetPrims);
     // sort each class by Schwarz factor, largest first (ties by pair id)
     for (int c = 0; c < 3; ++c) {
         auto &ids = ctx->pair_ids[c];
@@ -304,9 +349,16 @@ int eri_schwarz(myqc_ctx *ctx) {
         for (size_t r = 0; r < ids.size(); ++r) key[r] = {-ctx->schwarz[ids[r]], ids[r]};
         std::sort(key.begin(), key.end());   // descending factor, ties by pair id
         for (size_t r = 0; r < ids.size(); ++r) ids[r] = key[r].second;
-        if (!ids.empty())
+        if (!ids.empty()) {
             MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_sorted[c], ids.data(), sizeof(int) * ids.size(),
                                          hipMemcpyHostToDevice, ctx->stream));
+            const int np = (int)ids.size();
+            MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_ksoa[c], sizeof(double) * kKetPrims * kKetFields * (size_t)np));
+            k_ket_soa<<<dim3((np + 255) / 256, kKetPrims), 256, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_prims,
+                                                                                  ctx->d_sorted[c], np,
+                                                                                  ctx->d_ksoa[c]);
+            MYQC_HIP(ctx, hipGetLastError());
+        }
     }
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->have_schwarz = true;
@@ -343,6 +395,8 @@ static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX) {
     a.ftT = ctx->d_ftT;
     a.sorted_b = ctx->d_sorted[CB];
     a.sorted_k = ctx->d_sorted[CK];
+    a.ksoa = ctx->d_ksoa[CK];
+    a.nket = (long long)ctx->pair_ids[CK].size();
     a.cum = qc.d_cum;
     a.nbra = qc.nbra;
     a.nq = qc.nq;

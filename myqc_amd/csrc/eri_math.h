@@ -166,10 +166,10 @@ struct QuartetAcc {
 // SCREEN = false skips the reference's primitive-pair screen: used for the
 // Schwarz factors, since a primitive pair dropped from (ab|ab) by
 // EIJ^2 < 1e-14 can survive EIJ*EGH >= 1e-14 against a compact ket pair.
-template <int LA, int LB, int LC, int LD, bool SCREEN = true>
-MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
-                           const PrimPair *__restrict__ ppk, const double *__restrict__ ftT,
-                           QuartetAcc<LA, LB, LC, LD> &acc) {
+// The ket's primitive pairs come from kprim(j), j < ket.npp.
+template <int LA, int LB, int LC, int LD, bool SCREEN = true, typename KF>
+MYQC_HD void quartet_prims_f(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
+                             KF &&kprim, const double *__restrict__ ftT, QuartetAcc<LA, LB, LC, LD> &acc) {
     using A = QuartetAcc<LA, LB, LC, LD>;
     constexpr int EB = A::EB, EK = A::EK, M = A::M;
     constexpr int NB = nupto(EB);
@@ -182,7 +182,7 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
     for (int i = 0; i < bra.npp; ++i) {
         const PrimPair b = ppb[bra.pp0 + i];
         for (int j = 0; j < ket.npp; ++j) {
-            const PrimPair k = ppk[ket.pp0 + j];
+            const PrimPair k = kprim(j);
             if (SCREEN && b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
             const double p = b.p, q = k.p, pq = p + q;
             const double ipq = 1.0 / pq;
@@ -262,6 +262,13 @@ MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const Pri
             }
         }
     }
+}
+
+template <int LA, int LB, int LC, int LD, bool SCREEN = true>
+MYQC_HD void quartet_prims(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
+                           const PrimPair *__restrict__ ppk, const double *__restrict__ ftT,
+                           QuartetAcc<LA, LB, LC, LD> &acc) {
+    quartet_prims_f<LA, LB, LC, LD, SCREEN>(bra, ket, ppb, [&](int j) { return ppk[ket.pp0 + j]; }, ftT, acc);
 }
 
 // Horizontal recursion: contracted (ab|cd) with a,b,c,d component indices
