@@ -372,6 +372,9 @@ static void class_counts(myqc_ctx *ctx, int cb, int ck, double tol, std::vector<
     cum.assign(B.size() + 1, 0);
     std::vector<double> qk(K.size());
     for (size_t i = 0; i < K.size(); ++i) qk[i] = ctx->schwarz[K[i]];
+    // bras in descending Schwarz order need ascending thresholds: the number
+    // of kets with qk >= tol / qb only shrinks, so one pointer walks down
+    size_t ptr = qk.size();
     for (size_t r = 0; r < B.size(); ++r) {
         const double qb = ctx->schwarz[B[r]];
         long long ub;
@@ -379,8 +382,8 @@ static void class_counts(myqc_ctx *ctx, int cb, int ck, double tol, std::vector<
         else if (qb <= 0.0) ub = 0;
         else {
             const double need = tol / qb;
-            // number of kets with qk >= need (qk sorted descending)
-            ub = std::lower_bound(qk.begin(), qk.end(), need, [](double a, double b) { return a >= b; }) - qk.begin();
+            while (ptr > 0 && !(qk[ptr - 1] >= need)) --ptr;
+            ub = (long long)ptr;
         }
         long long cnt = (cb == ck) ? std::max(0LL, ub - (long long)r) : ub;
         cum[r + 1] = cum[r] + cnt;
@@ -490,18 +493,27 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
     }
     // screened ket count of every pair in each ket class: kets with
     // q_P q_Q >= tol form a prefix of the Schwarz-sorted class list
+    // (each class's pairs in descending Schwarz order need ascending
+    // thresholds tol / q_p, so one pointer per ket class walks down once)
     std::vector<int> pcnt((size_t)np * 3);
-    for (int p = 0; p < np; ++p) {
-        const double qp = ctx->schwarz[p];
+    for (int cb = 0; cb < 3; ++cb) {
+        const auto &ids = ctx->pair_ids[cb];
         for (int ck = 0; ck < 3; ++ck) {
             const auto &qk = qsorted[ck];
-            long long cnt;
-            if (tol <= 0.0) cnt = (long long)qk.size();
-            else if (qp <= 0.0) cnt = 0;
-            else
-                cnt = std::lower_bound(qk.begin(), qk.end(), tol / qp, [](double a, double b) { return a >= b; }) -
-                      qk.begin();
-            pcnt[(size_t)p * 3 + ck] = (int)cnt;
+            size_t ptr = qk.size();
+            for (size_t r = 0; r < ids.size(); ++r) {
+                const int p = ids[r];
+                const double qp = ctx->schwarz[p];
+                long long cnt;
+                if (tol <= 0.0) cnt = (long long)qk.size();
+                else if (qp <= 0.0) cnt = 0;
+                else {
+                    const double thr = tol / qp;
+                    while (ptr > 0 && !(qk[ptr - 1] >= thr)) --ptr;
+                    cnt = (long long)ptr;
+                }
+                pcnt[(size_t)p * 3 + ck] = (int)cnt;
+            }
         }
     }
     ctx->segoff.assign((size_t)ns * ns * 3, 0);
@@ -532,6 +544,16 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
     ctx->dig_rows = std::max(3, std::min(64, (int)(kLdsBudget / (sizeof(double) * (size_t)n)) - 3));
     std::vector<int> wg, unit, chunk, shell(4 * ns);
     std::vector<long long> unit_off, wg_cost;
+    {
+        long long nchunk = 0, nunit = 0;
+        for (size_t seg = 0; seg < ctx->segcnt.size(); ++seg) {
+            nchunk += (ctx->segcnt[seg] + 63) / 64;
+            nunit += (seg % 3 == 0) && (ctx->segcnt[seg] + ctx->segcnt[seg + 1] + ctx->segcnt[seg + 2] > 0);
+        }
+        chunk.reserve(4 * (size_t)nchunk);
+        unit.reserve(4 * (size_t)nunit);
+        unit_off.reserve(3 * (size_t)nunit);
+    }
     for (int X = 0; X < ns; ++X) {
         shell[4 * X + 0] = ctx->shells[X].ao;
         shell[4 * X + 1] = ctx->shells[X].L;

@@ -222,8 +222,14 @@ __global__ void k_gather_rows(const double *__restrict__ partial, const int *__r
     if (j >= n) return;
     const int A = ao_shell[i];
     const int ia = i - shell[4 * A], w0 = shell[4 * A + 2], w1 = shell[4 * A + 3];
-    double s = 0.0;
-    for (int w = w0; w < w1; ++w) s += partial[((size_t)w * 3 + ia) * n + j];
+    // four independent sums so that the loads of consecutive slots overlap
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    int w = w0;
+    for (; w + 4 <= w1; w += 4)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s4[q] += partial[((size_t)(w + q) * 3 + ia) * n + j];
+    for (; w < w1; ++w) s4[0] += partial[((size_t)w * 3 + ia) * n + j];
+    const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     Graw[i + (size_t)n * j] = s;
 }
 

@@ -22,7 +22,7 @@ namespace myqc {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-// slots: {tr P0, ||P0||_F^2} at slots[0..1]; bounds at slots[6..7]; control words at slots + 8
+// slots: bounds at slots[6..7]; control words at slots + 8
 __global__ __launch_bounds__(64) void k_gersh_rows(const double *__restrict__ F, int n, double *__restrict__ lohi) {
     const int i = blockIdx.x, lane = threadIdx.x;
     double r = 0.0;
@@ -69,15 +69,18 @@ __device__ __forceinline__ double wsum(double v) {
     return v;
 }
 
-// P0 = (emax I - F)/(emax - emin), with its trace and Frobenius norm into slot 0
-__global__ __launch_bounds__(64) void k_sp2_start(const double *__restrict__ F, double *__restrict__ P, int n,
-                                                 double *__restrict__ slots) {
-    const size_t e = (size_t)blockIdx.x * 64 + threadIdx.x;
+// P0 = (emax I - F)/(emax - emin): one workgroup per 16x16 tile, writing the
+// tile's trace and Frobenius-norm partials to tout (read by step 0)
+__global__ __launch_bounds__(256) void k_sp2_start(const double *__restrict__ F, double *__restrict__ P, int n,
+                                                  const double *__restrict__ slots, double *__restrict__ tout) {
+    __shared__ double s_t[4], s_f[4];
+    const int tiles = (n + 15) >> 4, tile = blockIdx.x;
+    const int i = (tile % tiles) * 16 + (threadIdx.x & 15), j = (tile / tiles) * 16 + (threadIdx.x >> 4);
     const double lo = slots[6], hi = slots[7];
     const double s = 1.0 / (hi - lo);
     double t = 0.0, f = 0.0;
-    if (e < (size_t)n * n) {
-        const int i = (int)(e % n), j = (int)(e / n);
+    if (i < n && j < n) {
+        const size_t e = i + (size_t)n * j;
         const double v = ((i == j ? hi : 0.0) - F[e]) * s;
         P[e] = v;
         f = v * v;
@@ -85,9 +88,15 @@ __global__ __launch_bounds__(64) void k_sp2_start(const double *__restrict__ F, 
     }
     t = wsum(t);
     f = wsum(f);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_t[w] = t;
+        s_f[w] = f;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        unsafeAtomicAdd(&slots[0], t);
-        unsafeAtomicAdd(&slots[1], f);
+        tout[2 * tile] = s_t[0] + s_t[1] + s_t[2] + s_t[3];
+        tout[2 * tile + 1] = s_f[0] + s_f[1] + s_f[2] + s_f[3];
     }
 }
 
@@ -99,8 +108,8 @@ __global__ __launch_bounds__(64) void k_sp2_start(const double *__restrict__ F, 
 // the GPU test of the builtin checks).
 //
 // The traces of P that the choice needs arrive as per-tile partials written
-// by step k-1 (tin, nin of them; step 0 reads the one pair k_sp2_start
-// summed), and this step writes Pn's per-tile partials to tout: no atomics,
+// by step k-1 (tin, nin of them; step 0 reads those of k_sp2_start), and this
+// step writes Pn's per-tile partials to tout: no atomics,
 // and every workgroup sums the same partials in the same order, so all of
 // them make the same choice.  Every load a step needs (the partials, the
 // control word, its MFMA operands and its own tile of P) is issued before
@@ -234,9 +243,10 @@ int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double 
                int n, int nocc, int *ok, int *steps) {
     *ok = 0;
     MYQC_CHECK(gershgorin_bounds(ctx, Fo, n, lohi, slots));
-    const size_t nn = (size_t)n * n;
-    k_sp2_start<<<(unsigned)((nn + 63) / 64), 64, 0, ctx->stream>>>(Fo, P, n, slots);
     const int tiles = (n + 15) / 16;
+    const size_t tt = (size_t)tiles * tiles;
+    // partials of step k go to tpart + 2 tt (k & 1); k_sp2_start plays step -1
+    k_sp2_start<<<(unsigned)tt, 256, 0, ctx->stream>>>(Fo, P, n, slots, tpart + 2 * tt);
     // launch about as many steps as the previous purification took (plus a
     // margin); steps past convergence cost an empty launch, and only the
     // status read at the end waits for the device
@@ -248,13 +258,10 @@ int sp2_purify(myqc_ctx *ctx, const double *Fo, double *P, double *work, double 
         for (; launched < target; ++launched) {
             const double *cur = (launched & 1) ? work : P;
             double *nxt = (launched & 1) ? P : work;
-            // traces in: slots[0..1] (k_sp2_start) for step 0, else the
-            // previous step's per-tile partials; out: this step's partials
-            const double *tin = launched == 0 ? slots : tpart + 2 * (size_t)tiles * tiles * ((launched - 1) & 1);
-            double *tout = tpart + 2 * (size_t)tiles * tiles * (launched & 1);
-            k_sp2_step_mfma<<<tiles * tiles, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, tin,
-                                                                       launched == 0 ? 1 : tiles * tiles, tout,
-                                                                       launched, first, ctrl);
+            const double *tin = tpart + 2 * tt * ((launched + 1) & 1);   // step launched - 1
+            double *tout = tpart + 2 * tt * (launched & 1);
+            k_sp2_step_mfma<<<(unsigned)tt, 64 * KS, 0, ctx->stream>>>(cur, nxt, n, (double)nocc, tin, (int)tt, tout,
+                                                                      launched, first, ctrl);
         }
         k_sp2_finish<<<64, 256, 0, ctx->stream>>>(P, work, n, ctrl);
         MYQC_HIP(ctx, hipGetLastError());
