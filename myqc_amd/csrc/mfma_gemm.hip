@@ -3,13 +3,19 @@
 //
 //   C = op(A) B,   op(A) = A or A^T,   all column-major FP64 in HBM.
 //
-// Each FP64 operand element x is split on its way into LDS as
+// Each FP64 operand element x is split as
 //   hi = bf16(x),   lo = bf16(x - hi)            (16 significant bits in all)
 // and the product is accumulated as  hi*hi  +  (hi*lo + lo*hi)  with
 // v_mfma_f32_16x16x32_bf16; the dropped lo*lo term is 2^-16 relative.  The
 // main and correction sums live in separate FP32 accumulators and are added
 // in FP64 at the store.  Relative error per output ~1e-5 (bf16 split) plus
 // FP32 accumulation over k; the ao2mo/MP2 test states it against FP64.
+//
+// The quarter transforms multiply one large operand (the integrals or a
+// partly transformed tensor) by a small block of C: the small operand is split
+// once into hi/lo planes (k contiguous per row or column, read with 16-byte
+// loads), the large one is split on its way into LDS, its next k-step loaded
+// into registers while the current one's MFMAs run.
 //
 // Tiling: a 256-thread workgroup owns a 128 x 64 tile of C; wavefront w
 // computes rows 64 (w & 1) .. +63 and columns 32 (w >> 1) .. +31 as 4 x 2
@@ -41,96 +47,6 @@ __device__ __forceinline__ void split(double x, uint16_t &hi, uint16_t &lo) {
     lo = bf16_bits(f - bf16_value(hi));
 }
 
-template <bool TRANSA>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(int m, int n, int k, const double *__restrict__ A, long long lda,
-                                                     long long sA, const double *__restrict__ B, long long ldb,
-                                                     long long sB, double *__restrict__ C, long long ldc, long long sC,
-                                                     long long tilesN) {
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[GM][GKP], sAl[GM][GKP], sBh[GN][GKP], sBl[GN][GKP];
-    const long long bx = blockIdx.x;
-    const long long bat = bx / tilesN, tn = bx - bat * tilesN;
-    const int m0 = blockIdx.y * GM;
-    const long long n0 = tn * GN;
-    A += bat * sA;
-    B += bat * sB;
-    C += bat * sC;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm = (w & 1) * 64, wn = (w >> 1) * 32;
-    const int r16 = lane & 15, q = lane >> 4;
-    f32x4 acc[4][2], cor[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-            cor[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-        }
-    for (int k0 = 0; k0 < k; k0 += GK) {
-        // A tile (GM x GK) -> sA[row][k]
-        if (TRANSA) {
-            // op(A)(i, kk) = A[kk + lda i]: 16 consecutive k of one row per thread
-            const int i = t >> 1, kb = (t & 1) * 16;
-            const int gi = m0 + i;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int gk = k0 + kb + e;
-                const double x = (gi < m && gk < k) ? A[gk + lda * gi] : 0.0;
-                split(x, sAh[i][kb + e], sAl[i][kb + e]);
-            }
-        } else {
-            // op(A)(i, kk) = A[i + lda kk]: 16 consecutive rows at one k per thread
-            const int kk = t >> 3, ib = (t & 7) * 16;
-            const int gk = k0 + kk;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int gi = m0 + ib + e;
-                const double x = (gi < m && gk < k) ? A[gi + lda * gk] : 0.0;
-                split(x, sAh[ib + e][kk], sAl[ib + e][kk]);
-            }
-        }
-        // B tile (GK x GN) -> sB[col][k]: 8 consecutive k of one column per thread
-        {
-            const int j = t >> 2, kb = (t & 3) * 8;
-            const long long gj = n0 + j;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int gk = k0 + kb + e;
-                const double x = (gj < n && gk < k) ? B[gk + ldb * gj] : 0.0;
-                split(x, sBh[j][kb + e], sBl[j][kb + e]);
-            }
-        }
-        __syncthreads();
-        bf16x8 bh[2], bl[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            bh[j] = *reinterpret_cast<const bf16x8 *>(&sBh[wn + 16 * j + r16][8 * q]);
-            bl[j] = *reinterpret_cast<const bf16x8 *>(&sBl[wn + 16 * j + r16][8 * q]);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&sAh[wm + 16 * i + r16][8 * q]);
-            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&sAl[wm + 16 * i + r16][8 * q]);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-                cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], cor[i][j], 0, 0, 0);
-                cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], cor[i][j], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int gi = m0 + wm + 16 * i + 4 * q + v;
-                const long long gj = n0 + wn + 16 * j + r16;
-                if (gi < m && gj < n) C[gi + ldc * gj] = (double)acc[i][j][v] + (double)cor[i][j][v];
-            }
-}
-
 // hi/lo planes of op(A) = A^T (A column-major, lda): rows i < m of length
 // kp (k rounded up to GK), zero beyond k and beyond m (up to mp rows)
 __global__ void k_split_rows_t(const double *__restrict__ A, long long lda, int m, int k, int mp, int kp,
@@ -146,7 +62,7 @@ __global__ void k_split_rows_t(const double *__restrict__ A, long long lda, int 
 // C = A^T B with A's hi/lo planes pre-split (k_split_rows_t) and B (k x n,
 // k-contiguous columns) split on its way into LDS with 16-byte vector loads;
 // the next k-step's B values are loaded into registers while this step's
-// MFMAs run.  Same tiling and accumulation as k_gemm_bf16x3.
+// MFMAs run.  Tiling and accumulation as described at the top of this file.
 __global__ __launch_bounds__(256) void k_gemm_bf16x3_tn(int m, int n, int k, int kp, const uint16_t *__restrict__ Ahi,
                                                         const uint16_t *__restrict__ Alo, const double *__restrict__ B,
                                                         long long ldb, double *__restrict__ C, long long ldc,
@@ -239,6 +155,89 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_tn(int m, int n, int k, int
             }
 }
 
+// C = A B for `batch` strided problems with A (m x k, column-major, lda,
+// FP64) split on its way into LDS and B's hi/lo planes pre-split once
+// (k_split_rows_t over B's columns; B is shared by the batch).  A wavefront
+// loads 64 consecutive rows of A at one k (coalesced); the next k-step's A
+// values are loaded into registers while this step's MFMAs run.  Tiling and
+// accumulation as described at the top of this file.
+__global__ __launch_bounds__(256) void k_gemm_bf16x3_nn(int m, int n, int k, int kp, const double *__restrict__ A,
+                                                        long long lda, long long sA, const uint16_t *__restrict__ Bhi,
+                                                        const uint16_t *__restrict__ Blo, double *__restrict__ C,
+                                                        long long ldc, long long sC, long long tilesN) {
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[GM][GKP], sAl[GM][GKP], sBh[GN][GKP], sBl[GN][GKP];
+    const long long bx = blockIdx.x;
+    const long long bat = bx / tilesN, tn = bx - bat * tilesN;
+    const int m0 = blockIdx.y * GM;
+    const long long n0 = tn * GN;
+    A += bat * sA;
+    C += bat * sC;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w & 1) * 64, wn = (w >> 1) * 32;
+    const int r16 = lane & 15, q = lane >> 4;
+    // A loader: row ar of the tile, k = ah + 2 e (e < 16)
+    const int ar = t & 127, ah = t >> 7;
+    const int gi = m0 + ar;
+    double na[16];
+    auto load_a = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int gk = k0 + ah + 2 * e;
+            na[e] = (gi < m && gk < k) ? A[gi + lda * gk] : 0.0;
+        }
+    };
+    // B loader: column bj of the tile, 8 consecutive k
+    const int bj = t >> 2, bk = (t & 3) * 8;
+    f32x4 acc[4][2], cor[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            cor[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        }
+    load_a(0);
+    for (int k0 = 0; k0 < k; k0 += GK) {
+        {
+            const size_t src = (size_t)(n0 + bj) * kp + k0 + bk;
+            *reinterpret_cast<uint4 *>(&sBh[bj][bk]) = *reinterpret_cast<const uint4 *>(Bhi + src);
+            *reinterpret_cast<uint4 *>(&sBl[bj][bk]) = *reinterpret_cast<const uint4 *>(Blo + src);
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) split(na[e], sAh[ar][ah + 2 * e], sAl[ar][ah + 2 * e]);
+        if (k0 + GK < k) load_a(k0 + GK);
+        __syncthreads();
+        bf16x8 bh[2], bl[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            bh[j] = *reinterpret_cast<const bf16x8 *>(&sBh[wn + 16 * j + r16][8 * q]);
+            bl[j] = *reinterpret_cast<const bf16x8 *>(&sBl[wn + 16 * j + r16][8 * q]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bf16x8 ahv = *reinterpret_cast<const bf16x8 *>(&sAh[wm + 16 * i + r16][8 * q]);
+            const bf16x8 alv = *reinterpret_cast<const bf16x8 *>(&sAl[wm + 16 * i + r16][8 * q]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bh[j], acc[i][j], 0, 0, 0);
+                cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bl[j], cor[i][j], 0, 0, 0);
+                cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bh[j], cor[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gii = m0 + wm + 16 * i + 4 * q + v;
+                const long long gjj = n0 + wn + 16 * j + r16;
+                if (gii < m && gjj < n) C[gii + ldc * gjj] = (double)acc[i][j][v] + (double)cor[i][j][v];
+            }
+}
+
 // C (m x n, ldc) = op(A) (m x k) * B (k x n, ldb) for `batch` strided
 // problems; FP64 in and out, split-bf16 MFMA products inside.  The transposed
 // single problem (the first quarter transform: a small A against a large,
@@ -250,7 +249,8 @@ int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A
     const long long gx = tilesN * batch;
     const int gy = (m + GM - 1) / GM;
     if (gx > 0x7fffffffLL || gy > 65535) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: grid too large");
-    if (transA && batch == 1) {
+    if (transA && batch != 1) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: op(A) = A^T takes one problem");
+    if (transA) {
         const int mp = gy * GM, kp = ((k + GK - 1) / GK) * GK;
         uint16_t *planes = nullptr;
         MYQC_CHECK(dalloc(ctx, (void **)&planes, sizeof(uint16_t) * 2 * (size_t)mp * kp));
@@ -264,14 +264,18 @@ int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A
         dfree((void **)&planes);
         return 0;
     }
-    const dim3 grid((unsigned)gx, (unsigned)gy);
-    if (transA)
-        k_gemm_bf16x3<true><<<grid, 256, 0, ctx->stream>>>(m, n, k, A, lda, strideA, B, ldb, strideB, C, ldc, strideC,
-                                                            tilesN);
-    else
-        k_gemm_bf16x3<false><<<grid, 256, 0, ctx->stream>>>(m, n, k, A, lda, strideA, B, ldb, strideB, C, ldc, strideC,
-                                                             tilesN);
+    // op(A) = A: B (shared by the batch) is split into planes once
+    if (strideB != 0) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: B must be shared by the batch");
+    const int np_ = (int)(tilesN * GN), kp = ((k + GK - 1) / GK) * GK;
+    uint16_t *planes = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&planes, sizeof(uint16_t) * 2 * (size_t)np_ * kp));
+    uint16_t *hi = planes, *lo = planes + (size_t)np_ * kp;
+    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(B, ldb, n, k, np_, kp, hi, lo);
+    k_gemm_bf16x3_nn<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(m, n, k, kp, A, lda, strideA, hi, lo, C,
+                                                                                ldc, strideC, tilesN);
     MYQC_HIP(ctx, hipGetLastError());
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dfree((void **)&planes);
     return 0;
 }
 
