@@ -10,6 +10,7 @@
 #include <ctime>
 #include <string>
 #include <unordered_map>
+#include <cstdint>
 #include <vector>
 
 #include "../../include/myqc_hip.h"
@@ -109,6 +110,7 @@ struct myqc_ctx {
 
     // ao2mo output kept on the device for mp2
     double *d_iajb = nullptr;             // [i][a][j][b] with i fastest (o*v*o*v)
+    uint16_t *d_planes = nullptr;         // split-bf16 GEMM: hi/lo planes of the small operand
     size_t iajb_elems = 0;
     int iajb_nocc = 0, iajb_nv = 0;
 

@@ -252,30 +252,24 @@ int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A
     if (transA && batch != 1) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: op(A) = A^T takes one problem");
     if (transA) {
         const int mp = gy * GM, kp = ((k + GK - 1) / GK) * GK;
-        uint16_t *planes = nullptr;
-        MYQC_CHECK(dalloc(ctx, (void **)&planes, sizeof(uint16_t) * 2 * (size_t)mp * kp));
-        uint16_t *hi = planes, *lo = planes + (size_t)mp * kp;
+        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_planes, sizeof(uint16_t) * 2 * (size_t)mp * kp));
+        uint16_t *hi = ctx->d_planes, *lo = ctx->d_planes + (size_t)mp * kp;
         k_split_rows_t<<<256, 256, 0, ctx->stream>>>(A, lda, m, k, mp, kp, hi, lo);
         const int vec_ok = (ldb % 2 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
         k_gemm_bf16x3_tn<<<dim3((unsigned)tilesN, (unsigned)gy), 256, 0, ctx->stream>>>(m, n, k, kp, hi, lo, B, ldb, C,
                                                                                       ldc, vec_ok);
         MYQC_HIP(ctx, hipGetLastError());
-        MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        dfree((void **)&planes);
         return 0;
     }
     // op(A) = A: B (shared by the batch) is split into planes once
     if (strideB != 0) return set_error(ctx, MYQC_E_ARG, "gemm_bf16x3: B must be shared by the batch");
     const int np_ = (int)(tilesN * GN), kp = ((k + GK - 1) / GK) * GK;
-    uint16_t *planes = nullptr;
-    MYQC_CHECK(dalloc(ctx, (void **)&planes, sizeof(uint16_t) * 2 * (size_t)np_ * kp));
-    uint16_t *hi = planes, *lo = planes + (size_t)np_ * kp;
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_planes, sizeof(uint16_t) * 2 * (size_t)np_ * kp));
+    uint16_t *hi = ctx->d_planes, *lo = ctx->d_planes + (size_t)np_ * kp;
     k_split_rows_t<<<256, 256, 0, ctx->stream>>>(B, ldb, n, k, np_, kp, hi, lo);
     k_gemm_bf16x3_nn<<<dim3((unsigned)gx, (unsigned)gy), 256, 0, ctx->stream>>>(m, n, k, kp, A, lda, strideA, hi, lo, C,
                                                                                 ldc, strideC, tilesN);
     MYQC_HIP(ctx, hipGetLastError());
-    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    dfree((void **)&planes);
     return 0;
 }
 
