@@ -163,29 +163,36 @@ struct BScfArgs {
     int max_iter;
 };
 
+// Workgroup reductions: a wavefront butterfly, then the per-wave results
+// summed in a fixed order (every thread returns the same value).
+__device__ __forceinline__ double wave_red_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_red_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 __device__ __forceinline__ double wg_sum(double v, double *s_red) {
+    v = wave_red_sum(v);
+    __syncthreads();   // s_red free
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
     __syncthreads();
-    s_red[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
-        __syncthreads();
-    }
-    const double r = s_red[0];
-    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_red[w];
     return r;
 }
 
 __device__ __forceinline__ double wg_max(double v, double *s_red) {
+    v = wave_red_max(v);
     __syncthreads();
-    s_red[threadIdx.x] = v;
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s_red[threadIdx.x] = fmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    const double r = s_red[0];
-    __syncthreads();
+    double r = -1e300;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, s_red[w]);
     return r;
 }
 
@@ -277,6 +284,63 @@ __device__ void jacobi(double *A, double *U, int n, double *s_rot, double *s_red
     }
 }
 
+__device__ __forceinline__ void wg_sum2(double &x, double &y, double *s_red) {
+    x = wave_red_sum(x);
+    y = wave_red_sum(y);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        s_red[2 * (threadIdx.x >> 6)] = x;
+        s_red[2 * (threadIdx.x >> 6) + 1] = y;
+    }
+    __syncthreads();
+    x = y = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        x += s_red[2 * w];
+        y += s_red[2 * w + 1];
+    }
+}
+
+// Projector onto the nocc lowest levels of the symmetric A (n x n, LDS, left
+// unchanged) into P by trace-correcting purification (Niklasson, PRB 66,
+// 155115 (2002)): P0 = (hi I - A)/(hi - lo) from Gershgorin bounds, then
+// P <- P^2 or 2P - P^2, whichever trace lands nearer nocc (tr P^2 = ||P||_F^2
+// for symmetric P).  Two steps after P is idempotent to 1e-8 with trace nocc
+// to 1e-6 it is final.  T: n x n LDS scratch.  False when that does not
+// happen within 100 steps (near-degenerate frontier levels).
+__device__ bool purify_lds(const double *A, double *P, double *T, int n, int nocc, double *s_red) {
+    double lo = 1e300, hi = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double r = 0.0;
+        for (int j = 0; j < n; ++j)
+            if (j != i) r += fabs(A[i + n * j]);
+        const double c = A[i * (n + 1)];
+        lo = fmin(lo, c - r);
+        hi = fmax(hi, c + r);
+    }
+    lo = -wg_max(-lo, s_red);
+    hi = wg_max(hi, s_red);
+    const double sc = 1.0 / fmax(hi - lo, 1e-300);
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) P[e] = ((e % n == e / n ? hi : 0.0) - A[e]) * sc;
+    int cs = -1;
+    for (int step = 0; step < 100; ++step) {
+        double t = 0.0, f = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const double v = P[e];
+            f += v * v;
+            if (e % n == e / n) t += v;
+        }
+        wg_sum2(t, f, s_red);   // also orders the previous step's writes of P
+        if (cs < 0 && step >= 8 && fabs(t - f) < 1e-8 && fabs(t - nocc) < 1e-6) cs = step;
+        if (cs >= 0 && step >= cs + 2) return true;
+        const bool square = fabs(f - nocc) < fabs(2.0 * t - f - nocc);
+        mm(T, P, P, n, false, 1.0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) P[e] = square ? T[e] : 2.0 * P[e] - T[e];
+    }
+    __syncthreads();
+    return false;
+}
+
 // indices of the nocc lowest eigenvalues (diagonal of A) into s_occ; thread 0
 __device__ void select_occ(const double *A, int n, int nocc, int *s_occ, int *s_tmp) {
     if (threadIdx.x == 0) {
@@ -338,57 +402,91 @@ __device__ bool diis_solve(const double *B, int MD, int m, double *M, double *c)
 }
 
 // G(D) = A + A^T accumulated from the packed unique integrals (degeneracy
-// weights 1/2 per index coincidence); F = H + G
+// weights 1/2 per index coincidence); F = H + G.  A wavefront takes one bra
+// pair IJ at a time and its lanes the kets KL <= IJ (a contiguous run of the
+// packed array), with the integrals loaded two steps ahead: the J(ij) term is
+// reduced across the wavefront, the other five go to LDS per value.
 __device__ void fock(const double *__restrict__ eri, const double *__restrict__ H, const double *D, double *Aacc,
                      double *F, int n, const unsigned char *s_pi, const unsigned char *s_pj) {
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Aacc[e] = 0.0;
+    // Aacc has an odd leading dimension so that the updates of a column
+    // (consecutive l, same i) fall in distinct LDS banks
+    const int la = n | 1;
+    for (int e = threadIdx.x; e < la * n; e += blockDim.x) Aacc[e] = 0.0;
     __syncthreads();
-    const long long P = tri(n), total = tri(P);
-    long long e = threadIdx.x;
-    long long IJ = (long long)floor((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
-    while (tri(IJ) > e) --IJ;
-    while (tri(IJ + 1) <= e) ++IJ;
-    long long KL = e - tri(IJ);
-    const int step = blockDim.x;
-    for (; e < total; e += step) {
-        double v = eri[e];
-        const int i = s_pi[IJ], j = s_pj[IJ], k = s_pi[KL], l = s_pj[KL];
-        if (i == j) v *= 0.5;
-        if (k == l) v *= 0.5;
-        if (IJ == KL) v *= 0.5;
-        atomicAdd(&Aacc[i + n * j], 2.0 * D[k + n * l] * v);
-        atomicAdd(&Aacc[k + n * l], 2.0 * D[i + n * j] * v);
-        atomicAdd(&Aacc[i + n * k], -0.5 * D[j + n * l] * v);
-        atomicAdd(&Aacc[j + n * k], -0.5 * D[i + n * l] * v);
-        atomicAdd(&Aacc[i + n * l], -0.5 * D[j + n * k] * v);
-        atomicAdd(&Aacc[j + n * l], -0.5 * D[i + n * k] * v);
-        KL += step;
-        while (KL > IJ) {
-            KL -= IJ + 1;
-            ++IJ;
+    const int P = (int)tri(n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    // the wavefront's steps are (IJ, first KL) with IJ = wave, wave + nwave,
+    // ... and KL advancing by 64 within a row; the integrals of the step two
+    // ahead are loaded while the current one is processed
+    auto advance = [&](int &ij, int &kl) {
+        kl += 64;
+        if (kl > ij) {
+            ij += nwave;
+            kl = 0;
+        }
+    };
+    auto load = [&](int ij, int kl) -> double {
+        const int x = kl + lane;
+        return (ij < P && x <= ij) ? eri[tri(ij) + x] : 0.0;
+    };
+    int pij = wave, pkl = 0;
+    double v0 = load(pij, pkl);
+    advance(pij, pkl);
+    double v1 = load(pij, pkl);
+    advance(pij, pkl);
+    int IJ = wave, KL0 = 0;
+    double jr = 0.0;
+    while (IJ < P) {
+        const double raw = v0;
+        v0 = v1;
+        v1 = load(pij, pkl);
+        advance(pij, pkl);
+        const int i = s_pi[IJ], j = s_pj[IJ];
+        const int KL = KL0 + lane;
+        if (KL <= IJ) {
+            const int k = s_pi[KL], l = s_pj[KL];
+            double v = (i == j) ? 0.5 * raw : raw;
+            if (k == l) v *= 0.5;
+            if (IJ == KL) v *= 0.5;
+            jr += 2.0 * D[k + n * l] * v;
+            const double dij = D[i + n * j];
+            atomicAdd(&Aacc[k + la * l], 2.0 * dij * v);
+            atomicAdd(&Aacc[i + la * k], -0.5 * D[j + n * l] * v);
+            atomicAdd(&Aacc[j + la * k], -0.5 * D[i + n * l] * v);
+            atomicAdd(&Aacc[i + la * l], -0.5 * D[j + n * k] * v);
+            atomicAdd(&Aacc[j + la * l], -0.5 * D[i + n * k] * v);
+        }
+        KL0 += 64;
+        if (KL0 > IJ) {   // row done (uniform across the wavefront)
+            jr = wave_red_sum(jr);
+            if (lane == 0) atomicAdd(&Aacc[i + la * j], jr);
+            jr = 0.0;
+            IJ += nwave;
+            KL0 = 0;
         }
     }
     __syncthreads();
     for (int x = threadIdx.x; x < n * n; x += blockDim.x) {
         const int i = x % n, j = x / n;
-        F[x] = H[x] + Aacc[i + n * j] + Aacc[j + n * i];
+        F[x] = H[x] + Aacc[i + la * j] + Aacc[j + la * i];
     }
     __syncthreads();
 }
 
-// LDS: L0 (G accumulator / Jacobi matrix), L1 (D), L2 (U), L3 (temp), each
-// nmax^2 doubles; then reductions, rotations, DIIS matrix and index tables.
+// LDS: L0 (G accumulator, (nmax | 1) nmax doubles / Jacobi matrix), L1 (D), L2
+// (U), L3 (temp), nmax^2 doubles each; then reductions, rotations, DIIS matrix and index tables.
 __host__ __device__ inline size_t scf_lds_bytes(int nmax, int threads) {
     const size_t n2 = (size_t)nmax * nmax;
-    return sizeof(double) * (4 * n2 + threads + 2 * (nmax + 2) + kBatchDiis * kBatchDiis + kBatchDiis + 8 +
+    return sizeof(double) * (4 * n2 + nmax + threads + 2 * (nmax + 2) + kBatchDiis * kBatchDiis + kBatchDiis + 8 +
                              (kBatchDiis + 1) * (kBatchDiis + 2)) +
            sizeof(int) * 2 * nmax + 2 * (size_t)tri(nmax) + 16;
 }
 
-__global__ __launch_bounds__(512) void k_batch_scf(BScfArgs a, int nmax) {
+// (at most 128 VGPRs: two 512-thread workgroups per CU)
+__global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
     extern __shared__ __attribute__((aligned(16))) double smem[];
     const size_t n2max = (size_t)nmax * nmax;
-    double *L0 = smem, *L1 = L0 + n2max, *L2 = L1 + n2max, *L3 = L2 + n2max;
+    double *L0 = smem, *L1 = L0 + n2max + nmax, *L2 = L1 + n2max, *L3 = L2 + n2max;   // L0: (nmax | 1) x nmax
     double *s_red = L3 + n2max;
     double *s_rot = s_red + blockDim.x;                  // 2 (nmax + 2)
     double *s_B = s_rot + 2 * (nmax + 2);                // kBatchDiis^2
@@ -540,10 +638,16 @@ __global__ __launch_bounds__(512) void k_batch_scf(BScfArgs a, int nmax) {
             E_old = E;
             head = (head + 1) % MD;
             __syncthreads();
-            // new orthonormal density from the nocc lowest levels
-            jacobi(L0, L2, n, s_rot, s_red);
-            select_occ(L0, n, nocc, s_occ, s_tmp);
-            occ_projector(Dp, L2, s_occ, n, nocc);
+            // new orthonormal density: the projector onto the nocc lowest
+            // levels of F' by purification, or by diagonalisation when the
+            // frontier levels are too close for it
+            if (purify_lds(L0, L2, L3, n, nocc, s_red)) {
+                for (int e = threadIdx.x; e < (int)nn; e += blockDim.x) Dp[e] = L2[e];
+            } else {
+                jacobi(L0, L2, n, s_rot, s_red);
+                select_occ(L0, n, nocc, s_occ, s_tmp);
+                occ_projector(Dp, L2, s_occ, n, nocc);
+            }
             __syncthreads();
             // keep D_k in L3 for the change; D_{k+1} = 2 X Dp X into L1
             for (int e = threadIdx.x; e < (int)nn; e += blockDim.x) L3[e] = L1[e];
