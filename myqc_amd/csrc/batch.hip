@@ -403,60 +403,83 @@ __device__ bool diis_solve(const double *B, int MD, int m, double *M, double *c)
 
 // G(D) = A + A^T accumulated from the packed unique integrals (degeneracy
 // weights 1/2 per index coincidence); F = H + G.  A wavefront takes one bra
-// pair IJ at a time and its lanes the kets KL <= IJ (a contiguous run of the
-// packed array), with the integrals loaded two steps ahead: the J(ij) term is
-// reduced across the wavefront, the other five go to LDS per value.
+// pair IJ at a time and each lane four consecutive kets KL <= IJ (a
+// contiguous run of the packed array), loaded a step ahead.  The J(ij)
+// term is reduced across the wavefront; the K terms on (i,k) and (j,k), whose
+// k repeats along a run of kets, are summed in registers while k stays the
+// same; the other three go to LDS per value.  Aacc and the copy Dl of D have
+// an odd leading dimension, so the updates and reads of a column (same i,
+// consecutive l) fall in distinct LDS banks.
 __device__ void fock(const double *__restrict__ eri, const double *__restrict__ H, const double *D, double *Aacc,
-                     double *F, int n, const unsigned char *s_pi, const unsigned char *s_pj) {
-    // Aacc has an odd leading dimension so that the updates of a column
-    // (consecutive l, same i) fall in distinct LDS banks
+                     double *Dl, double *F, int n, const unsigned char *s_pi, const unsigned char *s_pj) {
     const int la = n | 1;
-    for (int e = threadIdx.x; e < la * n; e += blockDim.x) Aacc[e] = 0.0;
+    for (int e = threadIdx.x; e < la * n; e += blockDim.x) {
+        const int i = e % la, j = e / la;
+        Aacc[e] = 0.0;
+        Dl[e] = i < n ? D[i + n * j] : 0.0;
+    }
     __syncthreads();
     const int P = (int)tri(n);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    // the wavefront's steps are (IJ, first KL) with IJ = wave, wave + nwave,
-    // ... and KL advancing by 64 within a row; the integrals of the step two
-    // ahead are loaded while the current one is processed
+    constexpr int U = 4, STEP = 64 * U;
     auto advance = [&](int &ij, int &kl) {
-        kl += 64;
+        kl += STEP;
         if (kl > ij) {
             ij += nwave;
             kl = 0;
         }
     };
-    auto load = [&](int ij, int kl) -> double {
-        const int x = kl + lane;
-        return (ij < P && x <= ij) ? eri[tri(ij) + x] : 0.0;
+    auto load = [&](int ij, int kl, double (&v)[U]) {
+        const int x = kl + U * lane;
+        const double *row = eri + tri(ij);
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = (ij < P && x + u <= ij) ? row[x + u] : 0.0;
     };
     int pij = wave, pkl = 0;
-    double v0 = load(pij, pkl);
-    advance(pij, pkl);
-    double v1 = load(pij, pkl);
+    double v0[U], v2[U];
+    load(pij, pkl, v0);
     advance(pij, pkl);
     int IJ = wave, KL0 = 0;
     double jr = 0.0;
     while (IJ < P) {
-        const double raw = v0;
-        v0 = v1;
-        v1 = load(pij, pkl);
+#pragma unroll
+        for (int u = 0; u < U; ++u) v2[u] = v0[u];
+        load(pij, pkl, v0);
         advance(pij, pkl);
         const int i = s_pi[IJ], j = s_pj[IJ];
-        const int KL = KL0 + lane;
-        if (KL <= IJ) {
-            const int k = s_pi[KL], l = s_pj[KL];
-            double v = (i == j) ? 0.5 * raw : raw;
-            if (k == l) v *= 0.5;
-            if (IJ == KL) v *= 0.5;
-            jr += 2.0 * D[k + n * l] * v;
-            const double dij = D[i + n * j];
-            atomicAdd(&Aacc[k + la * l], 2.0 * dij * v);
-            atomicAdd(&Aacc[i + la * k], -0.5 * D[j + n * l] * v);
-            atomicAdd(&Aacc[j + la * k], -0.5 * D[i + n * l] * v);
-            atomicAdd(&Aacc[i + la * l], -0.5 * D[j + n * k] * v);
-            atomicAdd(&Aacc[j + la * l], -0.5 * D[i + n * k] * v);
+        const double dij = Dl[i + la * j];
+        const double wij = (i == j) ? 0.5 : 1.0;
+        int kc = -1;
+        double aik = 0.0, ajk = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int KL = KL0 + U * lane + u;
+            if (KL <= IJ) {
+                const int k = s_pi[KL], l = s_pj[KL];
+                double v = v2[u] * wij;
+                if (k == l) v *= 0.5;
+                if (IJ == KL) v *= 0.5;
+                if (k != kc) {
+                    if (kc >= 0) {
+                        atomicAdd(&Aacc[i + la * kc], aik);
+                        atomicAdd(&Aacc[j + la * kc], ajk);
+                    }
+                    kc = k;
+                    aik = ajk = 0.0;
+                }
+                jr += 2.0 * Dl[k + la * l] * v;
+                aik -= 0.5 * Dl[j + la * l] * v;
+                ajk -= 0.5 * Dl[i + la * l] * v;
+                atomicAdd(&Aacc[k + la * l], 2.0 * dij * v);
+                atomicAdd(&Aacc[i + la * l], -0.5 * Dl[j + la * k] * v);
+                atomicAdd(&Aacc[j + la * l], -0.5 * Dl[i + la * k] * v);
+            }
         }
-        KL0 += 64;
+        if (kc >= 0) {
+            atomicAdd(&Aacc[i + la * kc], aik);
+            atomicAdd(&Aacc[j + la * kc], ajk);
+        }
+        KL0 += STEP;
         if (KL0 > IJ) {   // row done (uniform across the wavefront)
             jr = wave_red_sum(jr);
             if (lane == 0) atomicAdd(&Aacc[i + la * j], jr);
@@ -473,11 +496,12 @@ __device__ void fock(const double *__restrict__ eri, const double *__restrict__ 
     __syncthreads();
 }
 
-// LDS: L0 (G accumulator, (nmax | 1) nmax doubles / Jacobi matrix), L1 (D), L2
-// (U), L3 (temp), nmax^2 doubles each; then reductions, rotations, DIIS matrix and index tables.
+// LDS: L0 (G accumulator / Jacobi matrix) and L2 (padded D in the Fock build
+// / U / purification), (nmax | 1) nmax doubles each; L1 (D), L3 (temp), nmax^2
+// doubles each; then reductions, rotations, DIIS matrix and index tables.
 __host__ __device__ inline size_t scf_lds_bytes(int nmax, int threads) {
     const size_t n2 = (size_t)nmax * nmax;
-    return sizeof(double) * (4 * n2 + nmax + threads + 2 * (nmax + 2) + kBatchDiis * kBatchDiis + kBatchDiis + 8 +
+    return sizeof(double) * (4 * n2 + 2 * nmax + threads + 2 * (nmax + 2) + kBatchDiis * kBatchDiis + kBatchDiis + 8 +
                              (kBatchDiis + 1) * (kBatchDiis + 2)) +
            sizeof(int) * 2 * nmax + 2 * (size_t)tri(nmax) + 16;
 }
@@ -486,7 +510,8 @@ __host__ __device__ inline size_t scf_lds_bytes(int nmax, int threads) {
 __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
     extern __shared__ __attribute__((aligned(16))) double smem[];
     const size_t n2max = (size_t)nmax * nmax;
-    double *L0 = smem, *L1 = L0 + n2max + nmax, *L2 = L1 + n2max, *L3 = L2 + n2max;   // L0: (nmax | 1) x nmax
+    // L0 and L2: (nmax | 1) x nmax (the Fock build's padded accumulator and D)
+    double *L0 = smem, *L1 = L0 + n2max + nmax, *L2 = L1 + n2max, *L3 = L2 + n2max + nmax;
     double *s_red = L3 + n2max;
     double *s_rot = s_red + blockDim.x;                  // 2 (nmax + 2)
     double *s_B = s_rot + 2 * (nmax + 2);                // kBatchDiis^2
@@ -569,7 +594,7 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
         double E = 0.0, E_old = 0.0, dE_prev = 1e300, err_prev = 1e300;
         for (it = 1; it <= a.max_iter + 1; ++it) {
             // F = H + G(D) into W; E = (D.F + D.H)/2 + E_nuc
-            fock(eri, Hg, L1, L0, W, n, s_pi, s_pj);
+            fock(eri, Hg, L1, L0, L2, W, n, s_pi, s_pj);
             double ed = 0.0;
             for (int e = threadIdx.x; e < (int)nn; e += blockDim.x) ed += L1[e] * (W[e] + Hg[e]);
             E = 0.5 * wg_sum(ed, s_red) + mol.e_nuc;
