@@ -27,6 +27,7 @@
 
 #include "context.h"
 #include "guess.h"
+#include "ket_soa.h"
 #include "one_e_math.h"
 
 namespace myqc {
@@ -88,6 +89,8 @@ struct BEriArgs {
     const double *ftT;
     const BMol *mols;
     const int *cls0, *cls1, *cls2;
+    const double *ksoa;      // primitive pairs of the ket class list (ket_soa.h)
+    long long nket;          // entries of the ket class list
     const long long *qcum;   // nmol + 1 prefix counts of this class combination
     int nmol;
     long long nq;
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(256) void k_batch_eri(BEriArgs a) {
     constexpr int LA = BPL<CB>::a, LB = BPL<CB>::b, LC = BPL<CK>::a, LD = BPL<CK>::b;
     constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(LC), nd = ncart(LD);
     QuartetAcc<LA, LB, LC, LD> acc;
-    quartet_prims<LA, LB, LC, LD>(bra, ket, a.prims, a.prims, s_ft, acc);
+    quartet_prims_f<LA, LB, LC, LD>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, mol.cls0[CK] + r}, s_ft, acc);
     const double CD[3] = {ket.AB[0], ket.AB[1], ket.AB[2]};
     double out[na * nb * nc * nd];
     quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, CD, out);
@@ -706,6 +709,8 @@ struct BatchState {
     std::vector<int> order;
     BMol *d_mols = nullptr;
     int *d_order = nullptr, *d_pair_mol = nullptr, *d_cls[3] = {nullptr, nullptr, nullptr};
+    int ncls[3] = {0, 0, 0};
+    double *d_ksoa[3] = {nullptr, nullptr, nullptr};   // per class list (ket_soa.h)
     ShellPair *d_pairs = nullptr;
     PrimPair *d_prims = nullptr;
     double *d_atoms = nullptr, *d_ftT = nullptr;
@@ -742,6 +747,7 @@ struct BatchState {
         if (graph_learned) (void)hipGraphExecDestroy(graph_learned);
         graph_learned = nullptr;
         for (auto &q : d_qcum) dfree((void **)&q);
+        for (auto &q : d_ksoa) dfree((void **)&q);
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         nmol = 0;
@@ -768,6 +774,8 @@ static void launch_batch_eri(myqc_ctx *ctx, BatchState &b, int c, int threads) {
     a.cls0 = b.d_cls[0];
     a.cls1 = b.d_cls[1];
     a.cls2 = b.d_cls[2];
+    a.ksoa = b.d_ksoa[kCK[c]];
+    a.nket = b.ncls[kCK[c]];
     a.qcum = b.d_qcum[c];
     a.nmol = b.nmol;
     a.nq = b.nq[c];
@@ -1016,6 +1024,18 @@ extern "C" int myqc_batch_set(myqc_ctx *ctx, int nmol, const int *mol_nnuc, cons
     for (int c = 0; c < 3; ++c) MYQC_CHECK(upload(ctx, &b.d_cls[c], cls[c]));
     MYQC_CHECK(upload(ctx, &b.d_pairs, pairs));
     MYQC_CHECK(upload(ctx, &b.d_prims, prims));
+    for (const ShellPair &sp : pairs)
+        if (sp.npp > kKetPrims)
+            return set_error(ctx, MYQC_E_ARG, "myqc_batch_set: a shell pair has %d primitive pairs (at most %d)", sp.npp,
+                             kKetPrims);
+    for (int c = 0; c < 3; ++c) {
+        b.ncls[c] = (int)cls[c].size();
+        if (b.ncls[c] == 0) continue;
+        MYQC_CHECK(dalloc(ctx, (void **)&b.d_ksoa[c], sizeof(double) * kKetPrims * kKetFields * (size_t)b.ncls[c]));
+        k_ket_soa<<<dim3((b.ncls[c] + 255) / 256, kKetPrims), 256, 0, ctx->stream>>>(b.d_pairs, b.d_prims, b.d_cls[c],
+                                                                                  b.ncls[c], b.d_ksoa[c]);
+        MYQC_HIP(ctx, hipGetLastError());
+    }
     MYQC_CHECK(upload(ctx, &b.d_atoms, atoms));
     MYQC_CHECK(upload(ctx, &b.d_ftT, ftT));
     const size_t M = (size_t)std::max<long long>(b.total_mat, 1);

@@ -18,6 +18,7 @@
 #include <map>
 
 #include "context.h"
+#include "ket_soa.h"
 #include "system.h"
 
 namespace myqc {
@@ -90,45 +91,6 @@ template <int C> struct PairL;
 template <> struct PairL<0> { static constexpr int a = 0, b = 0; };
 template <> struct PairL<1> { static constexpr int a = 1, b = 0; };
 template <> struct PairL<2> { static constexpr int a = 1, b = 1; };
-
-// Ket primitive pairs as structure of arrays in Schwarz-rank order, so that
-// the lanes of a wavefront (consecutive ket ranks) read each field with one
-// coalesced load: p, P[3], PA[3], K, EIJ, 1/p.  STO-3G pairs have 3 x 3
-// primitive pairs; more are refused when the arrays are built.
-constexpr int kKetFields = 10, kKetPrims = 9;
-
-__global__ void k_ket_soa(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
-                          const int *__restrict__ sorted, int np, double *__restrict__ out) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (r >= np) return;
-    const ShellPair sp = pairs[sorted[r]];
-    PrimPair q{};
-    if (j < sp.npp) q = prims[sp.pp0 + j];
-    const double f[kKetFields] = {q.p, q.P[0], q.P[1], q.P[2], q.PA[0], q.PA[1], q.PA[2], q.K, q.EIJ, q.ip};
-#pragma unroll
-    for (int k = 0; k < kKetFields; ++k) out[((size_t)j * kKetFields + k) * np + r] = f[k];
-}
-
-struct KetSoa {
-    const double *__restrict__ s;
-    long long np, r;
-    __device__ __forceinline__ PrimPair operator()(int j) const {
-        const double *b = s + (size_t)j * kKetFields * np + r;
-        PrimPair q;
-        q.p = b[0];
-        q.P[0] = b[np];
-        q.P[1] = b[2 * np];
-        q.P[2] = b[3 * np];
-        q.PA[0] = b[4 * np];
-        q.PA[1] = b[5 * np];
-        q.PA[2] = b[6 * np];
-        q.K = b[7 * np];
-        q.EIJ = b[8 * np];
-        q.beta = 0.0;
-        q.ip = b[9 * np];
-        return q;
-    }
-};
 
 struct EriArgs {
     const ShellPair *pairs;
