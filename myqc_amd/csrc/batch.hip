@@ -199,15 +199,20 @@ __device__ __forceinline__ double wg_max(double v, double *s_red) {
     return r;
 }
 
-// C = alpha A B (or A B^T); all n x n column-major; caller synchronises
+// C = alpha A B (or A B^T); all n x n column-major; caller synchronises.
+// Operands may sit in HBM: the k loop is unrolled so that their loads issue
+// together.
 __device__ __forceinline__ void mm(double *C, const double *A, const double *B, int n, bool transB, double alpha) {
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e % n, j = e / n;
         double s = 0.0;
-        if (transB)
+        if (transB) {
+#pragma unroll 8
             for (int k = 0; k < n; ++k) s += A[i + n * k] * B[j + n * k];
-        else
+        } else {
+#pragma unroll 8
             for (int k = 0; k < n; ++k) s += A[i + n * k] * B[k + n * j];
+        }
         C[e] = alpha * s;
     }
 }
