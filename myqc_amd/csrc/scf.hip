@@ -10,6 +10,8 @@
 // purification in sp2.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include <rocsolver/rocsolver.h>
@@ -353,6 +355,13 @@ static bool diis_coeffs(const std::vector<double> &B, int m, std::vector<double>
 // prints E of the density it diagonalises, scf.f90:775-848).  With
 // precision FP32, F_k = F_{k-1} + G32(D_k - D_{k-1}) between FP64 builds every
 // 8 iterations, and convergence is only accepted on two FP64 builds in a row.
+// A run started from a given density or orbitals (a guess that is already
+// near the solution but not a Roothaan solution) level-shifts the virtual
+// space, F' + sigma (I - D'), while the DIIS error is above kShiftErr: that
+// keeps DIIS from wandering off the guess in the first iterations.  From the
+// core-Hamiltonian guess the shift slows convergence, so it is not used there.
+constexpr double kShift = 0.5, kShiftErr = 1.0;
+
 static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc, const double *C_guess,
                    const double *D_guess, double e_tol, double d_tol, int max_iter, int precision, double *energy,
                    double *C_out, double *eig_out, int *iters_out) {
@@ -395,6 +404,8 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
     bool final_fp64 = false, conv = false;
     double *fslot = nullptr;
     int it = 1;
+    // per-iteration convergence data to stderr when MYQC_SCF_TRACE is set
+    const bool trace = getenv("MYQC_SCF_TRACE") != nullptr;
     while (true) {
         const bool full = !incr || final_fp64 || since_full == 0 || since_full >= 8;
         if (full) {
@@ -423,6 +434,9 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
         double r[kRedJobs];
         MYQC_CHECK(reduce(ctx, b, J, r));
         E = 0.5 * (r[0] + r[1]) + e_nuc;
+        if (trace)
+            fprintf(stderr, "[scf] it %2d %s E %.12f dE %.2e max|dD| %.2e |err| %.2e hist %d\n", it,
+                    full ? "fp64" : "fp32", E, fabs(E - E_old), jdd >= 0 ? r[jdd] : -1.0, sqrt(fabs(r[jb0 + head])), hn);
         if (jdd >= 0 && it > 2 && dE_prev < e_tol && r[jdd] < d_tol && err_prev < 100 * d_tol) {
             if (!incr || (prev64 && full)) {
                 conv = true;
@@ -458,6 +472,10 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
         E_old = E;
         prev64 = full;
         head = (head + 1) % MD;
+        if ((C_guess || D_guess) && err_prev > kShiftErr) {
+            k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(Fuse, b.Do, b.T1, 1.0, -kShift, kShift, n);
+            Fuse = b.T1;
+        }
         MYQC_CHECK(purify(ctx, b, Fuse, b.Don, n, nocc));
         std::swap(b.D, b.Dn);   // Dn keeps D_k for the change
         std::swap(b.Do, b.Don);
