@@ -111,7 +111,9 @@ int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, doub
 /* myqc_rhf_scf started from an AO density D_guess (n x n, the 2 C_occ C_occ^T
  * convention of densRHF) instead of orbitals: the entry for the learned
  * guess.  No counterpart in the reference (restarts read Cui,
- * src/scf/scf.f90:148-156). */
+ * src/scf/scf.f90:148-156).  Runs from a supplied guess (this entry, or
+ * C_guess above) level-shift the virtual levels by 0.5 Eh while the DIIS
+ * error is above 1; the converged state is unaffected. */
 int myqc_rhf_scf_density(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
                          const double *D_guess, double e_tol, double d_tol, int max_iter, int precision,
                          double *energy, double *C, double *eig, int *iterations);
