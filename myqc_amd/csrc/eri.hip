@@ -433,7 +433,7 @@ static int build_classes(myqc_ctx *ctx, double tol) {
 // kets per lane of a digestion chunk (jk.hip)
 __host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 : row == 3 ? 4 : row == 9 ? 2 : 1; }
 
-static int build_expanded(myqc_ctx *ctx, double tol) {
+static int build_segments(myqc_ctx *ctx, double tol) {
     const int ns = (int)ctx->shells.size();
     ctx->ns = ns;
     const int np = (int)ctx->pairs.size();
@@ -494,6 +494,20 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
             }
         }
     ctx->n_expanded = off;
+    const size_t nseg = (size_t)ns * ns * 3;
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_segoff, sizeof(long long) * nseg));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_rank, sizeof(int) * np));
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_segoff, ctx->segoff.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_rank, rank.data(), sizeof(int) * np, hipMemcpyHostToDevice, ctx->stream));
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// The digestion schedule over the segments of build_segments.  It is built
+// on the host while the ERI kernels that fill the segments run.
+static int build_schedule(myqc_ctx *ctx) {
+    const int ns = ctx->ns;
     // digestion schedule: per row shell X, its units Y (all three ket
     // classes) packed into workgroups of about kTarget values whose Y shells
     // span at most dig_rows AOs (the D rows a workgroup stages in LDS: with
@@ -585,34 +599,28 @@ static int build_expanded(myqc_ctx *ctx, double tol) {
     ctx->n_chunks = (int)(chunk.size() / 4);
     ctx->n_units = (int)(unit.size() / 4);
     // ket metadata in Schwarz-rank order per class
+    std::vector<int> meta[3];
     for (int c = 0; c < 3; ++c) {
         const auto &ids = ctx->pair_ids[c];
         if (ids.empty()) continue;
-        std::vector<int> meta(ids.size());
+        meta[c].resize(ids.size());
         for (size_t r = 0; r < ids.size(); ++r) {
             const ShellPair &sp = ctx->pairs[ids[r]];
-            meta[r] = sp.ao_a | (sp.ao_b << 15) | (sp.same << 30);
+            meta[c][r] = sp.ao_a | (sp.ao_b << 15) | (sp.same << 30);
         }
-        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_meta[c], sizeof(int) * meta.size()));
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_meta[c], meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice,
-                                     ctx->stream));
-        MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_meta[c], sizeof(int) * meta[c].size()));
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_meta[c], meta[c].data(), sizeof(int) * meta[c].size(),
+                                     hipMemcpyHostToDevice, ctx->stream));
     }
     std::vector<int> ao_shell(ctx->n);
     for (int X = 0; X < ns; ++X)
         for (int k = 0; k < (ctx->shells[X].L ? 3 : 1); ++k) ao_shell[ctx->shells[X].ao + k] = X;
-    const size_t nseg = (size_t)ns * ns * 3;
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_segoff, sizeof(long long) * nseg));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_rank, sizeof(int) * np));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_wg, sizeof(int) * std::max<size_t>(wg.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit, sizeof(int) * std::max<size_t>(unit.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit_off, sizeof(long long) * std::max<size_t>(unit_off.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_chunk, sizeof(int) * std::max<size_t>(chunk.size(), 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_shell, sizeof(int) * shell.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_ao_shell, sizeof(int) * ao_shell.size()));
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_segoff, ctx->segoff.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice,
-                                 ctx->stream));
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_rank, rank.data(), sizeof(int) * np, hipMemcpyHostToDevice, ctx->stream));
     if (!wg.empty())
         MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_wg, wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice, ctx->stream));
     if (!unit.empty()) {
@@ -638,14 +646,12 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     free_eri(ctx, true);
     MYQC_CHECK(build_classes(ctx, tol));
     hc.mark("eri_store: quartet classes");
-    MYQC_CHECK(build_expanded(ctx, tol));
-    hc.mark("eri_store: segments and schedule");
+    MYQC_CHECK(build_segments(ctx, tol));
+    hc.mark("eri_store: segments");
     const size_t nx = (size_t)std::max<long long>(ctx->n_expanded, 1);
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x64, sizeof(double) * nx));
     if (want32) MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x32, sizeof(float) * nx));
     else dfree((void **)&ctx->d_x32);   // a kept buffer may be smaller than this build
-    const size_t part = (size_t)std::max(ctx->n_wg, 1) * 3 * ctx->n;
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * part));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair, sizeof(double) * 9 * ctx->pairs.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair32, sizeof(float) * 9 * ctx->pairs.size()));
     hipEvent_t ev;
@@ -657,6 +663,11 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     }
     MYQC_CHECK(timer_end(ctx, T_ERI, ev, ctx->eri_flops));
     hc.mark("eri_store: allocations and launches");
+    // the digestion schedule is built while the kernels run
+    MYQC_CHECK(build_schedule(ctx));
+    const size_t part = (size_t)std::max(ctx->n_wg, 1) * 3 * ctx->n;
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * part));
+    hc.mark("eri_store: schedule (host, beside the kernels)");
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hc.mark("eri_store: kernels");
     ctx->have_eri = true;
