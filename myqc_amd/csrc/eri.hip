@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <map>
+#include <thread>
 
 #include "context.h"
 #include "ket_soa.h"
@@ -304,13 +305,23 @@ int eri_schwarz(myqc_ctx *ctx) {
         if (sp.npp > kKetPrims)
             return set_error(ctx, MYQC_E_ARG, "a shell pair has %d primitive pairs (at most %d: STO-3G shells)",
                              sp.npp, kKetPrims);
-    // sort each class by Schwarz factor, largest first (ties by pair id)
+    // sort each class by Schwarz factor, largest first (ties by pair id); one
+    // host thread per class
+    {
+        auto sort_class = [ctx](int c) {
+            auto &ids = ctx->pair_ids[c];
+            std::vector<std::pair<double, int>> key(ids.size());
+            for (size_t r = 0; r < ids.size(); ++r) key[r] = {-ctx->schwarz[ids[r]], ids[r]};
+            std::sort(key.begin(), key.end());   // descending factor, ties by pair id
+            for (size_t r = 0; r < ids.size(); ++r) ids[r] = key[r].second;
+        };
+        std::thread t1(sort_class, 1), t2(sort_class, 2);
+        sort_class(0);
+        t1.join();
+        t2.join();
+    }
     for (int c = 0; c < 3; ++c) {
         auto &ids = ctx->pair_ids[c];
-        std::vector<std::pair<double, int>> key(ids.size());
-        for (size_t r = 0; r < ids.size(); ++r) key[r] = {-ctx->schwarz[ids[r]], ids[r]};
-        std::sort(key.begin(), key.end());   // descending factor, ties by pair id
-        for (size_t r = 0; r < ids.size(); ++r) ids[r] = key[r].second;
         if (!ids.empty()) {
             MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_sorted[c], ids.data(), sizeof(int) * ids.size(),
                                          hipMemcpyHostToDevice, ctx->stream));
