@@ -251,3 +251,60 @@ extern "C" int hc_fock_G_pairs(int nnuc, const double *xyz, int nset, const doub
         for (size_t j = 0; j < n; ++j) G[i + n * j] = A[i + n * j] + A[j + n * i];
     return 0;
 }
+
+// The three-copy row-owned digestion, restated on the host.  Each unique
+// quartet {R, S} (R the bra: higher class, or same class and rank(R) <=
+// rank(S)) is held by the bra pair in both orientations and by the ket pair
+// in its stored orientation only.  A copy whose row pair is R and partner S
+// adds, for the rows of its first shell, with G = A + A^T at the end:
+//   bra role (R bra, or R == S): J x (forward ? (R.same ? 1/2 : 1) : 0),
+//                                K x (R == S ? 1/2 : 1)
+//   ket role (R ket):            J x (R.same ? 1/2 : 1), no K
+// with J, K the row formula of hc_fock_G_rows (raw integrals).
+extern "C" int hc_fock_G_3copy(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                               const double *bas, int norb, const int *basinfo, const double *Ftab, const double *D,
+                               double *G) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    const size_t n = norb;
+    std::vector<double> XX(n * n * n * n);
+    if (hc_eri_full(nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab, XX.data()) != 0) return -1;
+    std::vector<double> Ds(n * n), A(n * n, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) Ds[i + n * j] = 0.5 * (D[i + n * j] + D[j + n * i]);
+    auto X = [&](size_t i, size_t j, size_t k, size_t l) { return XX[i + n * (j + n * (k + n * l))]; };
+    std::vector<int> rank(s.pairs.size());
+    for (int c = 0; c < 3; ++c)
+        for (size_t r = 0; r < s.ids[c].size(); ++r) rank[s.ids[c][r]] = (int)r;
+    for (int R = 0; R < (int)s.pairs.size(); ++R) {
+        const ShellPair &p = s.pairs[R];
+        const int cr = pclass(p);
+        for (int cs = 0; cs < 3; ++cs)
+            for (int S : s.ids[cs]) {
+                const ShellPair &q = s.pairs[S];
+                const bool bra = cr > cs || (cr == cs && rank[R] <= rank[S]);
+                for (int o = 0; o < 2; ++o) {
+                    if (o == 1 && (p.same || !bra)) continue;   // reverse copies: bra role, distinct shells
+                    const double wj = !bra ? (p.same ? 0.5 : 1.0) : (o == 0 ? (p.same ? 0.5 : 1.0) : 0.0);
+                    const double wk = !bra ? 0.0 : (R == S ? 0.5 : 1.0);
+                    const int sx = o == 0 ? p.ao_a : p.ao_b, sy = o == 0 ? p.ao_b : p.ao_a;
+                    const int nx = (o == 0 ? p.la : p.lb) ? 3 : 1, ny = (o == 0 ? p.lb : p.la) ? 3 : 1;
+                    const int nc = q.la ? 3 : 1, nd = q.lb ? 3 : 1;
+                    const double fj = q.same ? 1.0 : 2.0;
+                    for (int a = 0; a < nx; ++a)
+                        for (int b = 0; b < ny; ++b)
+                            for (int c = 0; c < nc; ++c)
+                                for (int d = 0; d < nd; ++d) {
+                                    const size_t i = sx + a, j = sy + b, k = q.ao_a + c, l = q.ao_b + d;
+                                    const double v = X(i, j, k, l);
+                                    A[i + n * j] += wj * fj * Ds[k + n * l] * v;
+                                    A[i + n * k] -= wk * 0.5 * Ds[j + n * l] * v;
+                                    if (!q.same) A[i + n * l] -= wk * 0.5 * Ds[j + n * k] * v;
+                                }
+                }
+            }
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) G[i + n * j] = A[i + n * j] + A[j + n * i];
+    return 0;
+}

@@ -126,3 +126,22 @@ def test_pair_owned_digestion_formula(hc, name):
         G = np.zeros(n * n)
         assert hc.hc_fock_G_pairs(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
         assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
+
+@pytest.mark.parametrize("name", ["co2", "h2o_2", "mol3"])
+def test_three_copy_digestion_formula(hc, name):
+    """The three-copy row-owned split the digestion kernel uses (the bra
+    pair's copies in both orientations, the ket pair's in its stored one;
+    J and K weights by role; G = A + A^T) reproduces RHFI2G's G for the
+    reference's density and for a nonsymmetric one."""
+    f, Z, s = _sys(name)
+    n = s.n
+    rng = np.random.default_rng(12)
+    for D, ref in ((f["D_guess"], f["G_guess"]), (None, None)):
+        if D is None:
+            D = rng.standard_normal((n, n))
+            ref = O.fock_G(O.eri_full(s), D)
+        Df = np.asfortranarray(D).ravel(order="F").copy()
+        G = np.zeros(n * n)
+        assert hc.hc_fock_G_3copy(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
+        assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
+
