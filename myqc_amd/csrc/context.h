@@ -84,6 +84,9 @@ struct myqc_ctx {
     int ns = 0;                           // number of shells
     std::vector<long long> segoff;        // ns*ns*3 segment offsets
     std::vector<int> segcnt;              // ns*ns*3 kets per segment
+    std::vector<int> segstart;            // ns*ns*3 rank of a segment's first ket (reverse segments)
+    std::vector<int> pair_of;             // ns*ns: the shell pair of (X, Y) in either orientation
+    std::vector<int> pair_rank, pair_cls; // rank within its class's sorted list, class
     long long *d_segoff = nullptr;
     int *d_rank = nullptr;                // rank of each pair within its class's sorted list
     double *d_x64 = nullptr;              // expanded values

@@ -174,22 +174,22 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     double out[BLK];
     quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, CD, out);
     if (MODE == 0) {
-        // copies for the row owners A (bra (A,B)), B (bra (B,A)), C and D
+        // copies for the row owners A (bra (A,B)), B (bra (B,A)) and C (ket (C,D))
         const int A = bra.sa, B = bra.sb, C = ket.sa, D = ket.sb, ns = a.ns;
         const long long rq = a.rank[pk], rp = a.rank[pb];
         auto at = [&](int ia, int ib, int ic, int id) { return out[((ia * nb + ib) * nc + ic) * nd + id]; };
         put_copy<na, nb, nc, nd>(a.x64, a.x32, a.segoff[((long long)A * ns + B) * 3 + CK] + rq * BLK,
                                  [&](int x, int y, int u, int w) { return at(x, y, u, w); });
+        // the bra's reverse segment holds only the kets of which the bra pair
+        // is the bra: in the same class those from its own rank on
         if (!bra.same)
-            put_copy<nb, na, nc, nd>(a.x64, a.x32, a.segoff[((long long)B * ns + A) * 3 + CK] + rq * BLK,
+            put_copy<nb, na, nc, nd>(a.x64, a.x32,
+                                     a.segoff[((long long)B * ns + A) * 3 + CK] + (rq - (CB == CK ? rp : 0)) * BLK,
                                      [&](int x, int y, int u, int w) { return at(y, x, u, w); });
-        if (pb != pk) {
+        // the ket pair's copy, in its stored orientation only
+        if (pb != pk)
             put_copy<nc, nd, na, nb>(a.x64, a.x32, a.segoff[((long long)C * ns + D) * 3 + CB] + rp * BLK,
                                      [&](int x, int y, int u, int w) { return at(u, w, x, y); });
-            if (!ket.same)
-                put_copy<nd, nc, na, nb>(a.x64, a.x32, a.segoff[((long long)D * ns + C) * 3 + CB] + rp * BLK,
-                                         [&](int x, int y, int u, int w) { return at(u, w, y, x); });
-        }
         return;
     }
     const int same_pair = (pb == pk);
@@ -489,28 +489,48 @@ static int build_segments(myqc_ctx *ctx, double tol) {
             }
         }
     }
+    // a pair's forward segment (its stored orientation) holds every screened
+    // partner; its reverse segment only the partners of which it is the bra
+    // (lower classes, and in its own class those from its own rank on)
+    std::vector<int> pcls(np, 0);
+    for (int c = 0; c < 3; ++c)
+        for (int id : ctx->pair_ids[c]) pcls[id] = c;
     ctx->segoff.assign((size_t)ns * ns * 3, 0);
     ctx->segcnt.assign((size_t)ns * ns * 3, 0);
+    ctx->segstart.assign((size_t)ns * ns * 3, 0);
     long long off = 0;
     for (int X = 0; X < ns; ++X)
         for (int Y = 0; Y < ns; ++Y) {
             const int p = pair_of[(size_t)X * ns + Y];
             const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
+            const bool fwd = p >= 0 && ctx->pairs[p].sa == X;
             for (int ck = 0; ck < 3; ++ck) {
-                const long long cnt = pcnt[(size_t)p * 3 + ck];
+                long long cnt = p >= 0 ? pcnt[(size_t)p * 3 + ck] : 0, start = 0;
+                if (p >= 0 && !fwd) {
+                    if (ck > pcls[p]) cnt = 0;
+                    else if (ck == pcls[p]) {
+                        start = rank[p];
+                        cnt = std::max(0LL, cnt - start);
+                    }
+                }
                 const size_t seg = ((size_t)X * ns + Y) * 3 + ck;
                 ctx->segoff[seg] = off;
                 ctx->segcnt[seg] = (int)cnt;
+                ctx->segstart[seg] = (int)start;
                 off += cnt * nxy * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
             }
         }
     ctx->n_expanded = off;
+    ctx->pair_of.swap(pair_of);
+    ctx->pair_rank = rank;
+    ctx->pair_cls.swap(pcls);
     const size_t nseg = (size_t)ns * ns * 3;
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_segoff, sizeof(long long) * nseg));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_rank, sizeof(int) * np));
     MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_segoff, ctx->segoff.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice,
                                  ctx->stream));
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_rank, rank.data(), sizeof(int) * np, hipMemcpyHostToDevice, ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_rank, ctx->pair_rank.data(), sizeof(int) * np, hipMemcpyHostToDevice,
+                                 ctx->stream));
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -534,7 +554,7 @@ static int build_schedule(myqc_ctx *ctx) {
     {
         long long nchunk = 0, nunit = 0;
         for (size_t seg = 0; seg < ctx->segcnt.size(); ++seg) {
-            nchunk += (ctx->segcnt[seg] + 63) / 64;
+            nchunk += (ctx->segcnt[seg] + 63) / 64 + 2;
             nunit += (seg % 3 == 0) && (ctx->segcnt[seg] + ctx->segcnt[seg + 1] + ctx->segcnt[seg + 2] > 0);
         }
         chunk.reserve(4 * (size_t)nchunk);
@@ -571,24 +591,46 @@ static int build_schedule(myqc_ctx *ctx) {
             if (row0 >= 0 && aoY + nY - row0 > ctx->dig_rows) close();
             unit.push_back(Y);
             for (int ck = 0; ck < 3; ++ck) unit.push_back(cnt[ck]);
+            const int p = ctx->pair_of[(size_t)X * ns + Y];
+            const bool fwd = ctx->pairs[p].sa == X;
+            const int cp = ctx->pair_cls[p], rp = ctx->pair_rank[p];
             for (int ck = 0; ck < 3; ++ck) {
-                const long long off = ctx->segoff[seg + ck];
-                unit_off.push_back(off);
+                const int start = ctx->segstart[seg + ck], end = start + cnt[ck];
                 const int row = nY * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
+                // offset such that the block of the ket with rank r sits at off + r * block
+                const long long off = ctx->segoff[seg + ck] - (long long)start * nX * row;
+                unit_off.push_back(off);
                 const int step = 64 * kets_per_lane(row);
-                for (int r0 = 0; r0 < cnt[ck]; r0 += step) {
-                    // a workgroup closes after about kTarget values, also
-                    // inside a unit: chunks are independent
-                    if (row0 < 0) row0 = aoY;
-                    row1 = aoY + nY;
-                    const int nk = std::min(step, cnt[ck] - r0);
-                    chunk.push_back((int)(unsigned)(off & 0xffffffffLL));
-                    chunk.push_back((int)(off >> 32));
-                    chunk.push_back(aoY | (ctx->shells[Y].L << 15) | (ck << 16) | (nk << 18));
-                    chunk.push_back(r0);
-                    acc += (long long)nX * row * nk;
-                    if (acc >= kTarget) close();
+                // role ranges of the segment's kets: [r0, r1) with flags
+                // 1 = bra role, 2 = forward orientation, 4 = the diagonal quartet
+                int cut[4] = {start, end, end, end}, flag[3] = {0, 0, 0};
+                const int fo = fwd ? 2 : 0;
+                if (ck < cp) flag[0] = 1 | fo;
+                else if (ck > cp) flag[0] = 0 | fo;
+                else {
+                    // same class: kets below the pair's rank are bras of it,
+                    // the pair itself is the diagonal quartet, the rest its kets
+                    const int d0 = std::min(std::max(rp, start), end);
+                    cut[1] = d0;
+                    cut[2] = std::min(d0 + (rp >= start && rp < end ? 1 : 0), end);
+                    flag[0] = 0 | fo;
+                    flag[1] = 1 | 4 | fo;
+                    flag[2] = 1 | fo;
                 }
+                for (int g = 0; g < 3; ++g)
+                    for (int r0 = cut[g]; r0 < cut[g + 1]; r0 += step) {
+                        // a workgroup closes after about kTarget values, also
+                        // inside a unit: chunks are independent
+                        if (row0 < 0) row0 = aoY;
+                        row1 = aoY + nY;
+                        const int nk = std::min(step, cut[g + 1] - r0);
+                        chunk.push_back((int)(unsigned)(off & 0xffffffffLL));
+                        chunk.push_back((int)(off >> 32));
+                        chunk.push_back(aoY | (ctx->shells[Y].L << 15) | (ck << 16) | (nk << 18) | (flag[g] << 28));
+                        chunk.push_back(r0);
+                        acc += (long long)nX * row * nk;
+                        if (acc >= kTarget) close();
+                    }
             }
         }
         close();

@@ -3,24 +3,30 @@
 // The reference's two-electron part of the Fock matrix is
 //     G(i,j) = sum_kl D(k,l)[XX(i,j,k,l) - XX(i,k,j,l)/4 - XX(i,l,j,k)/4]
 // (src/I2G/RHFI2G.f90:80-90), which only sees D through D + D^T, so it is
-// evaluated for the symmetrised D.  Each unique shell quartet (AB|CD) is
-// stored once per ordered bra shell pair (A,B) that owns rows of G -- up to
-// four copies -- in segments of kets sorted by Schwarz rank.  Row ownership
-// turns every update into a row-local one:
-//     J: G(i,j) += f D(k,l) (ij|kl),              f = 2 unless C == D
-//     K: G(i,k) -= D(j,l) (ij|kl) / 2,  and if C != D  G(i,l) -= D(j,k) (ij|kl) / 2
-// for i in A, j in B.  A workgroup owns the rows of one shell A for a run of
-// about 2^15 stored values: it stages in LDS the rows of D of the B shells it
-// meets (at most dig_rows AOs), then each wavefront takes chunks of kets of
-// one (A, B, ket class) segment in turn, with no barrier between chunks.  A
-// lane takes 1-8 kets of a chunk (more for the small classes) and issues all
-// their loads of one bra row before their arithmetic.  J is reduced across
-// the wavefront and K is added per ket, both into FP64 LDS rows of A (FP32
+// evaluated for the symmetrised D.  Each unique shell quartet (AB|CD), with
+// (A,B) its bra pair (the higher class, or the lower Schwarz rank in one
+// class), is stored three times, in segments of kets sorted by Schwarz rank:
+// under the ordered pairs (A,B) and (B,A), and under (C,D) in the ket pair's
+// stored orientation.  A copy under the ordered pair (X,Y) with ket (U,W)
+// updates only rows of shell X:
+//     J: A(i,j) += wJ f D(k,l) (ij|kl),           f = 2 unless U == W
+//     K: A(i,k) -= wK D(j,l) (ij|kl) / 2,  and if U != W  A(i,l) -= wK D(j,k) (ij|kl) / 2
+// for i in X, j in Y, and G = A + A^T.  The weights split the quartet's
+// terms so that each lands once: the bra's forward copy adds J and its K rows,
+// the bra's reverse copy its K rows only, the ket's copy J only (wK = 0);
+// wJ halves for a same-shell row pair, wK for the diagonal quartet (P|P).
+// (tests/hostcheck states this split on the host and checks it against
+// RHFI2G.)  A workgroup owns the rows of one shell A for a run of about 2^15
+// stored values: it stages in LDS the rows of D of the B shells it meets (at
+// most dig_rows AOs), then each wavefront takes chunks of kets of one (A, B,
+// ket class, role) range in turn, with no barrier between chunks.  A lane
+// takes 1-8 kets of a chunk (more for the small classes) and issues all their
+// loads of one bra row before their arithmetic.  J is reduced across the
+// wavefront and K is added per ket, both into FP64 LDS rows of A (FP32
 // values are accumulated in FP64 too); the rows go to the workgroup's own
 // partial slot.  Workgroups launch longest first.  A gather sums the partial
-// rows of each shell in a fixed order and G = (A + A^T)/2 restores symmetry:
-// no global atomics; the LDS atomics make the summation order, and so the
-// last bits, vary from run to run.
+// rows of each shell in a fixed order: no global atomics; the LDS atomics
+// make the summation order, and so the last bits, vary from run to run.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -73,7 +79,7 @@ __host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 :
 // kets issued before their arithmetic.
 template <int LA, int LB, int CK, typename TV>
 __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int r0, int nk,
-                                             long long off, int aoB) {
+                                             long long off, int aoB, double wJ, double wK) {
     constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
     constexpr int ROW = nb * nc * nd, BLK = na * ROW, KPL = kets_per_lane(ROW);
     const int n = a.n;
@@ -128,7 +134,7 @@ __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_row
                     for (int d = 0; d < nd; ++d) js += dq[j][c * nd + d] * x[j][(ib * nc + c) * nd + d];
                 jr[ib] += fj * js;
             }
-            if (!act[j]) continue;
+            if (!act[j] || wK == 0.0) continue;   // (wK is uniform across the wavefront)
             // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
 #pragma unroll
             for (int c = 0; c < nc; ++c) {
@@ -137,7 +143,7 @@ __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_row
                 for (int ib = 0; ib < nb; ++ib)
 #pragma unroll
                     for (int d = 0; d < nd; ++d) ks += dbd[j][ib * nd + d] * x[j][(ib * nc + c) * nd + d];
-                atomicAdd(&s_rows[ia * n + aoC[j] + c], -0.5 * (double)ks);
+                atomicAdd(&s_rows[ia * n + aoC[j] + c], -0.5 * wK * (double)ks);
             }
             // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
             if (!same[j]) {
@@ -148,29 +154,29 @@ __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_row
                     for (int ib = 0; ib < nb; ++ib)
 #pragma unroll
                         for (int c = 0; c < nc; ++c) ks += dbc[j][ib * nc + c] * x[j][(ib * nc + c) * nd + d];
-                    atomicAdd(&s_rows[ia * n + aoD[j] + d], -0.5 * (double)ks);
+                    atomicAdd(&s_rows[ia * n + aoD[j] + d], -0.5 * wK * (double)ks);
                 }
             }
         }
 #pragma unroll
         for (int ib = 0; ib < nb; ++ib) {
             const TV t = wave_sum(jr[ib]);
-            if (lane == 0) atomicAdd(&s_rows[ia * n + aoB + ib], (double)t);
+            if (lane == 0 && wJ != 0.0) atomicAdd(&s_rows[ia * n + aoB + ib], wJ * (double)t);
         }
     }
 }
 
 template <int LA, typename TV>
 __device__ __forceinline__ void digest_chunk_la(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int LB, int ck,
-                                                int r0, int nk, long long off, int aoB) {
+                                                int r0, int nk, long long off, int aoB, double wJ, double wK) {
     if (LB == 0) {
-        if (ck == 0) digest_chunk<LA, 0, 0>(a, s_rows, s_drow, r0, nk, off, aoB);
-        else if (ck == 1) digest_chunk<LA, 0, 1>(a, s_rows, s_drow, r0, nk, off, aoB);
-        else digest_chunk<LA, 0, 2>(a, s_rows, s_drow, r0, nk, off, aoB);
+        if (ck == 0) digest_chunk<LA, 0, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
+        else if (ck == 1) digest_chunk<LA, 0, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
+        else digest_chunk<LA, 0, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
     } else {
-        if (ck == 0) digest_chunk<LA, 1, 0>(a, s_rows, s_drow, r0, nk, off, aoB);
-        else if (ck == 1) digest_chunk<LA, 1, 1>(a, s_rows, s_drow, r0, nk, off, aoB);
-        else digest_chunk<LA, 1, 2>(a, s_rows, s_drow, r0, nk, off, aoB);
+        if (ck == 0) digest_chunk<LA, 1, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
+        else if (ck == 1) digest_chunk<LA, 1, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
+        else digest_chunk<LA, 1, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
     }
 }
 
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
     const int w = blockIdx.x;
     const int A = a.wg[6 * w], ch0 = a.wg[6 * w + 1], ch1 = a.wg[6 * w + 2];
     const int row0 = a.wg[6 * w + 3], nrows = a.wg[6 * w + 4], slot = a.wg[6 * w + 5];
-    const int LA = a.shell[4 * A + 1];
+    const int LA = a.shell[4 * A + 1], aoA = a.shell[4 * A];
     const int na = LA ? 3 : 1;
     for (int i = threadIdx.x; i < na * n; i += blockDim.x) s_rows[i] = 0;
     // D(row0 + b, j) = D(j, row0 + b): read columns, coalesced
@@ -203,10 +209,17 @@ __global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
         const int cn = c + 4;
         const int4 dn = cn < ch1 ? a.chunk[cn] : make_int4(0, 0, 0, 0);
         const long long off = (long long)(unsigned)d.x | ((long long)d.y << 32);
-        const int aoB = d.z & 0x7fff, LB = (d.z >> 15) & 1, ck = (d.z >> 16) & 3, nk = d.z >> 18;
+        const int aoB = d.z & 0x7fff, LB = (d.z >> 15) & 1, ck = (d.z >> 16) & 3, nk = (d.z >> 18) & 0x3ff;
+        const int role = (d.z >> 28) & 7;   // 1: bra role, 2: forward orientation, 4: diagonal quartet
+        // weights of the three-copy split (G = A + A^T): the bra pair's two
+        // orientations share its J and each adds its K rows, the ket pair's
+        // copy adds J only; halves for a same-shell row pair and the diagonal
+        const double half_same = (aoB == aoA) ? 0.5 : 1.0;
+        const double wJ = (role & 1) ? ((role & 2) ? half_same : 0.0) : half_same;
+        const double wK = (role & 1) ? ((role & 4) ? 0.5 : 1.0) : 0.0;
         const TV *drow = s_drow + (size_t)(aoB - row0) * n;
-        if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB);
-        else digest_chunk_la<1>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB);
+        if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK);
+        else digest_chunk_la<1>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK);
         c = cn;
         d = dn;
     }
@@ -252,10 +265,12 @@ __global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y
         Z[i] = X[i] + Y[i];
 }
 
-__global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n) {
+// Ds = scale (D + D^T)
+__global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n,
+                             double scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= n) return;
-    const double v = 0.5 * (D[i + (size_t)n * j] + D[j + (size_t)n * i]);
+    const double v = scale * (D[i + (size_t)n * j] + D[j + (size_t)n * i]);
     if (Ds) Ds[i + (size_t)n * j] = v;
     if (Ds32) Ds32[i + (size_t)n * j] = (float)v;
 }
@@ -331,14 +346,14 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     if ((size_t)(3 + ctx->dig_rows) * n * sizeof(double) > 64 * 1024)
         return set_error(ctx, MYQC_E_ARG, "digestion rows for n=%d exceed 64 KB of LDS", n);
     if (precision == MYQC_FP64) {
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, ctx->d_D, nullptr, n);
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, ctx->d_D, nullptr, n, 0.5);
         MYQC_CHECK(launch_digest<double>(ctx, ctx->d_D, ctx->d_x64, ctx->d_Dpair, ctx->d_partial, ctx->d_A));
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, nullptr, n);
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, nullptr, n, 1.0);
     } else {
         if (!ctx->have_eri32) return set_error(ctx, MYQC_E_STATE, "no FP32 integrals stored");
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n);
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n, 0.5);
         MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial, ctx->d_A));
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n);
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n, 1.0);
         k_add<<<256, 256, 0, ctx->stream>>>(d_Gout, ctx->d_G, d_Gout, nn);
     }
     MYQC_HIP(ctx, hipGetLastError());
