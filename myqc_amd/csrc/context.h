@@ -70,8 +70,9 @@ struct myqc_ctx {
     double *d_schwarz = nullptr;
     bool have_schwarz = false;
 
-    // stored integrals: each unique shell quartet block is kept in up to four
-    // copies, one per ordered bra shell pair (A,B) that owns rows of G, in
+    // stored integrals: each unique shell quartet block is kept in up to three
+    // copies (the bra pair in both orientations, the ket pair in its stored
+    // one), each under an ordered shell pair (A,B) that owns rows of G, in
     // segments [(A*ns + B)*3 + ket class] of kets sorted by Schwarz factor
     bool have_eri = false;
     bool have_eri32 = false;
