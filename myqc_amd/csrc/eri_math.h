@@ -139,6 +139,9 @@ struct PrimPair {
     double EIJ;      // exp(-alpha beta |AB|^2 / p): the reference's pair screen factor
     double beta;     // exponent of the second shell's primitive (kinetic energy)
     double ip;       // 1 / p
+    // default construction leaves the fields unset: arrays of primitive pairs
+    // are sized before they are filled
+    MYQC_HD PrimPair() {}
 };
 
 // Contracted shell pair.

@@ -18,9 +18,13 @@ static __global__ void k_ket_soa(const ShellPair *__restrict__ pairs, const Prim
     const int r = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (r >= np) return;
     const ShellPair sp = pairs[ids[r]];
-    PrimPair q{};
-    if (j < sp.npp) q = prims[sp.pp0 + j];
-    const double f[kKetFields] = {q.p, q.P[0], q.P[1], q.P[2], q.PA[0], q.PA[1], q.PA[2], q.K, q.EIJ, q.ip};
+    double f[kKetFields] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j < sp.npp) {
+        const PrimPair q = prims[sp.pp0 + j];
+        const double g[kKetFields] = {q.p, q.P[0], q.P[1], q.P[2], q.PA[0], q.PA[1], q.PA[2], q.K, q.EIJ, q.ip};
+#pragma unroll
+        for (int k = 0; k < kKetFields; ++k) f[k] = g[k];
+    }
 #pragma unroll
     for (int k = 0; k < kKetFields; ++k) out[((size_t)j * kKetFields + k) * np + r] = f[k];
 }

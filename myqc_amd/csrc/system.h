@@ -98,7 +98,18 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
     for (int c = 0; c < 3; ++c) pair_ids[c].clear();
     const int ns = (int)shells.size();
     // pairs first (cheap), then their primitive pairs in parallel: each pair
-    // owns the slice [pp0, pp0 + npp) of prims
+    // owns the slice [pp0, pp0 + npp) of prims, which is sized once
+    {
+        size_t npairs = 0, nprims = 0;
+        for (int A = 0; A < ns; ++A)
+            for (int B = A; B < ns; ++B) {
+                ++npairs;
+                nprims += shells[A].alpha.size() * shells[B].alpha.size();
+            }
+        pairs.reserve(npairs);
+        prims.resize(nprims);
+    }
+    size_t pnext = 0;
     for (int A = 0; A < ns; ++A)
         for (int B = A; B < ns; ++B) {
             int f = A, g = B;
@@ -107,7 +118,7 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
             ShellPair sp;
             const double *RA = xyz + 3 * sa.center, *RB = xyz + 3 * sb.center;
             for (int d = 0; d < 3; ++d) sp.AB[d] = RA[d] - RB[d];
-            sp.pp0 = (int)prims.size();
+            sp.pp0 = (int)pnext;
             sp.ao_a = sa.ao;
             sp.ao_b = sb.ao;
             sp.la = sa.L;
@@ -117,7 +128,7 @@ inline std::string build_system(int nnuc, const double *xyz /*[atom][3]*/, int n
             sp.sa = f;
             sp.sb = g;
             sp.npp = (int)(sa.alpha.size() * sb.alpha.size());
-            prims.resize(prims.size() + sp.npp);
+            pnext += sp.npp;
             pair_ids[sa.L + sb.L].push_back((int)pairs.size());
             pairs.push_back(sp);
         }
