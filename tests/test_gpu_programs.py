@@ -152,3 +152,61 @@ def test_program_error_sentinel(tmp_path):
     assert r.returncode != 0 and os.path.exists(os.path.join(d, "error"))
     r = subprocess.run([os.path.join(BIN, "int1e")], cwd=d, capture_output=True, text=True, timeout=60)
     assert not os.path.exists(os.path.join(d, "Suv"))
+
+
+def _write_record(path, a):
+    """One Fortran sequential unformatted record (under 2 GiB)."""
+    b = np.asfortranarray(a, dtype="<f8").ravel(order="F").tobytes()
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", len(b)) + b + struct.pack("<i", len(b)))
+
+
+def test_int2e_record_above_2gib(ctx, tmp_path):
+    """XX of a system with n >= 128 (19 waters, n = 133: 2.5 GB), which
+    int2e.f90:166,306 writes as one record.  This image's Fortran runtime
+    (flang, the compiler the reference is built with here) writes a record
+    above 2 GiB whole, with 4-byte markers holding its length modulo 2^32 (a
+    gfortran build would split it into subrecords instead).  The file reads back
+    to the C ABI's dense integrals, and when the reference's own RHFI2G has
+    been built from its sources (oracle/_ref), that program reads our XX
+    (src/I2G/RHFI2G.f90:72-74) and returns the G that myqc_fock_G gives."""
+    from myqc_amd import molecules as M
+    from myqc_amd.basis import build_basis
+    Z, x = M.water_cluster(19, seed=19)
+    xyz = M.to_bohr(Z, x)
+    n = build_basis(Z).norb
+    assert n >= 128
+    d = str(tmp_path)
+    _write_inputs(d, Z, xyz)
+    _run(d, "int2e")
+    path = os.path.join(d, "XX")
+    size = os.path.getsize(path)
+    nbytes = 8 * n ** 4
+    with open(path, "rb") as f:
+        (head,) = struct.unpack("<I", f.read(4))
+        f.seek(-4, 2)
+        (tail,) = struct.unpack("<I", f.read(4))
+    print(f"[XX] n={n} file {size} bytes, markers {head} {tail}", flush=True)
+    assert size == nbytes + 8 and head == tail == nbytes % 2 ** 32
+    XX = np.memmap(path, dtype="<f8", mode="r", offset=4, shape=(n ** 4,))
+    ctx.set_system(Z, xyz)
+    ref = ctx.eri_full(0.0).ravel(order="F")
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, n ** 4, size=200000)
+    assert np.array_equal(XX[idx], ref[idx])
+    del XX, ref
+    rhfi2g = os.path.join(ROOT, "oracle", "_ref", "RHFI2G")
+    if not os.path.exists(rhfi2g):
+        pytest.skip("reference RHFI2G not built here (oracle/build_ref.sh)")
+    D = rng.standard_normal((n, n))
+    D = 0.5 * (D + D.T)
+    _write_record(os.path.join(d, "Da"), D)
+    print(f"[program] reference RHFI2G in {d}", flush=True)
+    r = subprocess.run([rhfi2g], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and not os.path.exists(os.path.join(d, "error")), r.stdout + r.stderr
+    G_ref = _records(os.path.join(d, "Guv"))[0].reshape(n, n, order="F")
+    ctx.eri_build(0.0)
+    G = ctx.fock_G(D)
+    err = np.max(np.abs(G - G_ref))
+    print(f"[XX] reference RHFI2G on our XX: max |G - G_ref| = {err:.2e} (max |G| {np.max(np.abs(G_ref)):.2e})")
+    assert err <= 1e-9 * max(1.0, np.max(np.abs(G_ref)))
