@@ -73,6 +73,12 @@ int myqc_eri_build(myqc_ctx *ctx, double schwarz_tol, long long *n_stored);
  * precision MYQC_FP64 or MYQC_FP32. */
 int myqc_fock_G(myqc_ctx *ctx, const double *D, double *G, int precision);
 
+/* UHFI2G (src/I2G/UHFI2G.f90:80-93): the two spin Fock contributions
+ *   Ga(i,j) = sum_kl (Da + Db)(k,l) XX(i,j,k,l) - Da(k,l) XX(i,k,j,l), Gb likewise
+ * from the integrals of myqc_eri_build (FP64; n x n column-major, D as the
+ * spin densities C_occ C_occ^T of each spin, symmetrised). */
+int myqc_fock_G_uhf(myqc_ctx *ctx, const double *Da, const double *Db, double *Ga, double *Gb);
+
 /* densRHF drop-in: D(u,v) = 2 sum_{i<nocc} C(u,i) C(v,i)
  * (src/dens/dens.f90:115-124). */
 int myqc_density_rhf(myqc_ctx *ctx, const double *C, int nocc, double *D);

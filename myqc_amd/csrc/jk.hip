@@ -55,6 +55,7 @@ struct DigArgs {
     const TV *D;                              // symmetric density (or change), n x n
     double *partial;                          // [w][3][n]
     int n;
+    double jscale;                            // 1, or 0 for the exchange part alone
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
         // orientations share its J and each adds its K rows, the ket pair's
         // copy adds J only; halves for a same-shell row pair and the diagonal
         const double half_same = (aoB == aoA) ? 0.5 : 1.0;
-        const double wJ = (role & 1) ? ((role & 2) ? half_same : 0.0) : half_same;
+        const double wJ = a.jscale * ((role & 1) ? ((role & 2) ? half_same : 0.0) : half_same);
         const double wK = (role & 1) ? ((role & 4) ? 0.5 : 1.0) : 0.0;
         const TV *drow = s_drow + (size_t)(aoB - row0) * n;
         if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK);
@@ -294,7 +295,8 @@ int ensure_work(myqc_ctx *ctx) {
 }
 
 template <typename TV>
-static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double *partial, double *Graw) {
+static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double *partial, double *Graw,
+                         double jscale = 1.0) {
     const int np0 = (int)ctx->pair_ids[0].size(), np1 = (int)ctx->pair_ids[1].size(),
               np2 = (int)ctx->pair_ids[2].size();
     TV *dk0 = dk, *dk1 = dk + np0, *dk2 = dk + np0 + 3 * (size_t)np1;
@@ -322,6 +324,7 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
         a.D = D;
         a.partial = partial;
         a.n = ctx->n;
+        a.jscale = jscale;
         const size_t shm = (3 * sizeof(double) + ctx->dig_rows * sizeof(TV)) * (size_t)ctx->n;
         // timed alone: algorithmic bytes per pass are one read of every
         // stored unique integral plus the density in and G out (16 n^2)
@@ -360,9 +363,55 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     return 0;
 }
 
+// Z = a X + b Y
+__global__ void k_axpby(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, double a,
+                        double b, size_t cnt) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+        Z[i] = a * X[i] + b * Y[i];
+}
+
+// UHF: with G(D) = J[D] - K[D]/2 and the exchange part alone Gk(D) = -K[D]/2,
+// G_a = J[Da + Db] - K[Da] = G(Da + Db) + Gk(Da - Db), G_b = G(Da + Db) - Gk(Da - Db)
+int digest_G_uhf(myqc_ctx *ctx, const double *d_Da, const double *d_Db, double *d_Ga, double *d_Gb) {
+    if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "no stored integrals (call myqc_eri_build)");
+    MYQC_CHECK(ensure_work(ctx));
+    const int n = ctx->n;
+    const size_t nn = (size_t)n * n;
+    // total density into Ga, spin difference into Gb (both overwritten below)
+    k_axpby<<<256, 256, 0, ctx->stream>>>(d_Da, d_Db, d_Ga, 1.0, 1.0, nn);
+    k_axpby<<<256, 256, 0, ctx->stream>>>(d_Da, d_Db, d_Gb, 1.0, -1.0, nn);
+    k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Gb, ctx->d_D, nullptr, n, 0.5);
+    MYQC_CHECK(launch_digest<double>(ctx, ctx->d_D, ctx->d_x64, ctx->d_Dpair, ctx->d_partial, ctx->d_A, 0.0));
+    k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n, 1.0);   // Gk(Da - Db)
+    MYQC_CHECK(digest_G(ctx, d_Ga, d_Gb, MYQC_FP64));                                           // G(Da + Db)
+    k_axpby<<<256, 256, 0, ctx->stream>>>(d_Gb, ctx->d_G, d_Ga, 1.0, 1.0, nn);
+    k_axpby<<<256, 256, 0, ctx->stream>>>(d_Gb, ctx->d_G, d_Gb, 1.0, -1.0, nn);
+    MYQC_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
 }  // namespace myqc
 
 using namespace myqc;
+
+extern "C" int myqc_fock_G_uhf(myqc_ctx *ctx, const double *Da, const double *Db, double *Ga, double *Gb) {
+    if (!ctx || !Da || !Db || !Ga || !Gb) return MYQC_E_ARG;
+    if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "myqc_fock_G_uhf: call myqc_eri_build first");
+    const size_t nn = (size_t)ctx->n * ctx->n;
+    double *buf = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&buf, sizeof(double) * 4 * nn));
+    double *dA = buf, *dB = buf + nn, *gA = buf + 2 * nn, *gB = buf + 3 * nn;
+    int rc = 0;
+    if (hipMemcpyAsync(dA, Da, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(dB, Db, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "myqc_fock_G_uhf: upload failed");
+    if (rc == 0) rc = digest_G_uhf(ctx, dA, dB, gA, gB);
+    if (rc == 0 && (d2h(ctx, Ga, gA, sizeof(double) * nn) != hipSuccess ||
+                    d2h(ctx, Gb, gB, sizeof(double) * nn) != hipSuccess))
+        rc = set_error(ctx, MYQC_E_HIP, "myqc_fock_G_uhf: download failed");
+    dfree((void **)&buf);
+    return rc;
+}
 
 extern "C" int myqc_fock_G(myqc_ctx *ctx, const double *D, double *G, int precision) {
     if (!ctx || !D || !G) return MYQC_E_ARG;

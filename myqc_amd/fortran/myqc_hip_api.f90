@@ -57,6 +57,12 @@ MODULE myqc_hip_api
       INTEGER(c_int), VALUE :: precision
     END FUNCTION myqc_fock_G
 
+    INTEGER(c_int) FUNCTION myqc_fock_G_uhf(ctx, Da, Db, Ga, Gb) BIND(C, name='myqc_fock_G_uhf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: Da(*), Db(*), Ga(*), Gb(*)
+    END FUNCTION myqc_fock_G_uhf
+
     INTEGER(c_int) FUNCTION myqc_density_rhf(ctx, C, nocc, D) BIND(C, name='myqc_density_rhf')
       IMPORT :: c_int, c_ptr, c_double
       TYPE(c_ptr), VALUE :: ctx

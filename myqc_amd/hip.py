@@ -34,6 +34,7 @@ EXPORTS = {
     "myqc_eri_full": ([_vp, ctypes.c_double, _dp], ctypes.c_int),
     "myqc_eri_build": ([_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "myqc_fock_G": ([_vp, _dp, _dp, ctypes.c_int], ctypes.c_int),
+    "myqc_fock_G_uhf": ([_vp, _dp, _dp, _dp, _dp], ctypes.c_int),
     "myqc_density_rhf": ([_vp, _dp, ctypes.c_int, _dp], ctypes.c_int),
     "myqc_ao2mo_ovov": ([_vp, _dp, ctypes.c_int, _dp, ctypes.c_int], ctypes.c_int),
     "myqc_mp2_rhf": ([_vp, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
@@ -170,6 +171,14 @@ class Context:
         G = np.zeros((n, n), order="F")
         self._chk(self.L.myqc_fock_G(self.h, _p(Df), _p(G), precision))
         return G
+
+    def fock_G_uhf(self, Da, Db):
+        """UHFI2G: the two spin G matrices from the stored integrals."""
+        n = self.n
+        Ga = np.zeros((n, n), order="F")
+        Gb = np.zeros((n, n), order="F")
+        self._chk(self.L.myqc_fock_G_uhf(self.h, _p(_f(Da)), _p(_f(Db)), _p(Ga), _p(Gb)))
+        return Ga, Gb
 
     def density(self, C, nocc: int) -> np.ndarray:
         n = self.n

@@ -4,7 +4,7 @@
 # (objects, module files, executables) go only to oracle/_ref/.  Nothing of the
 # reference is copied into the repository.
 #
-# Built: int1e, int2e, RHFI2G, dens, ao2mo, mp2 (they need nothing beyond the
+# Built: int1e, int2e, RHFI2G, UHFI2G, dens, ao2mo, mp2 (they need nothing beyond the
 # Fortran runtime).  Not built: scf -- it calls LAPACK DSYGV/DSYEV and the image
 # has no LAPACK library, so it is unbuildable here; the SCF driver loop is
 # restated in oracle/scf_driver.py on top of the reference programs built here.
@@ -22,6 +22,7 @@ $FC $FFLAGS -c "$SRC/integrals/auxilary.f90" -o "$OUT/auxilary.o"
 $FC $FFLAGS -o "$OUT/int1e" "$SRC/integrals/int1e.f90" "$OUT/auxilary.o" "$OUT/env.o" "$OUT/basis.o"
 $FC $FFLAGS -o "$OUT/int2e" "$SRC/integrals/int2e.f90" "$OUT/auxilary.o" "$OUT/env.o" "$OUT/basis.o"
 $FC $FFLAGS -o "$OUT/RHFI2G" "$SRC/I2G/RHFI2G.f90" "$OUT/env.o"
+$FC $FFLAGS -o "$OUT/UHFI2G" "$SRC/I2G/UHFI2G.f90" "$OUT/env.o"
 $FC $FFLAGS -o "$OUT/dens" "$SRC/dens/dens.f90" "$OUT/env.o" "$OUT/basis.o"
 $FC $FFLAGS -o "$OUT/ao2mo" "$SRC/ao2mo/ao2mo.f90" "$OUT/env.o"
 $FC $FFLAGS -o "$OUT/mp2" "$SRC/mp2/mp2.f90" "$OUT/env.o"

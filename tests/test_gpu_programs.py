@@ -210,3 +210,38 @@ def test_int2e_record_above_2gib(ctx, tmp_path):
     err = np.max(np.abs(G - G_ref))
     print(f"[XX] reference RHFI2G on our XX: max |G - G_ref| = {err:.2e} (max |G| {np.max(np.abs(G_ref)):.2e})")
     assert err <= 1e-9 * max(1.0, np.max(np.abs(G_ref)))
+
+
+@pytest.mark.parametrize("mol", ["co2", "h2o_2"])
+def test_uhf_fock_matches_reference_UHFI2G(ctx, mol, tmp_path):
+    """myqc_fock_G_uhf against the reference's own UHFI2G program (built from
+    src/I2G/UHFI2G.f90 into oracle/_ref) reading the XX our int2e writes, for
+    random symmetric spin densities; with Da = Db = D/2 both equal RHF's G(D)."""
+    g = golden(mol)
+    n = int(g["n"])
+    Z = [int(z) for z in g["Z"]]
+    ctx.set_system(Z, g["xyz"])
+    ctx.eri_build(0.0)
+    rng = np.random.default_rng(5)
+    Da, Db = (0.5 * (M + M.T) for M in (rng.standard_normal((n, n)), rng.standard_normal((n, n))))
+    Ga, Gb = ctx.fock_G_uhf(Da, Db)
+    Gh, Gh2 = ctx.fock_G_uhf(0.5 * Da, 0.5 * Da)
+    G = ctx.fock_G(Da)
+    assert np.max(np.abs(Gh - G)) <= 1e-12 * max(1.0, np.max(np.abs(G))) and np.array_equal(Gh, Gh2)
+    uhfi2g = os.path.join(ROOT, "oracle", "_ref", "UHFI2G")
+    if not os.path.exists(uhfi2g):
+        pytest.skip("reference UHFI2G not built here (oracle/build_ref.sh)")
+    d = str(tmp_path)
+    _write_inputs(d, g["Z"], g["xyz"])
+    _run(d, "int2e")
+    with open(os.path.join(d, "Da"), "wb") as f:
+        for M in (Da, Db):
+            b = np.asfortranarray(M, dtype="<f8").ravel(order="F").tobytes()
+            f.write(struct.pack("<i", len(b)) + b + struct.pack("<i", len(b)))
+    print(f"[program] reference UHFI2G in {d}", flush=True)
+    r = subprocess.run([uhfi2g], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and not os.path.exists(os.path.join(d, "error")), r.stdout + r.stderr
+    GaR, GbR = (x.reshape(n, n, order="F") for x in _records(os.path.join(d, "Guv")))
+    scale = max(1.0, np.max(np.abs(GaR)))
+    assert np.max(np.abs(Ga - GaR)) <= 1e-10 * scale
+    assert np.max(np.abs(Gb - GbR)) <= 1e-10 * scale
