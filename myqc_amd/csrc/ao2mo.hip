@@ -4,7 +4,8 @@
 // (idx1..idx4_trans, src/ao2mo/ao2mo.f90:1306-1439) but runs each as one
 // dense GEMM (or a strided batch of GEMMs) over the device-resident dense AO
 // integrals; the FP64 form uses rocBLAS DGEMM (plain library GEMMs), the
-// split-precision form the hand-written MFMA kernel in mfma_gemm.hip.
+// split-precision form the hand-written MFMA kernels in mfma_gemm.hip, with
+// steps 1 and 2 fused so that the o n^3 intermediate is never stored.
 // The MP2 reduction restates mp2_rhf (src/mp2/mp2.f90:79-144) as one pass
 // with a deterministic two-stage sum.
 #include <cmath>
@@ -15,6 +16,7 @@
 namespace myqc {
 int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A, int lda, long long strideA,
                 const double *B, int ldb, long long strideB, double *C, int ldc, long long strideC, int batch);
+int ao2mo12_bf16x3(myqc_ctx *ctx, int o, int n, const double *C, const double *XX, double *M);
 }
 
 namespace myqc {
@@ -111,15 +113,22 @@ static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
     TRY(eri_fill_dense(ctx, ctx->have_eri ? ctx->eri_tol : 0.0, dXX));
     hipEvent_t ev;
     TRY(timer_begin(ctx, T_AO2MO, &ev));
-    // 1: L(i,v,l,d) = sum_u C(u,i) XX(u,v,l,d)          (idx1_trans)
-    TRY(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)o * N * N * N));
-    TRY(dgemm(ctx, rocblas_operation_transpose, o, n * n * n, n, dC, n, 0, dXX, n, 0, dL, o, 0, 1, precision));
-    dfree((void **)&dXX);
-    // 2: M(i,a,l,d) = sum_v L(i,v,l,d) C(v,a)            (idx2_trans)
-    TRY(dalloc(ctx, (void **)&dM, sizeof(double) * (size_t)o * nv * N * N));
-    TRY(dgemm(ctx, rocblas_operation_none, o, nv, n, dL, o, (long long)o * n, dC + N * o, n, 0, dM, o,
-              (long long)o * nv, n * n, precision));
-    dfree((void **)&dL);
+    if (precision == MYQC_BF16X3) {
+        // 1 + 2 fused per (l,d) block: L never leaves the workgroup
+        TRY(dalloc(ctx, (void **)&dM, sizeof(double) * (size_t)o * nv * N * N));
+        TRY(ao2mo12_bf16x3(ctx, o, n, dC, dXX, dM));
+        dfree((void **)&dXX);   // (hipFree waits for the kernel)
+    } else {
+        // 1: L(i,v,l,d) = sum_u C(u,i) XX(u,v,l,d)          (idx1_trans)
+        TRY(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)o * N * N * N));
+        TRY(dgemm(ctx, rocblas_operation_transpose, o, n * n * n, n, dC, n, 0, dXX, n, 0, dL, o, 0, 1, precision));
+        dfree((void **)&dXX);
+        // 2: M(i,a,l,d) = sum_v L(i,v,l,d) C(v,a)            (idx2_trans)
+        TRY(dalloc(ctx, (void **)&dM, sizeof(double) * (size_t)o * nv * N * N));
+        TRY(dgemm(ctx, rocblas_operation_none, o, nv, n, dL, o, (long long)o * n, dC + N * o, n, 0, dM, o,
+                  (long long)o * nv, n * n, precision));
+        dfree((void **)&dL);
+    }
     // 3: N(i,a,j,d) = sum_l M(i,a,l,d) C(l,j)            (idx3_trans)
     TRY(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)o * nv * o * N));
     TRY(dgemm(ctx, rocblas_operation_none, o * nv, o, n, dM, o * nv, (long long)o * nv * n, dC, n, 0, dN, o * nv,

@@ -238,6 +238,163 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_nn(int m, int n, int k, int
             }
 }
 
+// Quarter steps 1 and 2 of the RHF transform fused, for one (l,d) block per
+// workgroup (blockIdx.x = l + n d) and a 128-row tile of i (blockIdx.y) and a
+// 64-column tile of a (blockIdx.z):
+//   L(i,v) = sum_u C(u,i) XX(u,v,l,d),   M(i,a,l,d) = sum_v L(i,v) C(v,o+a)
+// L is formed 32 columns of v at a time in LDS and never leaves the
+// workgroup.  Each step keeps the split arithmetic of k_gemm_bf16x3_tn and
+// k_gemm_bf16x3_nn exactly: L is rounded as the FP64 value acc + cor those
+// kernels would store, then split again, and both k loops run in the same
+// 32-wide order, so M is bitwise the unfused result.  Co: planes of C_occ^T
+// (k_split_rows_t, rows i, kp), Cv: planes of C_virt^T (rows a, kp).
+__global__ __launch_bounds__(256) void k_ao2mo12_bf16x3(int o, int nv, int n, int kp, const uint16_t *__restrict__ Coh,
+                                                        const uint16_t *__restrict__ Col,
+                                                        const uint16_t *__restrict__ Cvh,
+                                                        const uint16_t *__restrict__ Cvl,
+                                                        const double *__restrict__ XX, double *__restrict__ M) {
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[GM][GKP], sAl[GM][GKP];   // C_occ^T tile (i, u)
+    __shared__ __attribute__((aligned(16))) uint16_t sBh[32][GKP], sBl[32][GKP];   // XX tile (v, u)
+    __shared__ __attribute__((aligned(16))) uint16_t sLh[GM][GKP], sLl[GM][GKP];   // L slice (i, v)
+    __shared__ __attribute__((aligned(16))) uint16_t sVh[64][GKP], sVl[64][GKP];   // C_virt^T tile (a, v)
+    const long long ld = blockIdx.x;
+    const int i0 = blockIdx.y * GM, a0 = blockIdx.z * 64;
+    const double *X = XX + ld * (long long)n * n;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int wr = 32 * w;   // the wavefront's 32 rows of i
+    // XX loader: column v = t >> 3 of the slice, u = 4 (t & 7) .. +3 of the k-step
+    const int bv = t >> 3, bu = (t & 7) * 4;
+    f32x4 accM[2][4], corM[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            accM[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            corM[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        }
+    for (int v0 = 0; v0 < n; v0 += 32) {
+        // C_virt^T planes for this slice of v (rows a0 .. a0 + 63)
+        {
+            const int row = t >> 2, ch = (t & 3) * 8;
+            const size_t src = (size_t)(a0 + row) * kp + v0 + ch;
+            *reinterpret_cast<uint4 *>(&sVh[row][ch]) = *reinterpret_cast<const uint4 *>(Cvh + src);
+            *reinterpret_cast<uint4 *>(&sVl[row][ch]) = *reinterpret_cast<const uint4 *>(Cvl + src);
+        }
+        f32x4 accL[2][2], corL[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                accL[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+                corL[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            }
+        for (int k0 = 0; k0 < n; k0 += GK) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int id = t + 256 * c, row = id >> 2, ch = (id & 3) * 8;
+                const size_t src = (size_t)(i0 + row) * kp + k0 + ch;
+                *reinterpret_cast<uint4 *>(&sAh[row][ch]) = *reinterpret_cast<const uint4 *>(Coh + src);
+                *reinterpret_cast<uint4 *>(&sAl[row][ch]) = *reinterpret_cast<const uint4 *>(Col + src);
+            }
+            {
+                const int v = v0 + bv;
+                uint16_t h[4], l[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int u = k0 + bu + e;
+                    const double x = (v < n && u < n) ? X[u + (long long)n * v] : 0.0;
+                    split(x, h[e], l[e]);
+                }
+                *reinterpret_cast<uint2 *>(&sBh[bv][bu]) =
+                    make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+                *reinterpret_cast<uint2 *>(&sBl[bv][bu]) =
+                    make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
+            }
+            __syncthreads();
+            bf16x8 bh[2], bl[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                bh[j] = *reinterpret_cast<const bf16x8 *>(&sBh[16 * j + r16][8 * q]);
+                bl[j] = *reinterpret_cast<const bf16x8 *>(&sBl[16 * j + r16][8 * q]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&sAh[wr + 16 * i + r16][8 * q]);
+                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&sAl[wr + 16 * i + r16][8 * q]);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accL[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], accL[i][j], 0, 0, 0);
+                    corL[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], corL[i][j], 0, 0, 0);
+                    corL[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], corL[i][j], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+        // the L slice, rounded as the stored FP64 value would be, split into LDS
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) {
+                    const int row = wr + 16 * i + 4 * q + vv, col = 16 * j + r16;
+                    const double lv = (double)accL[i][j][vv] + (double)corL[i][j][vv];
+                    split(lv, sLh[row][col], sLl[row][col]);
+                }
+        __syncthreads();
+        bf16x8 vh[4], vl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            vh[j] = *reinterpret_cast<const bf16x8 *>(&sVh[16 * j + r16][8 * q]);
+            vl[j] = *reinterpret_cast<const bf16x8 *>(&sVl[16 * j + r16][8 * q]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const bf16x8 lh = *reinterpret_cast<const bf16x8 *>(&sLh[wr + 16 * i + r16][8 * q]);
+            const bf16x8 ll = *reinterpret_cast<const bf16x8 *>(&sLl[wr + 16 * i + r16][8 * q]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                accM[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lh, vh[j], accM[i][j], 0, 0, 0);
+                corM[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lh, vl[j], corM[i][j], 0, 0, 0);
+                corM[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ll, vh[j], corM[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();   // sL and sV are refilled by the next slice
+    }
+    // M(i, a, l, d) at i + o (a + nv (l + n d))
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int gi = i0 + wr + 16 * i + 4 * q + vv, ga = a0 + 16 * j + r16;
+                if (gi < o && ga < nv)
+                    M[gi + (long long)o * (ga + (long long)nv * ld)] = (double)accM[i][j][vv] + (double)corM[i][j][vv];
+            }
+}
+
+// Quarter steps 1 and 2 of the transform in split-bf16: M (o x nv x n x n)
+// from C (n x n) and the dense integrals XX (n^4).
+int ao2mo12_bf16x3(myqc_ctx *ctx, int o, int n, const double *C, const double *XX, double *M) {
+    const int nv = n - o;
+    if (o <= 0 || nv <= 0) return set_error(ctx, MYQC_E_ARG, "ao2mo12_bf16x3: o=%d nv=%d", o, nv);
+    const int gy = (o + GM - 1) / GM, gz = (nv + 63) / 64;
+    const long long nld = (long long)n * n;
+    if (nld > 0x7fffffffLL || gy > 65535 || gz > 65535) return set_error(ctx, MYQC_E_ARG, "ao2mo12_bf16x3: too large");
+    const int kp = ((n + GK - 1) / GK) * GK;
+    const size_t po = (size_t)gy * GM * kp, pv = (size_t)gz * 64 * kp;
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_planes, sizeof(uint16_t) * 2 * (po + pv)));
+    uint16_t *coh = ctx->d_planes, *col = coh + po, *cvh = col + po, *cvl = cvh + pv;
+    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(C, n, o, n, gy * GM, kp, coh, col);
+    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(C + (size_t)n * o, n, nv, n, gz * 64, kp, cvh, cvl);
+    k_ao2mo12_bf16x3<<<dim3((unsigned)nld, (unsigned)gy, (unsigned)gz), 256, 0, ctx->stream>>>(o, nv, n, kp, coh, col, cvh,
+                                                                                             cvl, XX, M);
+    MYQC_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
 // C (m x n, ldc) = op(A) (m x k) * B (k x n, ldb) for `batch` strided
 // problems; FP64 in and out, split-bf16 MFMA products inside.  The transposed
 // single problem (the first quarter transform: a small A against a large,
