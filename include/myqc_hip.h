@@ -94,6 +94,35 @@ int myqc_ao2mo_ovov(myqc_ctx *ctx, const double *C, int nocc, double *iajb, int 
  * (src/mp2/mp2.f90:79-144) over the (ia|jb) of the last myqc_ao2mo_ovov. */
 int myqc_mp2_rhf(myqc_ctx *ctx, const double *eig, int nocc, double *e_aa, double *e_ab, double *e_tot);
 
+/* slow_ao2mo_MP2_UHF drop-in (src/ao2mo/ao2mo.f90:614-904): the three spin
+ * sets (ia|jb) with i,a from (Ca, nalpha) and j,b from (Cb, nbeta) as the set
+ * requires, written as the record streams of the reference's files, each
+ * record nvrt x nvrt' with a fastest:
+ *   ijab_aa: records (i,j), i < j < nalpha, i-major (:718-724);
+ *   ijab_bb: records (i,j), i < j < nbeta (:804-810);
+ *   ijab_ab: records (i,j), i < nalpha, j < nbeta, i-major, nvrt_a x nvrt_b (:890-896).
+ * Any output may be NULL; all three sets stay on the device for myqc_mp2_uhf.
+ * A set with no occupied or no virtual orbital is empty.  precision
+ * MYQC_FP64 or MYQC_BF16X3. */
+int myqc_ao2mo_ovov_uhf(myqc_ctx *ctx, const double *Ca, const double *Cb, int nalpha, int nbeta, double *ijab_aa,
+                        double *ijab_bb, double *ijab_ab, int precision);
+
+/* mp2_uhf drop-in (src/mp2/mp2.f90:154-232) over the sets of the last
+ * myqc_ao2mo_ovov_uhf:
+ *   E(AA) = sum_{i<j, a<b} ((ia|jb) - (ib|ja))^2 / (ea_i + ea_j - ea_a - ea_b), E(BB) likewise,
+ *   E(AB) = sum_{ijab} (ia|jb)^2 / (ea_i + eb_j - ea_a - eb_b),  E(MBPT2) = E(AA) + E(BB) + E(AB).
+ * eig_a, eig_b: the n orbital energies of each spin (the two lines of eig). */
+int myqc_mp2_uhf(myqc_ctx *ctx, const double *eig_a, const double *eig_b, int nalpha, int nbeta, double *e_aa,
+                 double *e_bb, double *e_ab, double *e_tot);
+
+/* mp2_uhf drop-in for a separate mp2 program: the same sums over the record
+ * streams of ijab_AA, ijab_BB and ijab_AB (layouts as myqc_ao2mo_ovov_uhf
+ * writes them; src/mp2/mp2.f90:176-222), uploaded to the device.
+ * nalpha + nvrt_a must equal nbeta + nvrt_b (= n). */
+int myqc_mp2_uhf_records(myqc_ctx *ctx, const double *ijab_aa, const double *ijab_bb, const double *ijab_ab,
+                         const double *eig_a, const double *eig_b, int nalpha, int nbeta, int nvrt_a, int nvrt_b,
+                         double *e_aa, double *e_bb, double *e_ab, double *e_tot);
+
 /* mp2_rhf drop-in for a separate mp2 program: the same sums over (ia|jb)
  * records read from an ijab_AB file (record (i,j) i-major, nvrt x nvrt with a
  * fastest; src/mp2/mp2.f90:99-135), uploaded to the device. */
@@ -123,6 +152,24 @@ int myqc_rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, doub
 int myqc_rhf_scf_density(myqc_ctx *ctx, const double *H, const double *S, int nocc, double e_nuc,
                          const double *D_guess, double e_tol, double d_tol, int max_iter, int precision,
                          double *energy, double *C, double *eig, int *iterations);
+
+/* densUHF drop-in: Da(u,v) = sum_{i<nalpha} Ca(u,i) Ca(v,i), Db likewise over
+ * nbeta (src/dens/dens.f90:213-228). */
+int myqc_density_uhf(myqc_ctx *ctx, const double *Ca, const double *Cb, int nalpha, int nbeta, double *Da,
+                     double *Db);
+
+/* In-process UHF: the scf.f90 UHF loop (src/scf/scf.f90:221-404 UHF,
+ * :909-1117 UHFiter) with the dens/UHFI2G spawns replaced by device-resident
+ * Da, Db, Ga, Gb and the integrals of myqc_eri_build, plus DIIS over both
+ * spins.  Ca_guess and Cb_guess are both NULL (both spins from the
+ * core-Hamiltonian orbitals, initUHF :518-622) or both given (a Cui restart
+ * file's two matrices, :314-320).  E = (Da.(Fa+H) + Db.(Fb+H))/2 + e_nuc
+ * (:969-975).  Converged when |dE| < e_tol and max|dD| (both spins) < d_tol.
+ * FP64.  Outputs: energy, Ca, Cb (n x n), eig_a, eig_b (n), iterations; each
+ * may be NULL. */
+int myqc_uhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nalpha, int nbeta, double e_nuc,
+                 const double *Ca_guess, const double *Cb_guess, double e_tol, double d_tol, int max_iter,
+                 double *energy, double *Ca, double *Cb, double *eig_a, double *eig_b, int *iterations);
 
 /* Diagnostics of the last myqc_rhf_scf: densities obtained by purification
  * and the iterations that fell back to diagonalisation.  No counterpart in

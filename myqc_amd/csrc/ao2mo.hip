@@ -21,35 +21,64 @@ int ao2mo12_bf16x3(myqc_ctx *ctx, int o, int n, const double *C, const double *X
 
 namespace myqc {
 
-// out[((i*o + j)*nv + b)*nv + a] = O[i + o*(a + nv*(j + o*b))]
-__global__ void k_records(const double *__restrict__ O, double *__restrict__ out, int o, int nv) {
-    const size_t total = (size_t)o * o * nv * nv;
+// Record streams of an ijab file: record (i,j) i-major, (a,b) a fastest
+// (src/ao2mo/ao2mo.f90:566-582 RHF; :718-724, :804-810, :890-896 UHF).  With
+// tri set only the records i < j are written (ijab_AA, ijab_BB), else all
+// o1 x o2 of them (ijab_AB).  O is [i][a][j][b] with i fastest.
+__device__ __forceinline__ size_t record_of(int i, int j, int o1, int o2, bool tri) {
+    return tri ? (size_t)i * (2 * o1 - i - 1) / 2 + (size_t)(j - i - 1) : (size_t)i * o2 + j;
+}
+
+__global__ void k_records(const double *__restrict__ O, double *__restrict__ out, int o1, int nv1, int o2, int nv2,
+                          int tri) {
+    const size_t total = (size_t)o1 * o2 * nv1 * nv2;
     for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (size_t)gridDim.x * blockDim.x) {
         size_t x = r;
-        const int a = (int)(x % nv); x /= nv;
-        const int b = (int)(x % nv); x /= nv;
-        const int j = (int)(x % o); x /= o;
+        const int a = (int)(x % nv1); x /= nv1;
+        const int b = (int)(x % nv2); x /= nv2;
+        const int j = (int)(x % o2); x /= o2;
         const int i = (int)x;
-        out[r] = O[i + (size_t)o * (a + (size_t)nv * (j + (size_t)o * b))];
+        if (tri && j <= i) continue;
+        out[(record_of(i, j, o1, o2, tri) * nv2 + b) * nv1 + a] = O[i + (size_t)o1 * (a + (size_t)nv1 * (j + (size_t)o2 * b))];
     }
 }
 
-// per-block partial sums of E(AB) (all i,j,a,b) and E(AA) (i<j, a<b)
-__global__ __launch_bounds__(256) void k_mp2(const double *__restrict__ O, const double *__restrict__ eig, int o, int nv,
+// the inverse: records to [i][a][j][b] (entries of absent records are left alone)
+__global__ void k_from_records(const double *__restrict__ rec, double *__restrict__ O, int o1, int nv1, int o2, int nv2,
+                               int tri) {
+    const size_t total = (size_t)o1 * o2 * nv1 * nv2;
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (size_t)gridDim.x * blockDim.x) {
+        size_t x = r;
+        const int a = (int)(x % nv1); x /= nv1;
+        const int b = (int)(x % nv2); x /= nv2;
+        const int j = (int)(x % o2); x /= o2;
+        const int i = (int)x;
+        if (tri && j <= i) continue;
+        O[i + (size_t)o1 * (a + (size_t)nv1 * (j + (size_t)o2 * b))] = rec[(record_of(i, j, o1, o2, tri) * nv2 + b) * nv1 + a];
+    }
+}
+
+// Per-block partial sums over one (ia|jb) set, O = [i][a][j][b] (i fastest),
+// D = e1[i] + e2[j] - e1[o1+a] - e2[o2+b]:
+//   part[2k]   = sum_{i<j, a<b} ((ia|jb) - (ib|ja))^2 / D   (same-spin sets only: same != 0)
+//   part[2k+1] = sum_{ijab} (ia|jb)^2 / D
+// (mp2_rhf, src/mp2/mp2.f90:79-144; mp2_uhf, :154-232)
+__global__ __launch_bounds__(256) void k_mp2(const double *__restrict__ O, const double *__restrict__ e1,
+                                             const double *__restrict__ e2, int o1, int nv1, int o2, int nv2, int same,
                                              double *__restrict__ part) {
-    const size_t total = (size_t)o * nv * o * nv;
+    const size_t total = (size_t)o1 * nv1 * o2 * nv2;
     double s1 = 0.0, s2 = 0.0;
     for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (size_t)gridDim.x * blockDim.x) {
         size_t x = r;
-        const int i = (int)(x % o); x /= o;
-        const int a = (int)(x % nv); x /= nv;
-        const int j = (int)(x % o); x /= o;
+        const int i = (int)(x % o1); x /= o1;
+        const int a = (int)(x % nv1); x /= nv1;
+        const int j = (int)(x % o2); x /= o2;
         const int b = (int)x;
         const double t = O[r];
-        const double den = eig[i] + eig[j] - eig[a + o] - eig[b + o];
+        const double den = e1[i] + e2[j] - e1[a + o1] - e2[b + o2];
         s2 += t * t / den;
-        if (i < j && a < b) {
-            const double u = t - O[i + (size_t)o * (b + (size_t)nv * (j + (size_t)o * a))];
+        if (same && i < j && a < b) {
+            const double u = t - O[i + (size_t)o1 * (b + (size_t)nv1 * (j + (size_t)o2 * a))];
             s1 += u * u / den;
         }
     }
@@ -83,17 +112,85 @@ static int dgemm(myqc_ctx *ctx, rocblas_operation ta, int m, int n, int k, const
     return 0;
 }
 
+// Quarter steps 1 and 2 (idx1_trans, idx2_trans):
+//   M(i,a,l,d) = sum_v [sum_u C(u,i) XX(u,v,l,d)] C(v,o+a)      (o x nv x n x n)
+// With release_xx, *dXX is freed once nothing reads it any more.
+static int quarter12(myqc_ctx *ctx, double **dXX, bool release_xx, const double *dC, int o, int precision,
+                     double **dM) {
+    const int n = ctx->n, nv = n - o;
+    const size_t N = n, msize = sizeof(double) * (size_t)o * nv * N * N;
+    if (precision == MYQC_BF16X3) {
+        // fused per (l,d) block: L never leaves the workgroup
+        MYQC_CHECK(dalloc(ctx, (void **)dM, msize));
+        MYQC_CHECK(ao2mo12_bf16x3(ctx, o, n, dC, *dXX, *dM));
+        if (release_xx) dfree((void **)dXX);   // (hipFree waits for the kernel)
+        return 0;
+    }
+    // 1: L(i,v,l,d) = sum_u C(u,i) XX(u,v,l,d)
+    double *dL = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)o * N * N * N));
+    int rc = dgemm(ctx, rocblas_operation_transpose, o, n * n * n, n, dC, n, 0, *dXX, n, 0, dL, o, 0, 1, precision);
+    if (rc == 0 && release_xx) dfree((void **)dXX);
+    // 2: M(i,a,l,d) = sum_v L(i,v,l,d) C(v,o+a)
+    if (rc == 0) rc = dalloc(ctx, (void **)dM, msize);
+    if (rc == 0)
+        rc = dgemm(ctx, rocblas_operation_none, o, nv, n, dL, o, (long long)o * n, dC + N * o, n, 0, *dM, o,
+                   (long long)o * nv, n * n, precision);
+    dfree((void **)&dL);
+    return rc;
+}
+
+// Quarter steps 3 and 4 (idx3_trans, idx4_trans) with the second pair's
+// orbitals C2 (o2 occupied):
+//   O(i,a,j,b) = sum_d [sum_l M(i,a,l,d) C2(l,j)] C2(d,o2+b)   (o1 x nv1 x o2 x nv2, i fastest)
+static int quarter34(myqc_ctx *ctx, const double *dM, int o1, int nv1, const double *dC2, int o2, int precision,
+                     double *dO) {
+    const int n = ctx->n, nv2 = n - o2, m1 = o1 * nv1;
+    const size_t N = n;
+    double *dN = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)m1 * o2 * N));
+    // 3: N(i,a,j,d) = sum_l M(i,a,l,d) C2(l,j)
+    int rc = dgemm(ctx, rocblas_operation_none, m1, o2, n, dM, m1, (long long)m1 * n, dC2, n, 0, dN, m1,
+                   (long long)m1 * o2, n, precision);
+    // 4: O(i,a,j,b) = sum_d N(i,a,j,d) C2(d,o2+b)
+    if (rc == 0)
+        rc = dgemm(ctx, rocblas_operation_none, m1 * o2, nv2, n, dN, m1 * o2, 0, dC2 + N * o2, n, 0, dO, m1 * o2, 0, 1,
+                   precision);
+    dfree((void **)&dN);
+    return rc;
+}
+
+static double transform_flops(int o1, int nv1, int o2, int nv2, int n) {
+    const double N = n;
+    return 2.0 * ((double)o1 * N * N * N * N + (double)o1 * nv1 * N * N * N + (double)o1 * nv1 * o2 * N * N +
+                  (double)o1 * nv1 * o2 * nv2 * N);
+}
+
+static int upload_square(myqc_ctx *ctx, const double *host, double **dev) {
+    const size_t nn = (size_t)ctx->n * ctx->n;
+    MYQC_CHECK(dalloc(ctx, (void **)dev, sizeof(double) * nn));
+    if (hipMemcpyAsync(*dev, host, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return set_error(ctx, MYQC_E_HIP, "ao2mo: upload failed");
+    return 0;
+}
+
+// dense AO integrals at the stored build's screening threshold
+static int dense_integrals(myqc_ctx *ctx, double **dXX) {
+    const size_t N = ctx->n;
+    MYQC_CHECK(dalloc(ctx, (void **)dXX, sizeof(double) * N * N * N * N));
+    return eri_fill_dense(ctx, ctx->have_eri ? ctx->eri_tol : 0.0, *dXX);
+}
+
+// RHF: (ia|jb) for the one set of orbitals (slow_ao2mo_MP2_RHF,
+// src/ao2mo/ao2mo.f90:465-602) into ctx->d_iajb
 static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
     const int n = ctx->n, nv = n - o;
-    const size_t N = n;
-    double *dXX = nullptr, *dC = nullptr, *dL = nullptr, *dM = nullptr, *dN = nullptr;
+    double *dXX = nullptr, *dC = nullptr, *dM = nullptr;
     int rc = 0;
     auto cleanup = [&]() {
         dfree((void **)&dXX);
         dfree((void **)&dC);
-        dfree((void **)&dL);
         dfree((void **)&dM);
-        dfree((void **)&dN);
     };
 #define TRY(x)          \
     do {                \
@@ -103,53 +200,145 @@ static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
             return rc;  \
         }               \
     } while (0)
-    TRY(dalloc(ctx, (void **)&dC, sizeof(double) * N * N));
-    if (hipMemcpyAsync(dC, C_host, sizeof(double) * N * N, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-        cleanup();
-        return set_error(ctx, MYQC_E_HIP, "ao2mo: upload failed");
-    }
-    TRY(dalloc(ctx, (void **)&dXX, sizeof(double) * N * N * N * N));
-    // dense AO integrals at the stored build's screening threshold
-    TRY(eri_fill_dense(ctx, ctx->have_eri ? ctx->eri_tol : 0.0, dXX));
+    TRY(upload_square(ctx, C_host, &dC));
+    TRY(dense_integrals(ctx, &dXX));
     hipEvent_t ev;
     TRY(timer_begin(ctx, T_AO2MO, &ev));
-    if (precision == MYQC_BF16X3) {
-        // 1 + 2 fused per (l,d) block: L never leaves the workgroup
-        TRY(dalloc(ctx, (void **)&dM, sizeof(double) * (size_t)o * nv * N * N));
-        TRY(ao2mo12_bf16x3(ctx, o, n, dC, dXX, dM));
-        dfree((void **)&dXX);   // (hipFree waits for the kernel)
-    } else {
-        // 1: L(i,v,l,d) = sum_u C(u,i) XX(u,v,l,d)          (idx1_trans)
-        TRY(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)o * N * N * N));
-        TRY(dgemm(ctx, rocblas_operation_transpose, o, n * n * n, n, dC, n, 0, dXX, n, 0, dL, o, 0, 1, precision));
-        dfree((void **)&dXX);
-        // 2: M(i,a,l,d) = sum_v L(i,v,l,d) C(v,a)            (idx2_trans)
-        TRY(dalloc(ctx, (void **)&dM, sizeof(double) * (size_t)o * nv * N * N));
-        TRY(dgemm(ctx, rocblas_operation_none, o, nv, n, dL, o, (long long)o * n, dC + N * o, n, 0, dM, o,
-                  (long long)o * nv, n * n, precision));
-        dfree((void **)&dL);
-    }
-    // 3: N(i,a,j,d) = sum_l M(i,a,l,d) C(l,j)            (idx3_trans)
-    TRY(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)o * nv * o * N));
-    TRY(dgemm(ctx, rocblas_operation_none, o * nv, o, n, dM, o * nv, (long long)o * nv * n, dC, n, 0, dN, o * nv,
-              (long long)o * nv * o, n, precision));
-    dfree((void **)&dM);
-    // 4: O(i,a,j,b) = sum_d N(i,a,j,d) C(d,b)            (idx4_trans)
+    TRY(quarter12(ctx, &dXX, true, dC, o, precision, &dM));
     const size_t oe = (size_t)o * nv * o * nv;
     if (ctx->iajb_elems != oe) {
         TRY(dalloc(ctx, (void **)&ctx->d_iajb, sizeof(double) * (oe ? oe : 1)));
         ctx->iajb_elems = oe;
     }
-    TRY(dgemm(ctx, rocblas_operation_none, o * nv * o, nv, n, dN, o * nv * o, 0, dC + N * o, n, 0, ctx->d_iajb,
-              o * nv * o, 0, 1, precision));
-    const double flops = 2.0 * ((double)o * N * N * N * N + (double)o * nv * N * N * N + (double)o * o * nv * N * N +
-                                (double)o * o * nv * nv * N);
-    TRY(timer_end(ctx, T_AO2MO, ev, flops));
+    TRY(quarter34(ctx, dM, o, nv, dC, o, precision, ctx->d_iajb));
+    TRY(timer_end(ctx, T_AO2MO, ev, transform_flops(o, nv, o, nv, n)));
     ctx->iajb_nocc = o;
     ctx->iajb_nv = nv;
     cleanup();
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// UHF: the three spin sets of slow_ao2mo_MP2_UHF (src/ao2mo/ao2mo.f90:614-904)
+// into ctx->d_uhf[0..2] = AA (Ca, Ca), BB (Cb, Cb), AB (i, a alpha; j, b beta).
+// The reference recomputes the first two quarter steps for each set; here
+// M_A = steps 1-2 with Ca serves both AA and AB, so the dense integrals are
+// contracted twice (Ca, Cb) instead of three times.  A set with no occupied
+// or no virtual orbital is empty.
+static int ao2mo_uhf(myqc_ctx *ctx, const double *Ca_host, const double *Cb_host, int oa, int ob, int precision) {
+    const int n = ctx->n, va = n - oa, vb = n - ob;
+    double *dXX = nullptr, *dCa = nullptr, *dCb = nullptr, *dMa = nullptr, *dMb = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() {
+        dfree((void **)&dXX);
+        dfree((void **)&dCa);
+        dfree((void **)&dCb);
+        dfree((void **)&dMa);
+        dfree((void **)&dMb);
+    };
+    ctx->have_uhf = false;
+    const int dims[3][4] = {{oa, va, oa, va}, {ob, vb, ob, vb}, {oa, va, ob, vb}};
+    for (int s = 0; s < 3; ++s) {
+        const size_t e = (size_t)dims[s][0] * dims[s][1] * dims[s][2] * dims[s][3];
+        TRY(dalloc(ctx, (void **)&ctx->d_uhf[s], sizeof(double) * (e ? e : 1)));
+        for (int k = 0; k < 4; ++k) ctx->uhf_dims[s][k] = dims[s][k];
+    }
+    const bool want_a = (size_t)oa * va > 0, want_b = (size_t)ob * vb > 0;
+    TRY(upload_square(ctx, Ca_host, &dCa));
+    TRY(upload_square(ctx, Cb_host, &dCb));
+    if (want_a || want_b) TRY(dense_integrals(ctx, &dXX));
+    hipEvent_t ev;
+    TRY(timer_begin(ctx, T_AO2MO, &ev));
+    double flops = 0.0;
+    if (want_a) TRY(quarter12(ctx, &dXX, !want_b, dCa, oa, precision, &dMa));
+    if (want_b) TRY(quarter12(ctx, &dXX, true, dCb, ob, precision, &dMb));
+    dfree((void **)&dXX);
+    if (want_a) {
+        TRY(quarter34(ctx, dMa, oa, va, dCa, oa, precision, ctx->d_uhf[0]));
+        flops += transform_flops(oa, va, oa, va, n);
+    }
+    if (want_b) {
+        TRY(quarter34(ctx, dMb, ob, vb, dCb, ob, precision, ctx->d_uhf[1]));
+        flops += transform_flops(ob, vb, ob, vb, n);
+    }
+    if (want_a && want_b) {
+        TRY(quarter34(ctx, dMa, oa, va, dCb, ob, precision, ctx->d_uhf[2]));
+        // steps 3-4 only: steps 1-2 are shared with the AA set
+        flops += 2.0 * ((double)oa * va * ob * n * n + (double)oa * va * ob * vb * n);
+    }
+    TRY(timer_end(ctx, T_AO2MO, ev, flops));
+    cleanup();
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_uhf = true;
 #undef TRY
+    return 0;
+}
+
+// the records of one set into a host buffer (nothing is written for an empty set)
+static int download_records(myqc_ctx *ctx, const double *dO, const int *d, int tri, double *host) {
+    const size_t nrec = tri ? (size_t)d[0] * (d[0] > 0 ? d[0] - 1 : 0) / 2 : (size_t)d[0] * d[2];
+    const size_t cnt = nrec * d[1] * d[3];
+    if (cnt == 0) return 0;
+    double *d_rec = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&d_rec, sizeof(double) * cnt));
+    k_records<<<1024, 256, 0, ctx->stream>>>(dO, d_rec, d[0], d[1], d[2], d[3], tri);
+    int rc = 0;
+    if (d2h(ctx, host, d_rec, sizeof(double) * cnt) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "ao2mo: download failed");
+    dfree((void **)&d_rec);
+    return rc;
+}
+
+// the two partial sums of k_mp2 over one set, folded on the host in a fixed order
+static int mp2_sums(myqc_ctx *ctx, const double *dO, const int *d, const double *d_e1, const double *d_e2, int same,
+                    double *s_same, double *s_all) {
+    *s_same = *s_all = 0.0;
+    if ((size_t)d[0] * d[1] * d[2] * d[3] == 0) return 0;
+    const int nblk = 512;
+    double *d_part = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&d_part, sizeof(double) * 2 * nblk));
+    std::vector<double> part(2 * nblk);
+    k_mp2<<<nblk, 256, 0, ctx->stream>>>(dO, d_e1, d_e2, d[0], d[1], d[2], d[3], same, d_part);
+    int rc = 0;
+    if (d2h(ctx, part.data(), d_part, sizeof(double) * 2 * nblk) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "mp2: download failed");
+    dfree((void **)&d_part);
+    if (rc) return rc;
+    for (int b = 0; b < nblk; ++b) {
+        *s_same += part[2 * b];
+        *s_all += part[2 * b + 1];
+    }
+    return 0;
+}
+
+// E(AA), E(BB), E(AB) of mp2_uhf (src/mp2/mp2.f90:154-232) from ctx->d_uhf
+static int mp2_uhf_sums(myqc_ctx *ctx, const double *eig_a, const double *eig_b, double *e_aa, double *e_bb,
+                        double *e_ab, double *e_tot) {
+    const int n = ctx->uhf_dims[0][0] + ctx->uhf_dims[0][1];
+    double *d_e = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&d_e, sizeof(double) * 2 * n));
+    double s[3] = {0.0, 0.0, 0.0}, unused = 0.0;
+    int rc = 0;
+    if (hipMemcpyAsync(d_e, eig_a, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(d_e + n, eig_b, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "mp2: upload failed");
+    hipEvent_t ev;
+    if (rc == 0) rc = timer_begin(ctx, T_MP2, &ev);
+    if (rc == 0) rc = mp2_sums(ctx, ctx->d_uhf[0], ctx->uhf_dims[0], d_e, d_e, 1, &s[0], &unused);
+    if (rc == 0) rc = mp2_sums(ctx, ctx->d_uhf[1], ctx->uhf_dims[1], d_e + n, d_e + n, 1, &s[1], &unused);
+    if (rc == 0) rc = mp2_sums(ctx, ctx->d_uhf[2], ctx->uhf_dims[2], d_e, d_e + n, 0, &unused, &s[2]);
+    if (rc == 0) {
+        double work = 0.0;
+        for (int k = 0; k < 3; ++k)
+            work += 8.0 * ctx->uhf_dims[k][0] * ctx->uhf_dims[k][1] * (double)ctx->uhf_dims[k][2] * ctx->uhf_dims[k][3];
+        rc = timer_end(ctx, T_MP2, ev, work);
+    }
+    dfree((void **)&d_e);
+    if (rc) return rc;
+    if (e_aa) *e_aa = s[0];
+    if (e_bb) *e_bb = s[1];
+    if (e_ab) *e_ab = s[2];
+    if (e_tot) *e_tot = s[0] + s[1] + s[2];
     return 0;
 }
 
@@ -168,16 +357,8 @@ extern "C" int myqc_ao2mo_ovov(myqc_ctx *ctx, const double *C, int nocc, double 
         return set_error(ctx, MYQC_E_ARG, "myqc_ao2mo_ovov: n=%zu needs more than 200 GB of dense AO integrals", n);
     MYQC_CHECK(ao2mo(ctx, C, nocc, precision));
     if (iajb) {
-        const int o = nocc, nv = ctx->n - nocc;
-        const size_t oe = (size_t)o * o * nv * nv;
-        double *d_rec = nullptr;
-        MYQC_CHECK(dalloc(ctx, (void **)&d_rec, sizeof(double) * (oe ? oe : 1)));
-        k_records<<<1024, 256, 0, ctx->stream>>>(ctx->d_iajb, d_rec, o, nv);
-        int rc = 0;
-        if (d2h(ctx, iajb, d_rec, sizeof(double) * oe) != hipSuccess)
-            rc = set_error(ctx, MYQC_E_HIP, "myqc_ao2mo_ovov: download failed");
-        dfree((void **)&d_rec);
-        return rc;
+        const int d[4] = {nocc, ctx->n - nocc, nocc, ctx->n - nocc};
+        return download_records(ctx, ctx->d_iajb, d, 0, iajb);
     }
     return 0;
 }
@@ -187,47 +368,25 @@ extern "C" int myqc_mp2_rhf(myqc_ctx *ctx, const double *eig, int nocc, double *
     if (!ctx->d_iajb || ctx->iajb_nocc != nocc)
         return set_error(ctx, MYQC_E_STATE, "myqc_mp2_rhf: no (ia|jb) for nocc=%d (call myqc_ao2mo_ovov)", nocc);
     const int o = nocc, nv = ctx->iajb_nv, n = o + nv;
-    double *d_eig = nullptr, *d_part = nullptr;
-    const int nblk = 512;
+    double *d_eig = nullptr;
     MYQC_CHECK(dalloc(ctx, (void **)&d_eig, sizeof(double) * n));
-    int rc = dalloc(ctx, (void **)&d_part, sizeof(double) * 2 * nblk);
-    if (rc == 0 && hipMemcpyAsync(d_eig, eig, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    int rc = 0;
+    if (hipMemcpyAsync(d_eig, eig, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         rc = set_error(ctx, MYQC_E_HIP, "upload failed");
-    std::vector<double> part(2 * nblk);
+    double s1 = 0.0, s2 = 0.0;
     if (rc == 0) {
         hipEvent_t ev;
+        const int d[4] = {o, nv, o, nv};
         rc = timer_begin(ctx, T_MP2, &ev);
-        k_mp2<<<nblk, 256, 0, ctx->stream>>>(ctx->d_iajb, d_eig, o, nv, d_part);
+        if (rc == 0) rc = mp2_sums(ctx, ctx->d_iajb, d, d_eig, d_eig, 1, &s1, &s2);
         if (rc == 0) rc = timer_end(ctx, T_MP2, ev, 8.0 * (double)o * o * nv * nv);
-        if (rc == 0 && d2h(ctx, part.data(), d_part, sizeof(double) * 2 * nblk) != hipSuccess)
-            rc = set_error(ctx, MYQC_E_HIP, "download failed");
     }
     dfree((void **)&d_eig);
-    dfree((void **)&d_part);
     if (rc) return rc;
-    double s1 = 0.0, s2 = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        s1 += part[2 * b];
-        s2 += part[2 * b + 1];
-    }
     if (e_aa) *e_aa = s1;
     if (e_ab) *e_ab = s2;
     if (e_tot) *e_tot = 2.0 * s1 + s2;
     return 0;
-}
-
-// (ia|jb) records of an ijab_AB file (record (i,j) i-major, (a,b) a fastest)
-// to the device's [i][a][j][b] layout
-__global__ void k_from_records(const double *__restrict__ rec, double *__restrict__ O, int o, int nv) {
-    const size_t total = (size_t)o * o * nv * nv;
-    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (size_t)gridDim.x * blockDim.x) {
-        size_t x = r;
-        const int a = (int)(x % nv); x /= nv;
-        const int b = (int)(x % nv); x /= nv;
-        const int j = (int)(x % o); x /= o;
-        const int i = (int)x;
-        O[i + (size_t)o * (a + (size_t)nv * (j + (size_t)o * b))] = rec[r];
-    }
 }
 
 extern "C" int myqc_mp2_rhf_records(myqc_ctx *ctx, const double *iajb_records, const double *eig, int nocc, int nvrt,
@@ -250,11 +409,78 @@ extern "C" int myqc_mp2_rhf_records(myqc_ctx *ctx, const double *iajb_records, c
         dfree((void **)&d_rec);
         return set_error(ctx, MYQC_E_HIP, "myqc_mp2_rhf_records: upload failed");
     }
-    k_from_records<<<1024, 256, 0, ctx->stream>>>(d_rec, ctx->d_iajb, nocc, nvrt);
+    k_from_records<<<1024, 256, 0, ctx->stream>>>(d_rec, ctx->d_iajb, nocc, nvrt, nocc, nvrt, 0);
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     dfree((void **)&d_rec);
     if (e != hipSuccess) return set_error(ctx, MYQC_E_HIP, "myqc_mp2_rhf_records: %s", hipGetErrorString(e));
     ctx->iajb_nocc = nocc;
     ctx->iajb_nv = nvrt;
     return myqc_mp2_rhf(ctx, eig, nocc, e_aa, e_ab, e_tot);
+}
+
+extern "C" int myqc_ao2mo_ovov_uhf(myqc_ctx *ctx, const double *Ca, const double *Cb, int nalpha, int nbeta,
+                                   double *ijab_aa, double *ijab_bb, double *ijab_ab, int precision) {
+    if (!ctx || !Ca || !Cb) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_ao2mo_ovov_uhf: no system set");
+    if (nalpha < 0 || nalpha > ctx->n || nbeta < 0 || nbeta > ctx->n)
+        return set_error(ctx, MYQC_E_ARG, "myqc_ao2mo_ovov_uhf: nalpha=%d nbeta=%d", nalpha, nbeta);
+    if (precision != MYQC_FP64 && precision != MYQC_BF16X3)
+        return set_error(ctx, MYQC_E_ARG, "myqc_ao2mo_ovov_uhf: precision %d", precision);
+    const size_t n = ctx->n;
+    if (n * n * n * n * 8 > (size_t)200 << 30)
+        return set_error(ctx, MYQC_E_ARG, "myqc_ao2mo_ovov_uhf: n=%zu needs more than 200 GB of dense AO integrals", n);
+    MYQC_CHECK(ao2mo_uhf(ctx, Ca, Cb, nalpha, nbeta, precision));
+    if (ijab_aa) MYQC_CHECK(download_records(ctx, ctx->d_uhf[0], ctx->uhf_dims[0], 1, ijab_aa));
+    if (ijab_bb) MYQC_CHECK(download_records(ctx, ctx->d_uhf[1], ctx->uhf_dims[1], 1, ijab_bb));
+    if (ijab_ab) MYQC_CHECK(download_records(ctx, ctx->d_uhf[2], ctx->uhf_dims[2], 0, ijab_ab));
+    return 0;
+}
+
+extern "C" int myqc_mp2_uhf(myqc_ctx *ctx, const double *eig_a, const double *eig_b, int nalpha, int nbeta,
+                            double *e_aa, double *e_bb, double *e_ab, double *e_tot) {
+    if (!ctx || !eig_a || !eig_b) return MYQC_E_ARG;
+    if (!ctx->have_uhf || ctx->uhf_dims[0][0] != nalpha || ctx->uhf_dims[1][0] != nbeta)
+        return set_error(ctx, MYQC_E_STATE, "myqc_mp2_uhf: no UHF (ia|jb) for nalpha=%d nbeta=%d (call myqc_ao2mo_ovov_uhf)",
+                         nalpha, nbeta);
+    return mp2_uhf_sums(ctx, eig_a, eig_b, e_aa, e_bb, e_ab, e_tot);
+}
+
+extern "C" int myqc_mp2_uhf_records(myqc_ctx *ctx, const double *ijab_aa, const double *ijab_bb,
+                                    const double *ijab_ab, const double *eig_a, const double *eig_b, int nalpha,
+                                    int nbeta, int nvrt_a, int nvrt_b, double *e_aa, double *e_bb, double *e_ab,
+                                    double *e_tot) {
+    if (!ctx || !eig_a || !eig_b) return MYQC_E_ARG;
+    if (nalpha < 0 || nbeta < 0 || nvrt_a < 0 || nvrt_b < 0 || nalpha + nvrt_a != nbeta + nvrt_b)
+        return set_error(ctx, MYQC_E_ARG, "myqc_mp2_uhf_records: nalpha=%d nbeta=%d nvrt_a=%d nvrt_b=%d", nalpha,
+                         nbeta, nvrt_a, nvrt_b);
+    const int dims[3][4] = {{nalpha, nvrt_a, nalpha, nvrt_a}, {nbeta, nvrt_b, nbeta, nvrt_b},
+                            {nalpha, nvrt_a, nbeta, nvrt_b}};
+    const double *recs[3] = {ijab_aa, ijab_bb, ijab_ab};
+    ctx->have_uhf = false;
+    for (int s = 0; s < 3; ++s) {
+        const int *d = dims[s];
+        const int tri = s < 2;
+        const size_t e = (size_t)d[0] * d[1] * d[2] * d[3];
+        const size_t nrec = tri ? (size_t)d[0] * (d[0] > 0 ? d[0] - 1 : 0) / 2 : (size_t)d[0] * d[2];
+        const size_t cnt = nrec * d[1] * d[3];
+        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_uhf[s], sizeof(double) * (e ? e : 1)));
+        for (int k = 0; k < 4; ++k) ctx->uhf_dims[s][k] = d[k];
+        if (e == 0) continue;
+        // same-spin sets hold only the records i < j: the rest stays zero and unread
+        MYQC_HIP(ctx, hipMemsetAsync(ctx->d_uhf[s], 0, sizeof(double) * e, ctx->stream));
+        if (cnt == 0) continue;
+        if (!recs[s]) return set_error(ctx, MYQC_E_ARG, "myqc_mp2_uhf_records: records of set %d missing", s);
+        double *d_rec = nullptr;
+        MYQC_CHECK(dalloc(ctx, (void **)&d_rec, sizeof(double) * cnt));
+        if (hipMemcpyAsync(d_rec, recs[s], sizeof(double) * cnt, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+            dfree((void **)&d_rec);
+            return set_error(ctx, MYQC_E_HIP, "myqc_mp2_uhf_records: upload failed");
+        }
+        k_from_records<<<1024, 256, 0, ctx->stream>>>(d_rec, ctx->d_uhf[s], d[0], d[1], d[2], d[3], tri);
+        const hipError_t err = hipStreamSynchronize(ctx->stream);
+        dfree((void **)&d_rec);
+        if (err != hipSuccess) return set_error(ctx, MYQC_E_HIP, "myqc_mp2_uhf_records: %s", hipGetErrorString(err));
+    }
+    ctx->have_uhf = true;
+    return mp2_uhf_sums(ctx, eig_a, eig_b, e_aa, e_bb, e_ab, e_tot);
 }

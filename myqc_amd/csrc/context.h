@@ -117,9 +117,15 @@ struct myqc_ctx {
     uint16_t *d_planes = nullptr;         // split-bf16 GEMM: hi/lo planes of the small operand
     size_t iajb_elems = 0;
     int iajb_nocc = 0, iajb_nv = 0;
+    // UHF (ia|jb) sets AA, BB, AB, each [i][a][j][b] with i fastest, and
+    // their (o1, nv1, o2, nv2)
+    double *d_uhf[3] = {nullptr, nullptr, nullptr};
+    int uhf_dims[3][4] = {};
+    bool have_uhf = false;
 
     // last RHF run: purification steps and diagonalisation fallbacks
-    myqc::ScfBuffers *scf = nullptr;       // RHF work arrays, kept across calls
+    myqc::ScfBuffers *scf = nullptr;       // RHF (and UHF alpha) work arrays, kept across calls
+    myqc::ScfBuffers *scf_b = nullptr;     // UHF beta work arrays
     myqc::BatchState *batch = nullptr;     // batched small-molecule RHF (batch.hip)
     myqc::MlpState *mlp = nullptr;         // learned-guess network (mlp.hip)
     int mlp_gen = 0;                      // bumped whenever the network's buffers move
@@ -198,6 +204,7 @@ int eri_schwarz(myqc_ctx *ctx);
 int eri_store(myqc_ctx *ctx, double tol, bool want32);
 int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX);
 int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision);
+int digest_G_uhf(myqc_ctx *ctx, const double *d_Da, const double *d_Db, double *d_Ga, double *d_Gb);
 int ensure_work(myqc_ctx *ctx);
 void free_eri(myqc_ctx *ctx, bool keep_memory = false);
 void free_scf(myqc_ctx *ctx);

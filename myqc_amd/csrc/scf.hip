@@ -72,7 +72,7 @@ __global__ void k_lincomb(const double *__restrict__ hF, size_t nn, LinComb lc, 
 // Job d: mode 0 sum x*y, mode 1 max |x - y|, mode 2 max |x - I|.  Stage one
 // writes one partial per (job, block); stage two folds them in a fixed order,
 // so results are bitwise reproducible.
-constexpr int kRedJobs = 12, kRedBlocks = 64;
+constexpr int kRedJobs = 24, kRedBlocks = 64;
 
 struct RedJobs {
     const double *x[kRedJobs];
@@ -163,10 +163,12 @@ struct ScfBuffers {
 };
 
 void free_scf(myqc_ctx *ctx) {
-    if (ctx->scf) {
-        ctx->scf->release();
-        delete ctx->scf;
-        ctx->scf = nullptr;
+    for (ScfBuffers **p : {&ctx->scf, &ctx->scf_b}) {
+        if (*p) {
+            (*p)->release();
+            delete *p;
+            *p = nullptr;
+        }
     }
 }
 
@@ -219,10 +221,10 @@ static int to_orth(myqc_ctx *ctx, ScfBuffers &b, const double *F, double *Fo, in
     return 0;
 }
 
-// D = 2 X Do X^T
-static int to_ao_density(myqc_ctx *ctx, ScfBuffers &b, const double *Do, double *D, int n) {
+// D = occ X Do X^T (occ = 2 for the RHF total density, 1 for one spin)
+static int to_ao_density(myqc_ctx *ctx, ScfBuffers &b, const double *Do, double *D, int n, double occ = 2.0) {
     MYQC_CHECK(gemm(ctx, NN, TT, n, n, n, 1.0, Do, n, b.X, n, 0.0, b.T1, n));
-    MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 2.0, b.X, n, b.T1, n, 0.0, D, n));
+    MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, occ, b.X, n, b.T1, n, 0.0, D, n));
     return 0;
 }
 
@@ -291,8 +293,10 @@ static int inv_sqrt_eig(myqc_ctx *ctx, ScfBuffers &b, int n) {
 // Orthonormal-basis density projector onto the nocc lowest levels of Fo by
 // purification; falls back to diagonalisation if it does not converge.
 static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int n, int nocc) {
-    if (nocc == 0) {
-        k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(nullptr, nullptr, P, 0.0, 0.0, 0.0, n);
+    if (nocc == 0 || nocc == n) {
+        // empty or full: P = 0 or I (a one-level spectrum gives purification
+        // no interval to map)
+        k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(nullptr, nullptr, P, 0.0, 0.0, nocc == n ? 1.0 : 0.0, n);
         return 0;
     }
     int ok = 0;
@@ -307,13 +311,14 @@ static int purify(myqc_ctx *ctx, ScfBuffers &b, const double *Fo, double *P, int
     return 0;
 }
 
-// D = 2 C_occ C_occ^T (src/dens/dens.f90:115-124)
-static int density(myqc_ctx *ctx, const double *C, int nocc, double *D, int n) {
+// D = occ C_occ C_occ^T: occ = 2 for RHF (src/dens/dens.f90:115-124), 1 for
+// one UHF spin (densUHF, :213-228)
+static int density(myqc_ctx *ctx, const double *C, int nocc, double *D, int n, double occ = 2.0) {
     if (nocc == 0) {
         k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(nullptr, nullptr, D, 0.0, 0.0, 0.0, n);
         return 0;
     }
-    return gemm(ctx, NN, TT, n, n, nocc, 2.0, C, n, C, n, 0.0, D, n);
+    return gemm(ctx, NN, TT, n, n, nocc, occ, C, n, C, n, 0.0, D, n);
 }
 
 // solve the DIIS system on the host (m <= 8), Gaussian elimination with pivoting
@@ -494,6 +499,147 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
     return 0;
 }
 
+// UHF (src/scf/scf.f90:221-404 UHF, :518-622 initUHF, :909-1117 UHFiter).
+// Per spin s: F_s = H + G_s[Da, Db] (the UHFI2G build), the density D_s =
+// C_s,occ C_s,occ^T of the nocc_s lowest levels (densUHF), and
+// E = (Da.(Fa + H) + Db.(Fb + H))/2 + E_nuc (UHFiter, :969-975).  The loop
+// is the RHF one run on both spins together: one DIIS extrapolation with
+// shared coefficients over B_ij = <ea_i, ea_j> + <eb_i, eb_j>, and
+// convergence on |dE|, max|dD| over both spins and the DIIS error.  The alpha
+// spin uses ctx->scf (which also holds H, S, X), the beta spin ctx->scf_b.
+static int uhf_scf(myqc_ctx *ctx, const double *H, const double *S, int na, int nb_, double e_nuc,
+                   const double *Ca_guess, const double *Cb_guess, double e_tol, double d_tol, int max_iter,
+                   double *energy, double *Ca_out, double *Cb_out, double *eiga_out, double *eigb_out,
+                   int *iters_out) {
+    const int n = ctx->n;
+    const size_t nn = (size_t)n * n;
+    const int MD = ScfBuffers::kDiis;
+    if (!ctx->scf) ctx->scf = new ScfBuffers();
+    if (!ctx->scf_b) ctx->scf_b = new ScfBuffers();
+    ScfBuffers &A = *ctx->scf, &B = *ctx->scf_b;
+    ScfBuffers *sp[2] = {&A, &B};
+    const int nocc[2] = {na, nb_};
+    MYQC_CHECK(A.ensure(ctx, n));
+    MYQC_CHECK(B.ensure(ctx, n));
+    MYQC_HIP(ctx, hipMemcpyAsync(A.H, H, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(A.S, S, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+    int ok = 0;
+    MYQC_CHECK(inv_sqrt_ns(ctx, A, n, &ok));
+    if (!ok) MYQC_CHECK(inv_sqrt_eig(ctx, A, n));
+    MYQC_HIP(ctx, hipMemcpyAsync(B.X, A.X, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
+    MYQC_HIP(ctx, hipMemcpyAsync(B.Xi, A.Xi, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool guessed = Ca_guess != nullptr;
+    if (guessed) {
+        // orbitals from a Cui restart file (scf.f90:314-320)
+        const double *Cg[2] = {Ca_guess, Cb_guess};
+        for (int s = 0; s < 2; ++s) {
+            ScfBuffers &b = *sp[s];
+            MYQC_HIP(ctx, hipMemcpyAsync(b.C, Cg[s], sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+            MYQC_CHECK(density(ctx, b.C, nocc[s], b.D, n, 1.0));
+            // Do = S^{1/2} D S^{1/2}
+            MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, b.D, n, b.Xi, n, 0.0, b.T1, n));
+            MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, b.Xi, n, b.T1, n, 0.0, b.Do, n));
+        }
+    } else {
+        // both spins from the core Hamiltonian's orbitals (initUHF, scf.f90:518-622)
+        MYQC_CHECK(to_orth(ctx, A, A.H, A.F, n));
+        for (int s = 0; s < 2; ++s) {
+            MYQC_CHECK(purify(ctx, *sp[s], A.F, sp[s]->Do, n, nocc[s]));
+            MYQC_CHECK(to_ao_density(ctx, *sp[s], sp[s]->Do, sp[s]->D, n, 1.0));
+        }
+    }
+    std::vector<double> Bm((size_t)MD * MD, 0.0);
+    int hist = 0, head = 0;
+    double E = 0.0, E_old = 0.0, dE_prev = 1e300, err_prev = 1e300;
+    bool conv = false;
+    double *fslot[2] = {nullptr, nullptr};
+    int it = 1;
+    const bool trace = getenv("MYQC_SCF_TRACE") != nullptr;
+    while (true) {
+        MYQC_CHECK(digest_G_uhf(ctx, A.D, B.D, A.G, B.G));
+        const int hn = std::min(hist + 1, MD);
+        RedJobs J{};
+        int jdd[2] = {-1, -1}, jb0[2] = {0, 0}, jf[2], jh[2];
+        for (int s = 0; s < 2; ++s) {
+            ScfBuffers &b = *sp[s];
+            k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(A.H, b.G, b.F, 1.0, 1.0, 0.0, n);
+            fslot[s] = b.hF + nn * head;
+            double *eslot = b.hE + nn * head;
+            MYQC_CHECK(to_orth(ctx, b, b.F, fslot[s], n));
+            MYQC_CHECK(gemm(ctx, NN, NN, n, n, n, 1.0, fslot[s], n, b.Do, n, 0.0, b.T2, n));
+            k_antisym<<<grid2(n), 128, 0, ctx->stream>>>(b.T2, eslot, n);
+            jf[s] = add_job(ctx, J, 0, b.D, b.F);
+            jh[s] = add_job(ctx, J, 0, b.D, A.H);
+            if (it > 1) jdd[s] = add_job(ctx, J, 1, b.D, b.Dn);   // Dn holds D_{k-1}
+        }
+        for (int s = 0; s < 2; ++s) {
+            jb0[s] = J.m;
+            for (int j = 0; j < hn; ++j) add_job(ctx, J, 0, sp[s]->hE + nn * head, sp[s]->hE + nn * j);
+        }
+        double r[kRedJobs];
+        MYQC_CHECK(reduce(ctx, A, J, r));
+        E = 0.5 * (r[jf[0]] + r[jh[0]] + r[jf[1]] + r[jh[1]]) + e_nuc;
+        const double dmax = it > 1 ? std::max(r[jdd[0]], r[jdd[1]]) : 1e300;
+        if (trace)
+            fprintf(stderr, "[uhf] it %2d E %.12f dE %.2e max|dD| %.2e |err| %.2e hist %d\n", it, E, fabs(E - E_old),
+                    it > 1 ? dmax : -1.0, sqrt(fabs(r[jb0[0] + head] + r[jb0[1] + head])), hn);
+        if (it > 2 && dE_prev < e_tol && dmax < d_tol && err_prev < 100 * d_tol) {
+            conv = true;
+            break;
+        }
+        if (it > max_iter) break;
+        for (int j = 0; j < hn; ++j) {
+            const double v = r[jb0[0] + j] + r[jb0[1] + j];
+            Bm[head * MD + j] = v;
+            Bm[j * MD + head] = v;
+        }
+        hist = hn;
+        const double *Fuse[2] = {fslot[0], fslot[1]};
+        if (hist >= 2) {
+            std::vector<double> Bs((size_t)hist * hist), c;
+            for (int i = 0; i < hist; ++i)
+                for (int j = 0; j < hist; ++j) Bs[i * hist + j] = Bm[i * MD + j];
+            if (diis_coeffs(Bs, hist, c)) {
+                LinComb lc{};
+                lc.m = hist;
+                for (int i = 0; i < hist; ++i) lc.c[i] = c[i];
+                for (int s = 0; s < 2; ++s) {
+                    k_lincomb<<<256, 256, 0, ctx->stream>>>(sp[s]->hF, nn, lc, sp[s]->Fx);
+                    Fuse[s] = sp[s]->Fx;
+                }
+            }
+        }
+        err_prev = sqrt(fabs(Bm[head * MD + head]));
+        dE_prev = fabs(E - E_old);
+        E_old = E;
+        head = (head + 1) % MD;
+        for (int s = 0; s < 2; ++s) {
+            ScfBuffers &b = *sp[s];
+            const double *Fs = Fuse[s];
+            if (guessed && err_prev > kShiftErr) {
+                k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(Fs, b.Do, b.T1, 1.0, -kShift, kShift, n);
+                Fs = b.T1;
+            }
+            MYQC_CHECK(purify(ctx, b, Fs, b.Don, n, nocc[s]));
+            std::swap(b.D, b.Dn);
+            std::swap(b.Do, b.Don);
+            MYQC_CHECK(to_ao_density(ctx, b, b.Do, b.D, n, 1.0));
+        }
+        ++it;
+    }
+    // orbitals of each spin's plain Fock matrix of the reported densities
+    for (int s = 0; s < 2; ++s) MYQC_CHECK(diag_orth(ctx, *sp[s], fslot[s], n));
+    if (energy) *energy = E;
+    if (Ca_out) MYQC_HIP(ctx, d2h(ctx, Ca_out, A.C, sizeof(double) * nn));
+    if (Cb_out) MYQC_HIP(ctx, d2h(ctx, Cb_out, B.C, sizeof(double) * nn));
+    if (eiga_out) MYQC_HIP(ctx, d2h(ctx, eiga_out, A.eig, sizeof(double) * n));
+    if (eigb_out) MYQC_HIP(ctx, d2h(ctx, eigb_out, B.eig, sizeof(double) * n));
+    if (iters_out) *iters_out = it - 1;
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!conv) return set_error(ctx, MYQC_E_NOCONV, "UHF did not converge in %d iterations", max_iter);
+    return 0;
+}
+
 }  // namespace myqc
 
 using namespace myqc;
@@ -554,4 +700,44 @@ extern "C" int myqc_rhf_scf_density(myqc_ctx *ctx, const double *H, const double
     ctx->scf_purify = ctx->scf_diag = ctx->scf_sp2_steps = 0;
     return rhf_scf(ctx, H, S, nocc, e_nuc, nullptr, D_guess, e_tol, d_tol, max_iter, precision, energy, C, eig,
                    iterations);
+}
+
+extern "C" int myqc_density_uhf(myqc_ctx *ctx, const double *Ca, const double *Cb, int nalpha, int nbeta, double *Da,
+                                double *Db) {
+    if (!ctx || !Ca || !Cb || !Da || !Db) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_density_uhf: no system set");
+    const int n = ctx->n;
+    if (nalpha < 0 || nalpha > n || nbeta < 0 || nbeta > n)
+        return set_error(ctx, MYQC_E_ARG, "myqc_density_uhf: nalpha=%d nbeta=%d", nalpha, nbeta);
+    const size_t nn = (size_t)n * n;
+    double *buf = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&buf, sizeof(double) * 2 * nn));
+    const double *Cs[2] = {Ca, Cb};
+    double *Ds[2] = {Da, Db};
+    const int occ[2] = {nalpha, nbeta};
+    int rc = 0;
+    for (int s = 0; s < 2 && rc == 0; ++s) {
+        if (hipMemcpyAsync(buf, Cs[s], sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = set_error(ctx, MYQC_E_HIP, "upload failed");
+        if (rc == 0) rc = density(ctx, buf, occ[s], buf + nn, n, 1.0);
+        if (rc == 0 && d2h(ctx, Ds[s], buf + nn, sizeof(double) * nn) != hipSuccess)
+            rc = set_error(ctx, MYQC_E_HIP, "download failed");
+    }
+    dfree((void **)&buf);
+    return rc;
+}
+
+extern "C" int myqc_uhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nalpha, int nbeta, double e_nuc,
+                            const double *Ca_guess, const double *Cb_guess, double e_tol, double d_tol, int max_iter,
+                            double *energy, double *Ca, double *Cb, double *eig_a, double *eig_b, int *iterations) {
+    if (!ctx || !H || !S) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_uhf_scf: no system set");
+    if (nalpha < 0 || nalpha > ctx->n || nbeta < 0 || nbeta > ctx->n || max_iter < 1 || e_tol <= 0 || d_tol <= 0)
+        return set_error(ctx, MYQC_E_ARG, "myqc_uhf_scf: invalid arguments");
+    if ((Ca_guess == nullptr) != (Cb_guess == nullptr))
+        return set_error(ctx, MYQC_E_ARG, "myqc_uhf_scf: give both guess orbital sets or neither");
+    if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "myqc_uhf_scf: call myqc_eri_build first");
+    ctx->scf_purify = ctx->scf_diag = ctx->scf_sp2_steps = 0;
+    return uhf_scf(ctx, H, S, nalpha, nbeta, e_nuc, Ca_guess, Cb_guess, e_tol, d_tol, max_iter, energy, Ca, Cb, eig_a,
+                   eig_b, iterations);
 }

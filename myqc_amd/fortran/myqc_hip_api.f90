@@ -103,6 +103,43 @@ MODULE myqc_hip_api
       REAL(c_double) :: energy, C(*), eig(*)
       INTEGER(c_int) :: iters
     END FUNCTION myqc_rhf_scf
+
+    INTEGER(c_int) FUNCTION myqc_density_uhf(ctx, Ca, Cb, nalpha, nbeta, Da, Db) BIND(C, name='myqc_density_uhf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: Ca(*), Cb(*), Da(*), Db(*)
+      INTEGER(c_int), VALUE :: nalpha, nbeta
+    END FUNCTION myqc_density_uhf
+
+    INTEGER(c_int) FUNCTION myqc_uhf_scf(ctx, H, S, nalpha, nbeta, e_nuc, Ca_guess, Cb_guess, e_tol, d_tol, &
+                                         max_iter, energy, Ca, Cb, eig_a, eig_b, iters) BIND(C, name='myqc_uhf_scf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: H(*), S(*)
+      INTEGER(c_int), VALUE :: nalpha, nbeta, max_iter
+      REAL(c_double), VALUE :: e_nuc, e_tol, d_tol
+      TYPE(c_ptr), VALUE :: Ca_guess, Cb_guess
+      REAL(c_double) :: energy, Ca(*), Cb(*), eig_a(*), eig_b(*)
+      INTEGER(c_int) :: iters
+    END FUNCTION myqc_uhf_scf
+
+    INTEGER(c_int) FUNCTION myqc_ao2mo_ovov_uhf(ctx, Ca, Cb, nalpha, nbeta, rec_aa, rec_bb, rec_ab, precision) &
+        BIND(C, name='myqc_ao2mo_ovov_uhf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: Ca(*), Cb(*), rec_aa(*), rec_bb(*), rec_ab(*)
+      INTEGER(c_int), VALUE :: nalpha, nbeta, precision
+    END FUNCTION myqc_ao2mo_ovov_uhf
+
+    INTEGER(c_int) FUNCTION myqc_mp2_uhf_records(ctx, rec_aa, rec_bb, rec_ab, eig_a, eig_b, nalpha, nbeta, &
+                                                 nvrt_a, nvrt_b, e_aa, e_bb, e_ab, e_tot) &
+        BIND(C, name='myqc_mp2_uhf_records')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: rec_aa(*), rec_bb(*), rec_ab(*), eig_a(*), eig_b(*)
+      INTEGER(c_int), VALUE :: nalpha, nbeta, nvrt_a, nvrt_b
+      REAL(c_double) :: e_aa, e_bb, e_ab, e_tot
+    END FUNCTION myqc_mp2_uhf_records
   END INTERFACE
 
 CONTAINS

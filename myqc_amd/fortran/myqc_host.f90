@@ -241,6 +241,29 @@ CONTAINS
     CLOSE(u)
   END SUBROUTINE write_text_matrix
 
+  ! two matrices, one list-directed record each (the UHF Cui file,
+  ! src/scf/scf.f90:316-320 read, :1081-1084 write)
+  SUBROUTINE read_text_matrix2(fname, A, B)
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    REAL(8), INTENT(OUT) :: A(:,:), B(:,:)
+    INTEGER :: u, ios
+    OPEN(newunit=u, file=fname, status='old', iostat=ios)
+    IF (ios /= 0) CALL host_fail("Couldn't open " // fname)
+    READ(u,*) A(:,:)
+    READ(u,*) B(:,:)
+    CLOSE(u)
+  END SUBROUTINE read_text_matrix2
+
+  SUBROUTINE write_text_matrix2(fname, A, B)
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    REAL(8), INTENT(IN) :: A(:,:), B(:,:)
+    INTEGER :: u
+    OPEN(newunit=u, file=fname, status='replace')
+    WRITE(u,*) A(:,:)
+    WRITE(u,*) B(:,:)
+    CLOSE(u)
+  END SUBROUTINE write_text_matrix2
+
   REAL(8) FUNCTION nuclear_repulsion(xyz, atoms) RESULT(e)
     REAL(8), INTENT(IN) :: xyz(0:,0:)
     INTEGER, INTENT(IN) :: atoms(0:)

@@ -40,6 +40,13 @@ EXPORTS = {
     "myqc_mp2_rhf": ([_vp, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "myqc_mp2_rhf_records": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
+    "myqc_density_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "myqc_uhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_double,
+                      ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
+    "myqc_ao2mo_ovov_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
+    "myqc_mp2_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "myqc_mp2_uhf_records": ([_vp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              _dp, _dp, _dp, _dp], ctypes.c_int),
     "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
                       ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
     "myqc_rhf_scf_density": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
@@ -249,6 +256,75 @@ class Context:
         self._chk(self.L.myqc_scf_info(self.h, ctypes.byref(npur), ctypes.byref(ndiag)))
         return dict(E=E.value, C=C, eig=eig, iterations=it.value, purifications=npur.value,
                     diag_fallbacks=ndiag.value)
+
+    # -- UHF ----------------------------------------------------------------
+
+    def density_uhf(self, Ca, Cb, nalpha: int, nbeta: int):
+        n = self.n
+        Da = np.zeros((n, n), order="F")
+        Db = np.zeros((n, n), order="F")
+        self._chk(self.L.myqc_density_uhf(self.h, _p(_f(Ca)), _p(_f(Cb)), nalpha, nbeta, _p(Da), _p(Db)))
+        return Da, Db
+
+    def uhf(self, H, S, nalpha: int, nbeta: int, e_nuc: float, Ca_guess=None, Cb_guess=None, e_tol=1e-10,
+            d_tol=1e-8, max_iter=300):
+        """In-process UHF from the core-Hamiltonian orbitals or (Ca_guess,
+        Cb_guess); returns dict(E, Ca, Cb, eig_a, eig_b, iterations)."""
+        n = self.n
+        Ca = np.zeros((n, n), order="F")
+        Cb = np.zeros((n, n), order="F")
+        ea = np.zeros(n)
+        eb = np.zeros(n)
+        E = ctypes.c_double()
+        it = ctypes.c_int()
+        ga = _f(Ca_guess) if Ca_guess is not None else None
+        gb = _f(Cb_guess) if Cb_guess is not None else None
+        self._chk(self.L.myqc_uhf_scf(self.h, _p(_f(H)), _p(_f(S)), nalpha, nbeta, e_nuc,
+                                      _p(ga) if ga is not None else None, _p(gb) if gb is not None else None,
+                                      e_tol, d_tol, max_iter, ctypes.byref(E), _p(Ca), _p(Cb), _p(ea), _p(eb),
+                                      ctypes.byref(it)))
+        return dict(E=E.value, Ca=Ca, Cb=Cb, eig_a=ea, eig_b=eb, iterations=it.value)
+
+    @staticmethod
+    def _uhf_shapes(n, nalpha, nbeta):
+        va, vb = n - nalpha, n - nbeta
+        return {"AA": (nalpha * (nalpha - 1) // 2 if nalpha > 0 else 0, va, va),
+                "BB": (nbeta * (nbeta - 1) // 2 if nbeta > 0 else 0, vb, vb),
+                "AB": (nalpha * nbeta, vb, va)}
+
+    def ao2mo_uhf(self, Ca, Cb, nalpha: int, nbeta: int, precision: int = FP64, download: bool = True):
+        """slow_ao2mo_MP2_UHF: the record streams of ijab_AA, ijab_BB and
+        ijab_AB, each shaped (records, nvrt', nvrt) so that [p, b, a] is
+        (ia|jb) of record p (or None if not downloaded)."""
+        shp = self._uhf_shapes(self.n, nalpha, nbeta)
+        if not download:
+            self._chk(self.L.myqc_ao2mo_ovov_uhf(self.h, _p(_f(Ca)), _p(_f(Cb)), nalpha, nbeta, None, None, None,
+                                                 precision))
+            return None
+        out = {k: np.zeros(max(int(np.prod(v)), 1)) for k, v in shp.items()}
+        self._chk(self.L.myqc_ao2mo_ovov_uhf(self.h, _p(_f(Ca)), _p(_f(Cb)), nalpha, nbeta, _p(out["AA"]),
+                                             _p(out["BB"]), _p(out["AB"]), precision))
+        return {k: out[k][:int(np.prod(shp[k]))].reshape(shp[k]) for k in shp}
+
+    def mp2_uhf(self, eig_a, eig_b, nalpha: int, nbeta: int):
+        """mp2_uhf on the sets of the last ao2mo_uhf: (E(AA), E(BB), E(AB), E(MBPT2))."""
+        ea = np.ascontiguousarray(eig_a, dtype=np.float64)
+        eb = np.ascontiguousarray(eig_b, dtype=np.float64)
+        r = [ctypes.c_double() for _ in range(4)]
+        self._chk(self.L.myqc_mp2_uhf(self.h, _p(ea), _p(eb), nalpha, nbeta, *[ctypes.byref(x) for x in r]))
+        return tuple(x.value for x in r)
+
+    def mp2_uhf_records(self, recs, eig_a, eig_b, nalpha: int, nbeta: int):
+        """mp2_uhf on host record streams (as ao2mo_uhf returns them)."""
+        n = len(eig_a)
+        ea = np.ascontiguousarray(eig_a, dtype=np.float64)
+        eb = np.ascontiguousarray(eig_b, dtype=np.float64)
+        rr = [np.ascontiguousarray(recs[k], dtype=np.float64).ravel() for k in ("AA", "BB", "AB")]
+        rr = [x if x.size else np.zeros(1) for x in rr]
+        r = [ctypes.c_double() for _ in range(4)]
+        self._chk(self.L.myqc_mp2_uhf_records(self.h, _p(rr[0]), _p(rr[1]), _p(rr[2]), _p(ea), _p(eb), nalpha,
+                                              nbeta, n - nalpha, n - nbeta, *[ctypes.byref(x) for x in r]))
+        return tuple(x.value for x in r)
 
     # -- batched small molecules -------------------------------------------
     def batch_set(self, molecules, ftab=None):

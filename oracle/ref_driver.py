@@ -66,12 +66,27 @@ def _fmt(x: float) -> str:
     return repr(float(x))
 
 
-def write_env(d, Z, xyz_bohr, charge=0, calc_mp2=True, scf_conv=11, verb=1):
+def spin_counts(Z, charge=0, mult=1):
+    """(nelcA, nelcB) as the reference parser sets them (src/parser/parser.f90:503-510)."""
     nelc = sum(Z) - charge
-    assert nelc % 2 == 0, "RHF needs a closed shell"
-    opts = [1, 1 if calc_mp2 else 0, 0, 0, 0, 1, 1000000, verb, scf_conv, charge, 1, 1, 1, 0, 0, 0, 0]
+    unpr = mult - 1
+    nb = (nelc - unpr) // 2
+    na = nb + unpr
+    if na + nb != nelc or nb < 0:
+        raise ValueError(f"charge {charge} and multiplicity {mult} not allowed")
+    return na, nb
+
+
+def write_env(d, Z, xyz_bohr, charge=0, calc_mp2=True, scf_conv=11, verb=1, ref=0, mult=1):
+    """envdat/nucpos/fmem as the reference parser writes them; options are
+    0-based (read type, calc, basis, ref, par, nodes, mem, verb, SCF_Conv,
+    charge, multi, units, ao2mo, ...; src/parser/parser.f90:683-729)."""
+    na, nb = spin_counts(Z, charge, mult)
+    if ref == 0 and na != nb:
+        raise ValueError("RHF needs a closed shell")
+    opts = [1, 1 if calc_mp2 else 0, 0, ref, 0, 1, 1000000, verb, scf_conv, charge, mult, 1, 1, 0, 0, 0, 0]
     with open(os.path.join(d, "envdat"), "w") as f:
-        f.write(f" {len(Z)}\n {nelc // 2} {nelc // 2}\n 17\n " + " ".join(str(o) for o in opts) + "\n")
+        f.write(f" {len(Z)}\n {na} {nb}\n 17\n " + " ".join(str(o) for o in opts) + "\n")
     with open(os.path.join(d, "nucpos"), "w") as f:
         for z, r in zip(Z, xyz_bohr):
             f.write(f" {z} {_fmt(r[0])} {_fmt(r[1])} {_fmt(r[2])}\n")
@@ -96,6 +111,18 @@ def read_text_matrix(path, n):
 def write_text_matrix(path, M):
     with open(path, "w") as f:
         f.write(" ".join(_fmt(v) for v in np.asarray(M).ravel(order="F")) + "\n")
+
+
+def write_text_matrices(path, mats):
+    """One list-directed record (line) per matrix, as scf.f90:1081-1084 writes Cui."""
+    with open(path, "w") as f:
+        for M in mats:
+            f.write(" ".join(_fmt(v) for v in np.asarray(M).ravel(order="F")) + "\n")
+
+
+def read_text_matrices(path, n, count):
+    vals = np.array(open(path).read().split(), dtype=np.float64)
+    return [vals[k * n * n:(k + 1) * n * n].reshape(n, n, order="F") for k in range(count)]
 
 
 def enuc(Z, xyz):
@@ -215,3 +242,156 @@ def run_reference(Z, xyz_bohr, charge=0, scf_conv=11, mp2=True, workdir=None, ke
     finally:
         if not keep and workdir is None:
             shutil.rmtree(d, ignore_errors=True)
+
+
+def ao2mo_ovov_pair(XX, C1, o1, C2, o2):
+    """(ia|jb) with i, a from (C1, o1) and j, b from (C2, o2), by the four
+    quarter steps of slow_ao2mo_MP2_UHF in order (ao2mo.f90:645-716 for AA,
+    :818-888 for AB)."""
+    L = np.einsum("tqrs,tp->pqrs", XX, C1[:, :o1], optimize=True)
+    M = np.einsum("ptrs,tq->pqrs", L, C1[:, o1:], optimize=True)
+    N = np.einsum("pqts,tr->pqrs", M, C2[:, :o2], optimize=True)
+    return np.einsum("pqrt,ts->pqrs", N, C2[:, o2:], optimize=True)
+
+
+def uhf_records(XX, Ca, Cb, na, nb):
+    """The record streams of ijab_AA, ijab_BB and ijab_AB
+    (ao2mo.f90:718-724, :804-810, :890-896): lists of (nvrt x nvrt') arrays."""
+    out = {}
+    for key, (C1, o1, C2, o2, tri) in {"AA": (Ca, na, Ca, na, True), "BB": (Cb, nb, Cb, nb, True),
+                                       "AB": (Ca, na, Cb, nb, False)}.items():
+        n = C1.shape[0]
+        if o1 * (n - o1) == 0 or o2 * (n - o2) == 0:
+            # no virtual orbital: the records are still written, with no payload
+            out[key] = [np.zeros((n - o1, n - o2)) for i in range(o1) for j in range(o2) if (j > i or not tri)]
+            continue
+        O = ao2mo_ovov_pair(XX, C1, o1, C2, o2)
+        out[key] = [O[i, :, j, :] for i in range(o1) for j in range(o2) if (j > i or not tri)]
+    return out
+
+
+def _spin_pair(d, name, n):
+    recs = read_unformatted(os.path.join(d, name))
+    return [r.reshape(n, n, order="F").copy() for r in recs[:2]]
+
+
+def uhf_energy_of(d, H, Enuc_, Ca, Cb, n):
+    """E of given UHF orbitals through the reference's own dens and UHFI2G
+    (UHFiter's energy, scf.f90:969-975); d holds envdat (ref=1), XX, basinfo."""
+    write_text_matrices(os.path.join(d, "Cui"), [Ca, Cb])
+    _run(d, "dens")
+    _run(d, "UHFI2G")
+    Da, Db = _spin_pair(d, "Da", n)
+    Ga, Gb = _spin_pair(d, "Guv", n)
+    return 0.5 * float(np.sum(Da * (H + Ga + H)) + np.sum(Db * (H + Gb + H))) + Enuc_
+
+
+def rhf_energy_of(d, H, Enuc_, C, n):
+    """E of given RHF orbitals through the reference's dens and RHFI2G
+    (RHFiter's energy, scf.f90:775-781)."""
+    write_text_matrix(os.path.join(d, "Cui"), C)
+    _run(d, "dens")
+    _run(d, "RHFI2G")
+    D = read_unformatted(os.path.join(d, "Da"))[0].reshape(n, n, order="F")
+    G = read_unformatted(os.path.join(d, "Guv"))[0].reshape(n, n, order="F")
+    return 0.5 * float(np.sum(D * (H + G + H))) + Enuc_
+
+
+def prepare(Z, xyz_bohr, charge=0, mult=1, ref=0, scf_conv=11, mp2=True, workdir=None):
+    """A work directory with the reference's int1e and int2e outputs."""
+    d = workdir or tempfile.mkdtemp(prefix="myqc_ref_", dir=os.path.join(REF))
+    os.makedirs(d, exist_ok=True)
+    write_env(d, Z, xyz_bohr, charge=charge, calc_mp2=mp2, scf_conv=scf_conv, ref=ref, mult=mult)
+    _run(d, "int1e")
+    _run(d, "int2e")
+    n = int(open(os.path.join(d, "basinfo")).read().split()[1])
+    S = read_text_matrix(os.path.join(d, "Suv"), n)
+    H = read_text_matrix(os.path.join(d, "Huv"), n)
+    XX = read_unformatted(os.path.join(d, "XX"))[0].reshape(n, n, n, n, order="F")
+    return d, n, S, H, XX
+
+
+def run_reference_uhf(Z, xyz_bohr, charge=0, mult=2, scf_conv=11, mp2=True, C_start=None, workdir=None,
+                      keep=False, max_iter=300):
+    """The UHF chain: int1e, int2e, the restated UHF loop (src/scf/scf.f90:
+    221-404 UHF, :518-622 initUHF, :909-1117 UHFiter, :1157-1267 checkConv)
+    spawning the reference's dens and UHFI2G where UHFiter does (:1091-1092),
+    then the restated slow_ao2mo_MP2_UHF records and the reference's mp2
+    (mp2_uhf).  DSYGV(1,'V','L') is scipy's LAPACK dsygv as in run_reference.
+    C_start = (Ca, Cb) replaces the core-Hamiltonian guess (a Cui restart)."""
+    if not have_ref() or not os.path.exists(os.path.join(REF, "UHFI2G")):
+        raise RuntimeError("reference programs not built (oracle/build_ref.sh)")
+    na, nb = spin_counts(Z, charge, mult)
+    d, n, S, H, XX = prepare(Z, xyz_bohr, charge, mult, 1, scf_conv, mp2, workdir)
+    try:
+        En = enuc(Z, xyz_bohr)
+        if C_start is None:
+            _, C = _dsygv(H, S)                       # initUHF: both spins from the core Hamiltonian
+            Ca, Cb = C.copy(), C.copy()
+        else:
+            Ca, Cb = (np.array(c, dtype=np.float64) for c in C_start)
+        write_text_matrices(os.path.join(d, "Cui"), [Ca, Cb])
+        _run(d, "dens")
+        _run(d, "UHFI2G")
+        lim = [10.0 ** (-k) for k in range(12)]
+        conv, it, energies = False, 0, []
+        while not conv:
+            Da, Db = _spin_pair(d, "Da", n)
+            Ga, Gb = _spin_pair(d, "Guv", n)
+            Fa, Fb = H + Ga, H + Gb
+            Eelc = 0.0
+            for v in range(n):
+                for u in range(n):
+                    Eelc += Da[u, v] * (Fa[u, v] + H[u, v]) + Db[u, v] * (Fb[u, v] + H[u, v])
+            energies.append(0.5 * Eelc + En)
+            eig_a, Ca = _dsygv(Fa, S)
+            eig_b, Cb = _dsygv(Fb, S)
+            write_text_matrices(os.path.join(d, "Cui"), [Ca, Cb])
+            write_unformatted(os.path.join(d, "Dold"), [Da.ravel(order="F"), Db.ravel(order="F")])
+            _run(d, "dens")
+            _run(d, "UHFI2G")
+            if it > 0:
+                Dan, Dbn = _spin_pair(d, "Da", n)
+                mdiff = max(np.max(np.abs(Da - Dan)), np.max(np.abs(Db - Dbn)))
+                conv = mdiff < lim[scf_conv]
+            it += 1
+            if it >= max_iter:
+                break
+        with open(os.path.join(d, "eig"), "w") as f:
+            for e in (eig_a, eig_b):
+                f.write(" ".join(_fmt(x) for x in e) + "\n")
+        out = dict(n=n, S=S, H=H, XX=XX, Enuc=En, E_scf=energies[-1], energies=np.array(energies), Ca=Ca, Cb=Cb,
+                   eig_a=eig_a, eig_b=eig_b, nalpha=na, nbeta=nb, iterations=it, converged=conv)
+        if mp2:
+            recs = uhf_records(XX, Ca, Cb, na, nb)
+            for key in ("AA", "BB", "AB"):
+                write_unformatted(os.path.join(d, "ijab_" + key), [r.ravel(order="F") for r in recs[key]])
+            txt = _run(d, "mp2")
+            vals = {}
+            for line in txt.splitlines():
+                for key in ("E(AA)", "E(BB)", "E(AB)", "E(MBPT2)"):
+                    if key + " =" in line:
+                        vals[key] = float(line.split("=")[1])
+            out.update(E_aa=vals["E(AA)"], E_bb=vals["E(BB)"], E_ab=vals["E(AB)"], E_mp2=vals["E(MBPT2)"],
+                       records=recs)
+        return out
+    finally:
+        if not keep and workdir is None:
+            shutil.rmtree(d, ignore_errors=True)
+
+
+def mp2_uhf_reference(d, recs, eig_a, eig_b):
+    """The reference's mp2 program (mp2_uhf, src/mp2/mp2.f90:154-232) on given
+    record streams and orbital energies; d holds envdat (ref=1) and basinfo."""
+    for key in ("AA", "BB", "AB"):
+        write_unformatted(os.path.join(d, "ijab_" + key), [np.asarray(r).ravel(order="F") for r in recs[key]])
+    with open(os.path.join(d, "eig"), "w") as f:
+        for e in (eig_a, eig_b):
+            f.write(" ".join(_fmt(x) for x in e) + "\n")
+    txt = _run(d, "mp2")
+    vals = {}
+    for line in txt.splitlines():
+        for key in ("E(AA)", "E(BB)", "E(AB)", "E(MBPT2)"):
+            if key + " =" in line:
+                vals[key] = float(line.split("=")[1])
+    return vals

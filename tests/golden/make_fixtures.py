@@ -11,6 +11,13 @@ orbital energies and MO coefficients (the scf.f90 loop, restated in
 oracle/ref_driver.py because scf needs LAPACK), and E(AA)/E(AB)/E(MBPT2)
 (mp2, on ijab_AB from the restated ao2mo transform).
 
+uhf_<mol>.npz hold the UHF chain (the restated UHF loop over the reference's
+dens/UHFI2G, its mp2 on the restated UHF records) for the reference's example
+inputs NO, OH, O (triplet) and H; uhf_no.npz also the reference's own NO
+outputs (examples/NO: MOLDEN orbital energies, Cui) and the energy of that Cui.
+ex_o1.npz / ex_be.npz add the reference's RHF example outputs (MOLDEN orbital
+energies; for O also its Cui and their energy).
+
 boys.npz holds the reference Boys routine's F_0..F_Q(T) for Q in
 {0,3,6,9,12} (oracle/boys_dump.f90 over the compiled auxilary module).
 """
@@ -67,6 +74,74 @@ def write(name, Z, xyz_ang, charge=0, max_eri=20000, seed=0, mp2=True):
           + (f" E_mp2={r['E_mp2']:.12f}" if mp2 else ""), flush=True)
 
 
+EXAMPLES = "/root/reference/examples"
+
+
+def molden_energies(path):
+    """Orbital energies of a MOLDEN file the reference wrote (its makeMOLDEN,
+    src/scf/scf.f90:1343-1466): (alpha, beta) arrays."""
+    import re
+    txt = open(path).read()
+    out = {"Alpha": [], "Beta": []}
+    for m in re.finditer(r"Ene=\s*(\S+)\s*\n\s*Spin=\s*(\w+)", txt):
+        out[m.group(2)].append(float(m.group(1)))
+    return np.array(out["Alpha"]), np.array(out["Beta"])
+
+
+def write_uhf(name, Z, xyz_ang, charge=0, mult=2, example=None):
+    """UHF chain of the reference (restated loop over its dens/UHFI2G, its mp2)
+    at SCF_Conv 11; with example = a directory of the reference's examples,
+    also its own MOLDEN orbital energies, its Cui and their energy."""
+    from oracle import ref_driver as R
+    xyz = M.to_bohr(Z, xyz_ang)
+    r = R.run_reference_uhf(Z, xyz, charge=charge, mult=mult, scf_conv=11, mp2=True)
+    n = r["n"]
+    d = dict(Z=np.array(Z, dtype=np.int32), xyz=xyz, charge=np.int32(charge), mult=np.int32(mult),
+             n=np.int32(n), nalpha=np.int32(r["nalpha"]), nbeta=np.int32(r["nbeta"]), S=r["S"], H=r["H"],
+             Enuc=np.float64(r["Enuc"]), E_scf=np.float64(r["E_scf"]), eig_a=r["eig_a"], eig_b=r["eig_b"],
+             Ca=r["Ca"], Cb=r["Cb"], iterations=np.int32(r["iterations"]), converged=np.bool_(r["converged"]),
+             E_aa=np.float64(r["E_aa"]), E_bb=np.float64(r["E_bb"]), E_ab=np.float64(r["E_ab"]),
+             E_mp2=np.float64(r["E_mp2"]))
+    if n <= 15:
+        d["XX_full"] = r["XX"]
+    if example:
+        ea, eb = molden_energies(os.path.join(EXAMPLES, example, "MOLDEN"))
+        Ca, Cb = R.read_text_matrices(os.path.join(EXAMPLES, example, "Cui"), n, 2)
+        wd, n2, S, H, XX = R.prepare(Z, xyz, charge, mult, 1, 11, False)
+        try:
+            E_ex = R.uhf_energy_of(wd, H, R.enuc(Z, xyz), Ca, Cb, n)
+        finally:
+            subprocess.run(["rm", "-rf", wd], check=True)
+        d.update(example_eig_a=ea, example_eig_b=eb, example_Ca=Ca, example_Cb=Cb, example_E=np.float64(E_ex))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+    print(f"{name}: n={n} na={r['nalpha']} nb={r['nbeta']} E_scf={r['E_scf']:.12f} iters={r['iterations']} "
+          f"conv={r['converged']} E_mp2={r['E_mp2']:.12f}" + (f" E(example Cui)={d['example_E']:.12f}" if example else ""),
+          flush=True)
+
+
+def write_rhf_example(name, Z, xyz_ang, example, with_cui):
+    """RHF example outputs of the reference: MOLDEN orbital energies (and, with
+    with_cui, its Cui and the energy of those orbitals through its dens and
+    RHFI2G), beside the reference chain's own converged values."""
+    from oracle import ref_driver as R
+    write(name, Z, xyz_ang, mp2=False)
+    g = dict(np.load(os.path.join(OUT, name + ".npz")))
+    ea, _ = molden_energies(os.path.join(EXAMPLES, example, "MOLDEN"))
+    g["example_eig"] = ea
+    if with_cui:
+        n = int(g["n"])
+        xyz = M.to_bohr(Z, xyz_ang)
+        C = R.read_text_matrices(os.path.join(EXAMPLES, example, "Cui"), n, 1)[0]
+        wd, _, S, H, XX = R.prepare(Z, xyz, 0, 1, 0, 11, False)
+        try:
+            g["example_E"] = np.float64(R.rhf_energy_of(wd, H, R.enuc(Z, xyz), C, n))
+        finally:
+            subprocess.run(["rm", "-rf", wd], check=True)
+        g["example_C"] = C
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **g)
+    print(f"{name}: example eig {ea[:3]}" + (f" E(example Cui)={g['example_E']:.12f}" if with_cui else ""))
+
+
 def boys():
     ftab = os.path.join(ROOT, "myqc_amd", "data", "Ftab")
     subprocess.run(["cp", ftab, os.path.join(REF, "Ftab")], check=True)
@@ -93,6 +168,13 @@ def main():
     write("h2o", *M.water())
     write("h2o_2", *M.water_cluster(2, seed=2, box=4.0))
     write("h2o_4", *M.water_cluster(4, seed=4, box=5.5))
+    # the reference's own examples (src inputs under examples/)
+    write_uhf("uhf_no", [7, 8], np.array([[0, 0, 0], [0, 0, 1.160871722356318]]), mult=2, example="NO")
+    write_uhf("uhf_oh", [8, 1], np.array([[0, 0, 0], [0, 0, 1.0]]), mult=2)
+    write_uhf("uhf_o3", [8], np.array([[0.0, 0.0, 0.0]]), mult=3)
+    write_uhf("uhf_h", [1], np.array([[0.0, 0.0, 0.0]]), mult=2)
+    write_rhf_example("ex_o1", [8], np.array([[0.0, 0.0, 0.0]]), "O/singlet", True)
+    write_rhf_example("ex_be", [4], np.array([[0.0, 0.0, 0.0]]), "Be", False)
     rng = np.random.default_rng(4096)
     k = 0
     while k < 6:

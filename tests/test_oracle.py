@@ -199,3 +199,84 @@ def test_mlp_reference_gradient_matches_finite_differences():
         fd[k] = (mlp.loss_grad(p + e, X, T, Mk, n_in, width, n_out)[0]
                  - mlp.loss_grad(p - e, X, T, Mk, n_in, width, n_out)[0]) / (2 * h)
     assert np.max(np.abs(fd - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
+
+
+def test_fixtures_reproduce_reference_example_outputs():
+    """The oracle chains against the outputs the reference itself ships
+    (examples/NO: UHF MOLDEN orbital energies and Cui; examples/O/singlet and
+    examples/Be: RHF).  The energy of the shipped Cui goes through the
+    reference's own dens + UHFI2G/RHFI2G; the example runs stopped at
+    SCF_Conv 7 (density 1e-7), the fixtures at 1e-11."""
+    g = golden("uhf_no")
+    assert abs(float(g["E_scf"]) - float(g["example_E"])) <= 1e-9
+    assert np.max(np.abs(g["eig_a"] - g["example_eig_a"])) <= 5e-7
+    assert np.max(np.abs(g["eig_b"] - g["example_eig_b"])) <= 5e-7
+    g = golden("ex_o1")
+    assert abs(float(g["E_scf"]) - float(g["example_E"])) <= 1e-9
+    assert np.max(np.abs(g["eig"] - g["example_eig"])) <= 1e-8   # MOLDEN prints 8 decimals
+    g = golden("ex_be")
+    assert np.max(np.abs(g["eig"] - g["example_eig"])) <= 1e-8
+
+
+def _uhf_fock(g):
+    n, na, nb = int(g["n"]), int(g["nalpha"]), int(g["nbeta"])
+    X = g["XX_full"]
+    Ca, Cb = g["Ca"], g["Cb"]
+    Da, Db = Ca[:, :na] @ Ca[:, :na].T, Cb[:, :nb] @ Cb[:, :nb].T
+    J = np.einsum("ijkl,kl->ij", X, Da + Db)
+    Ka, Kb = np.einsum("ikjl,kl->ij", X, Da), np.einsum("ikjl,kl->ij", X, Db)
+    return n, na, nb, Da, Db, g["H"] + J - Ka, g["H"] + J - Kb
+
+
+@pytest.mark.parametrize("name", ["uhf_no", "uhf_oh", "uhf_o3", "uhf_h"])
+def test_uhf_fixture_is_a_stationary_point(name):
+    """The UHF fixtures (the reference's dens/UHFI2G chain) satisfy the UHF
+    equations built independently from their dense XX: F_s = H + J[Da+Db] -
+    K[D_s] commutes with D_s S, Tr(D_s S) = nocc_s, and
+    E = (Da.(Fa+H) + Db.(Fb+H))/2 + E_nuc.  The one-electron H atom has
+    E = H_00 / S_00 (its contracted 1s is not renormalised, so S_00 != 1)."""
+    g = golden(name)
+    n, na, nb, Da, Db, Fa, Fb = _uhf_fock(g)
+    S = g["S"]
+    for D, F, k in ((Da, Fa, na), (Db, Fb, nb)):
+        assert abs(np.trace(D @ S) - k) <= 1e-9
+        assert np.max(np.abs(F @ D @ S - S @ D @ F)) <= 1e-6
+    E = 0.5 * (np.sum(Da * (Fa + g["H"])) + np.sum(Db * (Fb + g["H"]))) + float(g["Enuc"])
+    assert abs(E - float(g["E_scf"])) <= 1e-9
+    if name == "uhf_h":
+        assert abs(float(g["E_scf"]) - float(g["H"][0, 0] / S[0, 0])) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["uhf_no", "uhf_oh", "uhf_o3"])
+def test_uhf_mp2_fixture_against_spin_orbital_formula(name):
+    """The reference's mp2_uhf values in the fixtures, against the MP2 energy
+    written in spin orbitals, sum_{i<j,a<b} |<ij||ab>|^2 / D over the
+    combined alpha+beta orbital set, from the dense XX (a different route:
+    one antisymmetrised sum instead of three spin blocks)."""
+    g = golden(name)
+    n, na, nb = int(g["n"]), int(g["nalpha"]), int(g["nbeta"])
+    X = g["XX_full"]
+    # spin orbitals: occupied (alpha then beta), virtual likewise
+    Cs = {"a": g["Ca"], "b": g["Cb"]}
+    es = {"a": g["eig_a"], "b": g["eig_b"]}
+    occ = [("a", i) for i in range(na)] + [("b", i) for i in range(nb)]
+    vir = [("a", a) for a in range(na, n)] + [("b", b) for b in range(nb, n)]
+
+    def mo(p, q, r, s):   # (pq|rs) in chemists' notation, zero unless spins pair up
+        if p[0] != q[0] or r[0] != s[0]:
+            return 0.0
+        return np.einsum("uvlk,u,v,l,k->", X, Cs[p[0]][:, p[1]], Cs[q[0]][:, q[1]], Cs[r[0]][:, r[1]],
+                         Cs[s[0]][:, s[1]])
+
+    E = 0.0
+    for x in range(len(occ)):
+        for y in range(x + 1, len(occ)):
+            for u in range(len(vir)):
+                for v in range(u + 1, len(vir)):
+                    i, j, a, b = occ[x], occ[y], vir[u], vir[v]
+                    t = mo(i, a, j, b) - mo(i, b, j, a)
+                    if t == 0.0:
+                        continue
+                    E += t * t / (es[i[0]][i[1]] + es[j[0]][j[1]] - es[a[0]][a[1]] - es[b[0]][b[1]])
+    assert abs(E - float(g["E_mp2"])) <= 1e-10
+    assert abs(float(g["E_aa"]) + float(g["E_bb"]) + float(g["E_ab"]) - float(g["E_mp2"])) <= 1e-14
