@@ -123,6 +123,30 @@ int myqc_mp2_uhf_records(myqc_ctx *ctx, const double *ijab_aa, const double *ija
                          const double *eig_a, const double *eig_b, int nalpha, int nbeta, int nvrt_a, int nvrt_b,
                          double *e_aa, double *e_bb, double *e_ab, double *e_tot);
 
+/* slow_ao2mo_CIS_UHF drop-in (src/ao2mo/ao2mo.f90:919-1227): the five record
+ * streams of the reference's CIS files, each record (j,b) j-major with b
+ * fastest, each record over (i,a) with a fastest (upS = nocc_S nvrt_S):
+ *   ajib_aa (upA x upA): (ai|jb), alpha;     ajbi_aa (upA x upA): (ab|ji), alpha;
+ *   ajib_ab (upB records of upA): (a_A i_A|j_B b_B);
+ *   ajib_bb, ajbi_bb (upB x upB): beta.
+ * Any output may be NULL; all sets stay on the device for myqc_cis_uhf.  FP64. */
+int myqc_ao2mo_cis_uhf(myqc_ctx *ctx, const double *Ca, const double *Cb, int nalpha, int nbeta, double *ajib_aa,
+                       double *ajbi_aa, double *ajib_ab, double *ajib_bb, double *ajbi_bb);
+
+/* cis_scf_uhf drop-in (src/CIS/cis.f90:87-223): the UHF-CIS matrix
+ *   A(ia, jb) = (ai|jb) - (ab|ji) + delta_ij delta_ab (e_a - e_i) within a spin,
+ *   A(ia_A, jb_B) = (ai|jb)_AB,
+ * of dimension upA + upB, diagonalised whole; the lowest nroots eigenvalues
+ * (excitation energies, ascending) into energies and their eigenvectors
+ * (column-major, (upA + upB) x nroots; alpha (i,a) with a fastest, then beta)
+ * into vectors (either may be NULL; nroots is clamped to upA + upB, as
+ * options(15) is at cis.f90:212-213).  The five record streams are those of
+ * myqc_ao2mo_cis_uhf; all NULL takes the sets it left on the device.
+ * eig_a, eig_b: the n orbital energies of each spin. */
+int myqc_cis_uhf(myqc_ctx *ctx, const double *ajib_aa, const double *ajbi_aa, const double *ajib_ab,
+                 const double *ajib_bb, const double *ajbi_bb, const double *eig_a, const double *eig_b, int nalpha,
+                 int nbeta, int nvrt_a, int nvrt_b, int nroots, double *energies, double *vectors);
+
 /* mp2_rhf drop-in for a separate mp2 program: the same sums over (ia|jb)
  * records read from an ijab_AB file (record (i,j) i-major, nvrt x nvrt with a
  * fastest; src/mp2/mp2.f90:99-135), uploaded to the device. */

@@ -16,7 +16,8 @@
 namespace myqc {
 int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A, int lda, long long strideA,
                 const double *B, int ldb, long long strideB, double *C, int ldc, long long strideC, int batch);
-int ao2mo12_bf16x3(myqc_ctx *ctx, int o, int n, const double *C, const double *XX, double *M);
+int ao2mo12_bf16x3(myqc_ctx *ctx, int np, const double *Cp, int nq, const double *Cq, int n, const double *XX,
+                   double *M);
 }
 
 namespace myqc {
@@ -112,50 +113,50 @@ static int dgemm(myqc_ctx *ctx, rocblas_operation ta, int m, int n, int k, const
     return 0;
 }
 
-// Quarter steps 1 and 2 (idx1_trans, idx2_trans):
-//   M(i,a,l,d) = sum_v [sum_u C(u,i) XX(u,v,l,d)] C(v,o+a)      (o x nv x n x n)
+// Quarter steps 1 and 2 (idx1_trans, idx2_trans) over two column blocks of
+// orbitals, Cp (n x np) and Cq (n x nq), leading dimension n:
+//   M(p,q,l,d) = sum_v [sum_u Cp(u,p) XX(u,v,l,d)] Cq(v,q)      (np x nq x n x n)
 // With release_xx, *dXX is freed once nothing reads it any more.
-static int quarter12(myqc_ctx *ctx, double **dXX, bool release_xx, const double *dC, int o, int precision,
-                     double **dM) {
-    const int n = ctx->n, nv = n - o;
-    const size_t N = n, msize = sizeof(double) * (size_t)o * nv * N * N;
+static int quarter12(myqc_ctx *ctx, double **dXX, bool release_xx, const double *Cp, int np, const double *Cq, int nq,
+                     int precision, double **dM) {
+    const int n = ctx->n;
+    const size_t N = n, msize = sizeof(double) * (size_t)np * nq * N * N;
     if (precision == MYQC_BF16X3) {
         // fused per (l,d) block: L never leaves the workgroup
         MYQC_CHECK(dalloc(ctx, (void **)dM, msize));
-        MYQC_CHECK(ao2mo12_bf16x3(ctx, o, n, dC, *dXX, *dM));
+        MYQC_CHECK(ao2mo12_bf16x3(ctx, np, Cp, nq, Cq, n, *dXX, *dM));
         if (release_xx) dfree((void **)dXX);   // (hipFree waits for the kernel)
         return 0;
     }
-    // 1: L(i,v,l,d) = sum_u C(u,i) XX(u,v,l,d)
+    // 1: L(p,v,l,d) = sum_u Cp(u,p) XX(u,v,l,d)
     double *dL = nullptr;
-    MYQC_CHECK(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)o * N * N * N));
-    int rc = dgemm(ctx, rocblas_operation_transpose, o, n * n * n, n, dC, n, 0, *dXX, n, 0, dL, o, 0, 1, precision);
+    MYQC_CHECK(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)np * N * N * N));
+    int rc = dgemm(ctx, rocblas_operation_transpose, np, n * n * n, n, Cp, n, 0, *dXX, n, 0, dL, np, 0, 1, precision);
     if (rc == 0 && release_xx) dfree((void **)dXX);
-    // 2: M(i,a,l,d) = sum_v L(i,v,l,d) C(v,o+a)
+    // 2: M(p,q,l,d) = sum_v L(p,v,l,d) Cq(v,q)
     if (rc == 0) rc = dalloc(ctx, (void **)dM, msize);
     if (rc == 0)
-        rc = dgemm(ctx, rocblas_operation_none, o, nv, n, dL, o, (long long)o * n, dC + N * o, n, 0, *dM, o,
-                   (long long)o * nv, n * n, precision);
+        rc = dgemm(ctx, rocblas_operation_none, np, nq, n, dL, np, (long long)np * n, Cq, n, 0, *dM, np,
+                   (long long)np * nq, n * n, precision);
     dfree((void **)&dL);
     return rc;
 }
 
 // Quarter steps 3 and 4 (idx3_trans, idx4_trans) with the second pair's
-// orbitals C2 (o2 occupied):
-//   O(i,a,j,b) = sum_d [sum_l M(i,a,l,d) C2(l,j)] C2(d,o2+b)   (o1 x nv1 x o2 x nv2, i fastest)
-static int quarter34(myqc_ctx *ctx, const double *dM, int o1, int nv1, const double *dC2, int o2, int precision,
-                     double *dO) {
-    const int n = ctx->n, nv2 = n - o2, m1 = o1 * nv1;
+// orbital blocks Cr (n x nr) and Cs (n x ns):
+//   O(pq,r,s) = sum_d [sum_l M(pq,l,d) Cr(l,r)] Cs(d,s)     (m1 x nr x ns, pq fastest)
+static int quarter34(myqc_ctx *ctx, const double *dM, int m1, const double *Cr, int nr, const double *Cs, int ns,
+                     int precision, double *dO) {
+    const int n = ctx->n;
     const size_t N = n;
     double *dN = nullptr;
-    MYQC_CHECK(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)m1 * o2 * N));
-    // 3: N(i,a,j,d) = sum_l M(i,a,l,d) C2(l,j)
-    int rc = dgemm(ctx, rocblas_operation_none, m1, o2, n, dM, m1, (long long)m1 * n, dC2, n, 0, dN, m1,
-                   (long long)m1 * o2, n, precision);
-    // 4: O(i,a,j,b) = sum_d N(i,a,j,d) C2(d,o2+b)
+    MYQC_CHECK(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)m1 * nr * N));
+    // 3: N(pq,r,d) = sum_l M(pq,l,d) Cr(l,r)
+    int rc = dgemm(ctx, rocblas_operation_none, m1, nr, n, dM, m1, (long long)m1 * n, Cr, n, 0, dN, m1,
+                   (long long)m1 * nr, n, precision);
+    // 4: O(pq,r,s) = sum_d N(pq,r,d) Cs(d,s)
     if (rc == 0)
-        rc = dgemm(ctx, rocblas_operation_none, m1 * o2, nv2, n, dN, m1 * o2, 0, dC2 + N * o2, n, 0, dO, m1 * o2, 0, 1,
-                   precision);
+        rc = dgemm(ctx, rocblas_operation_none, m1 * nr, ns, n, dN, m1 * nr, 0, Cs, n, 0, dO, m1 * nr, 0, 1, precision);
     dfree((void **)&dN);
     return rc;
 }
@@ -204,13 +205,13 @@ static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
     TRY(dense_integrals(ctx, &dXX));
     hipEvent_t ev;
     TRY(timer_begin(ctx, T_AO2MO, &ev));
-    TRY(quarter12(ctx, &dXX, true, dC, o, precision, &dM));
+    TRY(quarter12(ctx, &dXX, true, dC, o, dC + (size_t)n * o, nv, precision, &dM));
     const size_t oe = (size_t)o * nv * o * nv;
     if (ctx->iajb_elems != oe) {
         TRY(dalloc(ctx, (void **)&ctx->d_iajb, sizeof(double) * (oe ? oe : 1)));
         ctx->iajb_elems = oe;
     }
-    TRY(quarter34(ctx, dM, o, nv, dC, o, precision, ctx->d_iajb));
+    TRY(quarter34(ctx, dM, o * nv, dC, o, dC + (size_t)n * o, nv, precision, ctx->d_iajb));
     TRY(timer_end(ctx, T_AO2MO, ev, transform_flops(o, nv, o, nv, n)));
     ctx->iajb_nocc = o;
     ctx->iajb_nv = nv;
@@ -221,11 +222,14 @@ static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
 
 // UHF: the three spin sets of slow_ao2mo_MP2_UHF (src/ao2mo/ao2mo.f90:614-904)
 // into ctx->d_uhf[0..2] = AA (Ca, Ca), BB (Cb, Cb), AB (i, a alpha; j, b beta).
+// With vvoo, also the same-spin (ab|ij) of slow_ao2mo_CIS_UHF (:919-1227)
+// into ctx->d_vvoo[0..1] (alpha, beta), [a][b][i][j] with a fastest, from the
+// same dense integrals.
 // The reference recomputes the first two quarter steps for each set; here
 // M_A = steps 1-2 with Ca serves both AA and AB, so the dense integrals are
 // contracted twice (Ca, Cb) instead of three times.  A set with no occupied
 // or no virtual orbital is empty.
-static int ao2mo_uhf(myqc_ctx *ctx, const double *Ca_host, const double *Cb_host, int oa, int ob, int precision) {
+int ao2mo_uhf(myqc_ctx *ctx, const double *Ca_host, const double *Cb_host, int oa, int ob, int precision, bool vvoo) {
     const int n = ctx->n, va = n - oa, vb = n - ob;
     double *dXX = nullptr, *dCa = nullptr, *dCb = nullptr, *dMa = nullptr, *dMb = nullptr;
     int rc = 0;
@@ -236,7 +240,7 @@ static int ao2mo_uhf(myqc_ctx *ctx, const double *Ca_host, const double *Cb_host
         dfree((void **)&dMa);
         dfree((void **)&dMb);
     };
-    ctx->have_uhf = false;
+    ctx->have_uhf = ctx->have_vvoo = false;
     const int dims[3][4] = {{oa, va, oa, va}, {ob, vb, ob, vb}, {oa, va, ob, vb}};
     for (int s = 0; s < 3; ++s) {
         const size_t e = (size_t)dims[s][0] * dims[s][1] * dims[s][2] * dims[s][3];
@@ -250,19 +254,37 @@ static int ao2mo_uhf(myqc_ctx *ctx, const double *Ca_host, const double *Cb_host
     hipEvent_t ev;
     TRY(timer_begin(ctx, T_AO2MO, &ev));
     double flops = 0.0;
-    if (want_a) TRY(quarter12(ctx, &dXX, !want_b, dCa, oa, precision, &dMa));
-    if (want_b) TRY(quarter12(ctx, &dXX, true, dCb, ob, precision, &dMb));
+    if (want_a) TRY(quarter12(ctx, &dXX, !want_b && !vvoo, dCa, oa, dCa + (size_t)n * oa, va, precision, &dMa));
+    if (want_b) TRY(quarter12(ctx, &dXX, !vvoo, dCb, ob, dCb + (size_t)n * ob, vb, precision, &dMb));
+    if (vvoo) {
+        const double *Cs[2] = {dCa, dCb};
+        const int os[2] = {oa, ob}, vs[2] = {va, vb};
+        for (int s = 0; s < 2; ++s) {
+            const size_t e = (size_t)vs[s] * vs[s] * os[s] * os[s];
+            TRY(dalloc(ctx, (void **)&ctx->d_vvoo[s], sizeof(double) * (e ? e : 1)));
+            ctx->vvoo_dims[s][0] = vs[s];
+            ctx->vvoo_dims[s][1] = os[s];
+            if (e == 0) continue;
+            double *dV = nullptr;
+            const double *Cv = Cs[s] + (size_t)n * os[s];
+            rc = quarter12(ctx, &dXX, false, Cv, vs[s], Cv, vs[s], precision, &dV);
+            if (rc == 0) rc = quarter34(ctx, dV, vs[s] * vs[s], Cs[s], os[s], Cs[s], os[s], precision, ctx->d_vvoo[s]);
+            dfree((void **)&dV);
+            TRY(rc);
+            flops += transform_flops(vs[s], vs[s], os[s], os[s], n);
+        }
+    }
     dfree((void **)&dXX);
     if (want_a) {
-        TRY(quarter34(ctx, dMa, oa, va, dCa, oa, precision, ctx->d_uhf[0]));
+        TRY(quarter34(ctx, dMa, oa * va, dCa, oa, dCa + (size_t)n * oa, va, precision, ctx->d_uhf[0]));
         flops += transform_flops(oa, va, oa, va, n);
     }
     if (want_b) {
-        TRY(quarter34(ctx, dMb, ob, vb, dCb, ob, precision, ctx->d_uhf[1]));
+        TRY(quarter34(ctx, dMb, ob * vb, dCb, ob, dCb + (size_t)n * ob, vb, precision, ctx->d_uhf[1]));
         flops += transform_flops(ob, vb, ob, vb, n);
     }
     if (want_a && want_b) {
-        TRY(quarter34(ctx, dMa, oa, va, dCb, ob, precision, ctx->d_uhf[2]));
+        TRY(quarter34(ctx, dMa, oa * va, dCb, ob, dCb + (size_t)n * ob, vb, precision, ctx->d_uhf[2]));
         // steps 3-4 only: steps 1-2 are shared with the AA set
         flops += 2.0 * ((double)oa * va * ob * n * n + (double)oa * va * ob * vb * n);
     }
@@ -270,6 +292,7 @@ static int ao2mo_uhf(myqc_ctx *ctx, const double *Ca_host, const double *Cb_host
     cleanup();
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->have_uhf = true;
+    ctx->have_vvoo = vvoo;
 #undef TRY
     return 0;
 }
@@ -429,7 +452,7 @@ extern "C" int myqc_ao2mo_ovov_uhf(myqc_ctx *ctx, const double *Ca, const double
     const size_t n = ctx->n;
     if (n * n * n * n * 8 > (size_t)200 << 30)
         return set_error(ctx, MYQC_E_ARG, "myqc_ao2mo_ovov_uhf: n=%zu needs more than 200 GB of dense AO integrals", n);
-    MYQC_CHECK(ao2mo_uhf(ctx, Ca, Cb, nalpha, nbeta, precision));
+    MYQC_CHECK(ao2mo_uhf(ctx, Ca, Cb, nalpha, nbeta, precision, false));
     if (ijab_aa) MYQC_CHECK(download_records(ctx, ctx->d_uhf[0], ctx->uhf_dims[0], 1, ijab_aa));
     if (ijab_bb) MYQC_CHECK(download_records(ctx, ctx->d_uhf[1], ctx->uhf_dims[1], 1, ijab_bb));
     if (ijab_ab) MYQC_CHECK(download_records(ctx, ctx->d_uhf[2], ctx->uhf_dims[2], 0, ijab_ab));

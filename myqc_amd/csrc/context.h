@@ -122,6 +122,11 @@ struct myqc_ctx {
     double *d_uhf[3] = {nullptr, nullptr, nullptr};
     int uhf_dims[3][4] = {};
     bool have_uhf = false;
+    // same-spin (ab|ij) for UHF-CIS (alpha, beta), [a][b][i][j] with a
+    // fastest, and their (nvrt, nocc)
+    double *d_vvoo[2] = {nullptr, nullptr};
+    int vvoo_dims[2][2] = {};
+    bool have_vvoo = false;
 
     // last RHF run: purification steps and diagonalisation fallbacks
     myqc::ScfBuffers *scf = nullptr;       // RHF (and UHF alpha) work arrays, kept across calls

@@ -238,16 +238,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_nn(int m, int n, int k, int
             }
 }
 
-// Quarter steps 1 and 2 of the RHF transform fused, for one (l,d) block per
+// Quarter steps 1 and 2 of the transform fused, for one (l,d) block per
 // workgroup (blockIdx.x = l + n d) and a 128-row tile of i (blockIdx.y) and a
-// 64-column tile of a (blockIdx.z):
-//   L(i,v) = sum_u C(u,i) XX(u,v,l,d),   M(i,a,l,d) = sum_v L(i,v) C(v,o+a)
+// 64-column tile of a (blockIdx.z), with i over the first orbital block Cp and
+// a over the second, Cq (o and nv below are their widths):
+//   L(i,v) = sum_u Cp(u,i) XX(u,v,l,d),   M(i,a,l,d) = sum_v L(i,v) Cq(v,a)
 // L is formed 32 columns of v at a time in LDS and never leaves the
 // workgroup.  Each step keeps the split arithmetic of k_gemm_bf16x3_tn and
 // k_gemm_bf16x3_nn exactly: L is rounded as the FP64 value acc + cor those
 // kernels would store, then split again, and both k loops run in the same
-// 32-wide order, so M is bitwise the unfused result.  Co: planes of C_occ^T
-// (k_split_rows_t, rows i, kp), Cv: planes of C_virt^T (rows a, kp).
+// 32-wide order, so M is bitwise the unfused result.  Co: planes of Cp^T
+// (k_split_rows_t, rows i, kp), Cv: planes of Cq^T (rows a, kp).
 __global__ __launch_bounds__(256) void k_ao2mo12_bf16x3(int o, int nv, int n, int kp, const uint16_t *__restrict__ Coh,
                                                         const uint16_t *__restrict__ Col,
                                                         const uint16_t *__restrict__ Cvh,
@@ -375,22 +376,24 @@ __global__ __launch_bounds__(256) void k_ao2mo12_bf16x3(int o, int nv, int n, in
             }
 }
 
-// Quarter steps 1 and 2 of the transform in split-bf16: M (o x nv x n x n)
-// from C (n x n) and the dense integrals XX (n^4).
-int ao2mo12_bf16x3(myqc_ctx *ctx, int o, int n, const double *C, const double *XX, double *M) {
-    const int nv = n - o;
-    if (o <= 0 || nv <= 0) return set_error(ctx, MYQC_E_ARG, "ao2mo12_bf16x3: o=%d nv=%d", o, nv);
-    const int gy = (o + GM - 1) / GM, gz = (nv + 63) / 64;
+// Quarter steps 1 and 2 of the transform in split-bf16 over two column
+// blocks of orbitals: M (np x nq x n x n), M(p,q,l,d) = sum_uv Cp(u,p) Cq(v,q)
+// XX(u,v,l,d), from Cp (n x np) and Cq (n x nq), both with leading dimension n,
+// and the dense integrals XX (n^4).
+int ao2mo12_bf16x3(myqc_ctx *ctx, int np, const double *Cp, int nq, const double *Cq, int n, const double *XX,
+                   double *M) {
+    if (np <= 0 || nq <= 0) return set_error(ctx, MYQC_E_ARG, "ao2mo12_bf16x3: np=%d nq=%d", np, nq);
+    const int gy = (np + GM - 1) / GM, gz = (nq + 63) / 64;
     const long long nld = (long long)n * n;
     if (nld > 0x7fffffffLL || gy > 65535 || gz > 65535) return set_error(ctx, MYQC_E_ARG, "ao2mo12_bf16x3: too large");
     const int kp = ((n + GK - 1) / GK) * GK;
     const size_t po = (size_t)gy * GM * kp, pv = (size_t)gz * 64 * kp;
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_planes, sizeof(uint16_t) * 2 * (po + pv)));
     uint16_t *coh = ctx->d_planes, *col = coh + po, *cvh = col + po, *cvl = cvh + pv;
-    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(C, n, o, n, gy * GM, kp, coh, col);
-    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(C + (size_t)n * o, n, nv, n, gz * 64, kp, cvh, cvl);
-    k_ao2mo12_bf16x3<<<dim3((unsigned)nld, (unsigned)gy, (unsigned)gz), 256, 0, ctx->stream>>>(o, nv, n, kp, coh, col, cvh,
-                                                                                             cvl, XX, M);
+    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(Cp, n, np, n, gy * GM, kp, coh, col);
+    k_split_rows_t<<<256, 256, 0, ctx->stream>>>(Cq, n, nq, n, gz * 64, kp, cvh, cvl);
+    k_ao2mo12_bf16x3<<<dim3((unsigned)nld, (unsigned)gy, (unsigned)gz), 256, 0, ctx->stream>>>(np, nq, n, kp, coh, col,
+                                                                                             cvh, cvl, XX, M);
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
 }

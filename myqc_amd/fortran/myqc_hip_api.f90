@@ -140,6 +140,23 @@ MODULE myqc_hip_api
       INTEGER(c_int), VALUE :: nalpha, nbeta, nvrt_a, nvrt_b
       REAL(c_double) :: e_aa, e_bb, e_ab, e_tot
     END FUNCTION myqc_mp2_uhf_records
+
+    INTEGER(c_int) FUNCTION myqc_ao2mo_cis_uhf(ctx, Ca, Cb, nalpha, nbeta, ajib_aa, ajbi_aa, ajib_ab, ajib_bb, &
+                                               ajbi_bb) BIND(C, name='myqc_ao2mo_cis_uhf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: Ca(*), Cb(*), ajib_aa(*), ajbi_aa(*), ajib_ab(*), ajib_bb(*), ajbi_bb(*)
+      INTEGER(c_int), VALUE :: nalpha, nbeta
+    END FUNCTION myqc_ao2mo_cis_uhf
+
+    INTEGER(c_int) FUNCTION myqc_cis_uhf(ctx, ajib_aa, ajbi_aa, ajib_ab, ajib_bb, ajbi_bb, eig_a, eig_b, nalpha, &
+                                         nbeta, nvrt_a, nvrt_b, nroots, energies, vectors) BIND(C, name='myqc_cis_uhf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: ajib_aa(*), ajbi_aa(*), ajib_ab(*), ajib_bb(*), ajbi_bb(*), eig_a(*), eig_b(*)
+      INTEGER(c_int), VALUE :: nalpha, nbeta, nvrt_a, nvrt_b, nroots
+      REAL(c_double) :: energies(*), vectors(*)
+    END FUNCTION myqc_cis_uhf
   END INTERFACE
 
 CONTAINS

@@ -45,6 +45,9 @@ EXPORTS = {
                       ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
     "myqc_ao2mo_ovov_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
     "myqc_mp2_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "myqc_ao2mo_cis_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "myqc_cis_uhf": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int, _dp, _dp], ctypes.c_int),
     "myqc_mp2_uhf_records": ([_vp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               _dp, _dp, _dp, _dp], ctypes.c_int),
     "myqc_rhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
@@ -325,6 +328,45 @@ class Context:
         self._chk(self.L.myqc_mp2_uhf_records(self.h, _p(rr[0]), _p(rr[1]), _p(rr[2]), _p(ea), _p(eb), nalpha,
                                               nbeta, n - nalpha, n - nbeta, *[ctypes.byref(x) for x in r]))
         return tuple(x.value for x in r)
+
+    CIS_STREAMS = ("ajib_AA", "ajbi_AA", "ajib_AB", "ajib_BB", "ajbi_BB")
+
+    def ao2mo_cis_uhf(self, Ca, Cb, nalpha: int, nbeta: int, download: bool = True):
+        """slow_ao2mo_CIS_UHF: the five record streams, each shaped
+        (records, elements) = (upS', upS) (or None if not downloaded)."""
+        n = self.n
+        upA, upB = nalpha * (n - nalpha), nbeta * (n - nbeta)
+        shp = {"ajib_AA": (upA, upA), "ajbi_AA": (upA, upA), "ajib_AB": (upB, upA), "ajib_BB": (upB, upB),
+               "ajbi_BB": (upB, upB)}
+        if not download:
+            self._chk(self.L.myqc_ao2mo_cis_uhf(self.h, _p(_f(Ca)), _p(_f(Cb)), nalpha, nbeta, None, None, None, None,
+                                                None))
+            return None
+        out = {k: np.zeros(max(v[0] * v[1], 1)) for k, v in shp.items()}
+        self._chk(self.L.myqc_ao2mo_cis_uhf(self.h, _p(_f(Ca)), _p(_f(Cb)), nalpha, nbeta,
+                                            *[_p(out[k]) for k in self.CIS_STREAMS]))
+        return {k: out[k][:shp[k][0] * shp[k][1]].reshape(shp[k]) for k in shp}
+
+    def cis_uhf(self, eig_a, eig_b, nalpha: int, nbeta: int, nroots: int, streams=None):
+        """cis_scf_uhf: the lowest nroots excitation energies and eigenvectors
+        ((upA + upB) x nroots), from the given record streams or the sets of
+        the last ao2mo_cis_uhf."""
+        n = len(eig_a)
+        dim = nalpha * (n - nalpha) + nbeta * (n - nbeta)
+        k = min(nroots, dim)
+        E = np.zeros(max(k, 1))
+        V = np.zeros((dim, max(k, 1)), order="F")
+        ea = np.ascontiguousarray(eig_a, dtype=np.float64)
+        eb = np.ascontiguousarray(eig_b, dtype=np.float64)
+        if streams is None:
+            ptrs = [None] * 5
+        else:
+            arrs = [np.ascontiguousarray(streams[s], dtype=np.float64).ravel() for s in self.CIS_STREAMS]
+            arrs = [a if a.size else np.zeros(1) for a in arrs]
+            ptrs = [_p(a) for a in arrs]
+        self._chk(self.L.myqc_cis_uhf(self.h, *ptrs, _p(ea), _p(eb), nalpha, nbeta, n - nalpha, n - nbeta, nroots,
+                                      _p(E), _p(V)))
+        return E[:k], V[:, :k]
 
     # -- batched small molecules -------------------------------------------
     def batch_set(self, molecules, ftab=None):

@@ -395,3 +395,50 @@ def mp2_uhf_reference(d, recs, eig_a, eig_b):
             if key + " =" in line:
                 vals[key] = float(line.split("=")[1])
     return vals
+
+
+def cis_uhf_streams(XX, Ca, Cb, na, nb):
+    """The five record streams of slow_ao2mo_CIS_UHF (ao2mo.f90:919-1227) by
+    its quarter steps in order: ajib = Om(a,i,j,b) with (a,i) from the first
+    spin's (virtual, occupied) and (j,b) from the second's (occupied, virtual);
+    ajbi = Om(a,b,j,i) of one spin.  Each stream is (records, elements):
+    record (j,b) j-major, elements (i,a) with a fastest."""
+    def quarter(C1, C2, C3, C4):
+        L = np.einsum("tqrs,tp->pqrs", XX, C1, optimize=True)
+        M = np.einsum("ptrs,tq->pqrs", L, C2, optimize=True)
+        N = np.einsum("pqts,tr->pqrs", M, C3, optimize=True)
+        return np.einsum("pqrt,ts->pqrs", N, C4, optimize=True)
+
+    def ovov(C1, o1, C2, o2):
+        O = quarter(C1[:, o1:], C1[:, :o1], C2[:, :o2], C2[:, o2:])      # Om(a,i,j,b)
+        v1, v2 = C1.shape[0] - o1, C2.shape[0] - o2
+        return O.transpose(2, 3, 1, 0).reshape(o2 * v2, o1 * v1)        # [(j,b)][(i,a)]
+
+    def vvoo(C, o):
+        O = quarter(C[:, o:], C[:, o:], C[:, :o], C[:, :o])              # Om(a,b,r,s)
+        v = C.shape[0] - o
+        # record (j,b), element (i,a) = Om(a,b,j,i)
+        return O.transpose(2, 1, 3, 0).reshape(o * v, o * v)
+
+    return {"ajib_AA": ovov(Ca, na, Ca, na), "ajbi_AA": vvoo(Ca, na), "ajib_AB": ovov(Ca, na, Cb, nb),
+            "ajib_BB": ovov(Cb, nb, Cb, nb), "ajbi_BB": vvoo(Cb, nb)}
+
+
+def cis_uhf_matrix(streams, eig_a, eig_b, na, nb):
+    """The A matrix of cis_scf_uhf (cis.f90:124-190): column c is record c;
+    AA = ajib - ajbi plus e_a - e_i on the diagonal, the AB block top right,
+    BB like AA; symmetric from its upper triangle, as DSYEV('U') reads it."""
+    n = len(eig_a)
+    va, vb = n - na, n - nb
+    upA, upB = na * va, nb * vb
+    A = np.zeros((upA + upB, upA + upB))
+    A[:upA, :upA] = (streams["ajib_AA"] - streams["ajbi_AA"]).T
+    A[:upA, upA:] = streams["ajib_AB"].T
+    A[upA:, upA:] = (streams["ajib_BB"] - streams["ajbi_BB"]).T
+    for i in range(na):
+        for a in range(va):
+            A[i * va + a, i * va + a] += eig_a[na + a] - eig_a[i]
+    for j in range(nb):
+        for b in range(vb):
+            A[upA + j * vb + b, upA + j * vb + b] += eig_b[nb + b] - eig_b[j]
+    return np.triu(A) + np.triu(A, 1).T

@@ -4,8 +4,11 @@
 (src/mp2/mp2.f90:154-232), against the committed fixtures of the reference's
 own programs (uhf_*.npz: its dens/UHFI2G loop and its mp2; uhf_no.npz also
 the NO outputs the reference ships), its mp2 program run on our records, and
-closed-shell identities with the RHF path.  The Fortran programs scf, ao2mo
-and mp2 run with REF=UHF at the end.
+closed-shell identities with the RHF path.  UHF-CIS (slow_ao2mo_CIS_UHF,
+src/ao2mo/ao2mo.f90:919-1227; cis_scf_uhf, src/CIS/cis.f90:87-223) against
+the oracle's restated streams and matrix, and OH against the CFOUR output the
+reference ships.  The Fortran programs scf, ao2mo, mp2 and cis run with
+REF=UHF at the end.
 """
 from __future__ import annotations
 
@@ -186,13 +189,14 @@ def _records(path):
     return out
 
 
-def _write_inputs(d, Z, xyz_bohr, mult, scf_conv=11):
+def _write_inputs(d, Z, xyz_bohr, mult, scf_conv=11, calc=1, excite=0, e_num=0):
     """envdat as the reference parser writes it for REF=UHF
-    (src/parser/parser.f90:503-531): nelcA = nelcB + MULTI - 1."""
+    (src/parser/parser.f90:503-531): nelcA = nelcB + MULTI - 1.  options are
+    0-based: calc (1), ref (3), SCF_Conv (8), multi (10), excite (13), E_NUM (15)."""
     nelc = int(sum(Z))
     nb = (nelc - (mult - 1)) // 2
     na = nb + mult - 1
-    opts = [1, 1, 0, 1, 0, 1, 1000000, 1, scf_conv, 0, mult, 1, 1, 0, 0, 0, 0]
+    opts = [1, calc, 0, 1, 0, 1, 1000000, 1, scf_conv, 0, mult, 1, 1, excite, 0, e_num, 0]
     with open(os.path.join(d, "envdat"), "w") as f:
         f.write(f" {len(Z)}\n {na} {nb}\n 17\n " + " ".join(map(str, opts)) + "\n")
     with open(os.path.join(d, "nucpos"), "w") as f:
@@ -262,3 +266,86 @@ def test_uhf_programs_end_to_end(tmp_path):
         theirs = _run(d, REF_MP2)
         for lab, v in zip(("E(AA) =", "E(BB) =", "E(AB) =", "E(MBPT2) ="), ours):
             assert abs(_number(theirs, lab) - v) <= 1e-10, lab
+
+
+# -- UHF-CIS ----------------------------------------------------------------------
+
+CFOUR_OH = [0.0, 0.228665544103, 0.400333416145]   # examples/OH/cfour/out, TDA roots 1-3
+
+
+@pytest.mark.parametrize("name", ["uhf_no", "uhf_oh", "uhf_o3"])
+def test_cis_streams_match_oracle(ctx, name):
+    """The five streams of slow_ao2mo_CIS_UHF against its quarter steps on the
+    reference's dense XX, for the fixture's orbitals."""
+    from oracle import ref_driver as R
+    g = golden(name)
+    _setup(ctx, g)
+    na, nb = int(g["nalpha"]), int(g["nbeta"])
+    got = ctx.ao2mo_cis_uhf(g["Ca"], g["Cb"], na, nb)
+    want = R.cis_uhf_streams(g["XX_full"], g["Ca"], g["Cb"], na, nb)
+    for key in ctx.CIS_STREAMS:
+        assert got[key].shape == want[key].shape, key
+        if want[key].size:
+            scale = max(1.0, np.max(np.abs(want[key])))
+            assert np.max(np.abs(got[key] - want[key])) <= 1e-10 * scale, key
+
+
+@pytest.mark.parametrize("name", ["uhf_no", "uhf_oh", "uhf_o3"])
+def test_cis_matches_oracle(ctx, name):
+    """Every excitation energy of the UHF-CIS matrix against numpy's
+    eigensolver on the restated matrix; eigenvectors up to sign for isolated
+    roots; device-resident and record-stream entry points agree.  OH also
+    against the CFOUR output the reference ships."""
+    from oracle import ref_driver as R
+    g = golden(name)
+    _setup(ctx, g)
+    n, na, nb = int(g["n"]), int(g["nalpha"]), int(g["nbeta"])
+    dim = na * (n - na) + nb * (n - nb)
+    streams = ctx.ao2mo_cis_uhf(g["Ca"], g["Cb"], na, nb)
+    E, V = ctx.cis_uhf(g["eig_a"], g["eig_b"], na, nb, dim)
+    A = R.cis_uhf_matrix(R.cis_uhf_streams(g["XX_full"], g["Ca"], g["Cb"], na, nb), g["eig_a"], g["eig_b"], na, nb)
+    w, U = np.linalg.eigh(A)
+    print(f"[cis] {name}: dim {dim}, lowest {E[:4]}", flush=True)
+    assert np.max(np.abs(E - w)) <= 1e-10
+    for k in range(dim):
+        gap = min([abs(w[k] - w[j]) for j in range(dim) if j != k] or [1.0])
+        if gap > 1e-6:
+            assert abs(abs(V[:, k] @ U[:, k]) - 1.0) <= 1e-8, k
+    E2, V2 = ctx.cis_uhf(g["eig_a"], g["eig_b"], na, nb, 3, streams=streams)
+    assert np.max(np.abs(E2 - E[:3])) <= 1e-12
+    if name == "uhf_oh":
+        assert np.max(np.abs(E[:3] - CFOUR_OH)) <= 1e-5
+
+
+def test_cis_programs_end_to_end(tmp_path):
+    """int1e, scf, ao2mo and cis on the reference's OH example (REF=UHF,
+    MULTI=2, CALC=SCF, EXCITE=CIS, E_NUM=4): the five stream files, the
+    ints_AA/ints_BB files, and the printed eigenvalues against the oracle on
+    this run's own orbitals and against CFOUR."""
+    from oracle import ref_driver as R
+    if not all(os.path.exists(os.path.join(BIN, p)) for p in ("int1e", "scf", "ao2mo", "cis")):
+        pytest.fail("Fortran host programs not built (myqc_amd/build.py)")
+    g = golden("uhf_oh")
+    n = int(g["n"])
+    d = str(tmp_path)
+    na, nb = _write_inputs(d, g["Z"], g["xyz"], 2, calc=0, excite=1, e_num=4)
+    _run(d, os.path.join(BIN, "int1e"))
+    _run(d, os.path.join(BIN, "scf"))
+    _run(d, os.path.join(BIN, "ao2mo"))
+    out = _run(d, os.path.join(BIN, "cis"))
+    vals = np.array([float(x) for x in re.findall(r"Eigenvalue \(a\.u\.\)\s+([-+0-9.Ee]+)", out)])
+    assert vals.size == 4, out
+    cv = np.array(open(os.path.join(d, "Cui")).read().split(), dtype=np.float64)
+    Ca, Cb = cv[:n * n].reshape(n, n, order="F"), cv[n * n:].reshape(n, n, order="F")
+    ev = np.array(" ".join(open(os.path.join(d, "eig")).read().split()).split(), dtype=np.float64)
+    st = R.cis_uhf_streams(g["XX_full"], Ca, Cb, na, nb)
+    for key in ("ajib_AA", "ajbi_AA", "ajib_AB", "ajib_BB", "ajbi_BB"):
+        recs = _records(os.path.join(d, key))
+        assert len(recs) == st[key].shape[0], key
+        for x, y in zip(recs, st[key]):
+            assert np.max(np.abs(x - y)) <= 1e-10, key
+    ints = _records(os.path.join(d, "ints_AA"))
+    assert np.max(np.abs(np.array(ints) - (st["ajib_AA"] - st["ajbi_AA"]))) <= 1e-10
+    w = np.linalg.eigvalsh(R.cis_uhf_matrix(st, ev[:n], ev[n:], na, nb))
+    assert np.max(np.abs(vals - w[:4])) <= 1e-9
+    assert np.max(np.abs(vals[:3] - CFOUR_OH)) <= 1e-5

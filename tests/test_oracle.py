@@ -280,3 +280,41 @@ def test_uhf_mp2_fixture_against_spin_orbital_formula(name):
                     E += t * t / (es[i[0]][i[1]] + es[j[0]][j[1]] - es[a[0]][a[1]] - es[b[0]][b[1]])
     assert abs(E - float(g["E_mp2"])) <= 1e-10
     assert abs(float(g["E_aa"]) + float(g["E_bb"]) + float(g["E_ab"]) - float(g["E_mp2"])) <= 1e-14
+
+
+def test_cis_oracle_h2_singlet_triplet_closed_form():
+    """UHF-CIS on closed-shell H2 (STO-3G, R = 1.4 bohr; one occupied and one
+    virtual orbital per spin): the 2 x 2 CIS matrix has the triplet e2 - e1 -
+    J12 and the singlet e2 - e1 - J12 + 2 K12 (Szabo & Ostlund, sec. 4.3), with
+    J12 = (11|22) and K12 = (12|12) in the MO basis.  Pins the roles and signs
+    of (ai|jb) and (ab|ji) in the restated streams and matrix."""
+    from oracle import ref_driver as R
+    Z = [1, 1]
+    xyz = np.array([[0, 0, -0.7], [0, 0, 0.7]])
+    s = O.System(Z, xyz, build_basis(Z), FT)
+    S, H = O.int1e(s)
+    XX = O.eri_full(s)
+    r = O.rhf(S, H, XX, 1, O.enuc(Z, xyz))
+    C, e = r["C"], r["eig"]
+    mo = np.einsum("uvlk,up,vq,lr,ks->pqrs", XX, C, C, C, C)
+    J12, K12 = mo[0, 0, 1, 1], mo[0, 1, 0, 1]
+    st = R.cis_uhf_streams(XX, C, C, 1, 1)
+    w = np.linalg.eigvalsh(R.cis_uhf_matrix(st, e, e, 1, 1))
+    d = e[1] - e[0]
+    assert np.allclose(w, [d - J12, d - J12 + 2 * K12], atol=1e-12, rtol=0)
+    assert K12 > 1e-3   # the two states are distinct
+
+
+def test_cis_oracle_oh_matches_cfour():
+    """UHF-CIS of OH (examples/OH: STO-3G, MULTI=2, EXCITE=CIS, E_NUM=4) from
+    the reference chain's orbitals, against the TDA (CIS) excitation energies
+    in the CFOUR output the reference ships (examples/OH/cfour/out): 0,
+    0.228665544, 0.400333416 Eh.  CFOUR's integrals use the exact pi and Boys
+    function (its E(SCF) is 9e-6 Eh from the reference's), so the agreement
+    is to 1e-5."""
+    from oracle import ref_driver as R
+    g = golden("uhf_oh")
+    na, nb = int(g["nalpha"]), int(g["nbeta"])
+    st = R.cis_uhf_streams(g["XX_full"], g["Ca"], g["Cb"], na, nb)
+    w = np.linalg.eigvalsh(R.cis_uhf_matrix(st, g["eig_a"], g["eig_b"], na, nb))
+    assert np.max(np.abs(w[:3] - [0.0, 0.228665544103, 0.400333416145])) <= 1e-5
