@@ -79,6 +79,17 @@ int myqc_fock_G(myqc_ctx *ctx, const double *D, double *G, int precision);
  * spin densities C_occ C_occ^T of each spin, symmetrised). */
 int myqc_fock_G_uhf(myqc_ctx *ctx, const double *Da, const double *Db, double *Ga, double *Gb);
 
+/* int1p drop-in (src/integrals/int1p.f90:140-355): the dipole integrals
+ * P_c(u,v) = <u| r_c |v> about the coordinate origin, c = x, y, z, as three
+ * n x n column-major matrices one after another (the three records of Pxuv,
+ * :230-233). */
+int myqc_int1p(myqc_ctx *ctx, double *P);
+
+/* cphf_dipole_rhf drop-in (src/props/cphf.f90:106-122; :80-131): the SCF dipole
+ * moment mu_c = -sum_uv D(u,v) P_c(v,u) + sum_A Z_A R_A,c (atomic units) for
+ * the total density D (n x n; the Da record of an RHF run). */
+int myqc_dipole_rhf(myqc_ctx *ctx, const double *D, double *mu);
+
 /* densRHF drop-in: D(u,v) = 2 sum_{i<nocc} C(u,i) C(v,i)
  * (src/dens/dens.f90:115-124). */
 int myqc_density_rhf(myqc_ctx *ctx, const double *C, int nocc, double *D);

@@ -60,7 +60,7 @@ def build_lib(verbose: bool = True) -> str:
 
 
 def build_fortran(verbose: bool = True):
-    """Fortran host programs (int1e, int2e, scf, ao2mo, mp2, cis) over the C ABI."""
+    """Fortran host programs (int1e, int2e, scf, ao2mo, mp2, cis, int1p, cphf) over the C ABI."""
     fc = shutil.which("flang") or shutil.which("amdflang")
     if fc is None or not os.path.isdir(FSRC):
         if verbose:
@@ -76,7 +76,7 @@ def build_fortran(verbose: bool = True):
         o = os.path.join(BUILD, m.replace(".f90", ".o"))
         _run([fc, *flags, "-c", os.path.join(FSRC, m), "-o", o])
         objs.append(o)
-    for prog in ["int1e", "int2e", "scf", "ao2mo", "mp2", "cis"]:
+    for prog in ["int1e", "int2e", "scf", "ao2mo", "mp2", "cis", "int1p", "cphf"]:
         src = os.path.join(FSRC, prog + ".f90")
         if not os.path.exists(src):
             continue

@@ -1,6 +1,8 @@
 // one_e.hip -- overlap S and core Hamiltonian H = T + V over contracted s/p
 // shell pairs, one thread per shell pair (int1e drop-in,
-// src/integrals/int1e.f90:132-280; the per-pair math is in one_e_math.h).
+// src/integrals/int1e.f90:132-280; the per-pair math is in one_e_math.h),
+// the dipole integrals of int1p (src/integrals/int1p.f90:140-355) and the
+// SCF dipole moment of cphf_dipole_rhf (src/props/cphf.f90:106-121).
 #include <cmath>
 
 #include "context.h"
@@ -30,9 +32,94 @@ __global__ __launch_bounds__(128) void k_one_e(const ShellPair *__restrict__ pai
         }
 }
 
+// P_c (c = x, y, z) at P + c n^2, both triangles
+__global__ __launch_bounds__(128) void k_dipole(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
+                                                int npairs, int n, double *__restrict__ P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    const ShellPair sp = pairs[t];
+    double dip[3][3][3];
+    dipole_pair(sp, prims, dip);
+    const int na = sp.la ? 3 : 1, nb = sp.lb ? 3 : 1;
+    const size_t nn = (size_t)n * n;
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < nb; ++b) {
+                const size_t i = sp.ao_a + a, j = sp.ao_b + b;
+                P[c * nn + i + n * j] = dip[c][a][b];
+                P[c * nn + j + n * i] = dip[c][a][b];
+            }
+}
+
+// mu[c] = -sum_uv D(u,v) P_c(v,u): one workgroup per component, a fixed-order fold
+__global__ __launch_bounds__(256) void k_dipole_contract(const double *__restrict__ P, const double *__restrict__ D,
+                                                         int n, double *__restrict__ mu) {
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
+    const size_t nn = (size_t)n * n;
+    double acc = 0.0;
+    for (size_t e = threadIdx.x; e < nn; e += blockDim.x) {
+        const size_t u = e % n, v = e / n;
+        acc += D[e] * P[c * nn + v + (size_t)n * u];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mu[c] = -sh[0];
+}
+
+static int dipole_integrals(myqc_ctx *ctx, double *dP) {
+    const int n = ctx->n, np = (int)ctx->pairs.size();
+    k_dipole<<<(np + 127) / 128, 128, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_prims, np, n, dP);
+    MYQC_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
 }  // namespace myqc
 
 using namespace myqc;
+
+extern "C" int myqc_int1p(myqc_ctx *ctx, double *P) {
+    if (!ctx || !P) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_int1p: no system set");
+    const size_t nn = (size_t)ctx->n * ctx->n;
+    double *dP = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&dP, sizeof(double) * 3 * nn));
+    int rc = dipole_integrals(ctx, dP);
+    if (rc == 0 && d2h(ctx, P, dP, sizeof(double) * 3 * nn) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "myqc_int1p: download failed");
+    dfree((void **)&dP);
+    return rc;
+}
+
+extern "C" int myqc_dipole_rhf(myqc_ctx *ctx, const double *D, double *mu) {
+    if (!ctx || !D || !mu) return MYQC_E_ARG;
+    if (!ctx->have_system) return set_error(ctx, MYQC_E_STATE, "myqc_dipole_rhf: no system set");
+    const int n = ctx->n;
+    const size_t nn = (size_t)n * n;
+    double *buf = nullptr;
+    MYQC_CHECK(dalloc(ctx, (void **)&buf, sizeof(double) * (4 * nn + 3)));
+    double *dP = buf, *dD = buf + 3 * nn, *dmu = buf + 4 * nn;
+    int rc = 0;
+    if (hipMemcpyAsync(dD, D, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "myqc_dipole_rhf: upload failed");
+    if (rc == 0) rc = dipole_integrals(ctx, dP);
+    if (rc == 0) {
+        k_dipole_contract<<<3, 256, 0, ctx->stream>>>(dP, dD, n, dmu);
+        if (hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "myqc_dipole_rhf: launch failed");
+    }
+    if (rc == 0 && d2h(ctx, mu, dmu, sizeof(double) * 3) != hipSuccess)
+        rc = set_error(ctx, MYQC_E_HIP, "myqc_dipole_rhf: download failed");
+    dfree((void **)&buf);
+    if (rc) return rc;
+    // nuclear part (cphf.f90:118-122)
+    for (int a = 0; a < ctx->nnuc; ++a)
+        for (int c = 0; c < 3; ++c) mu[c] += ctx->Z[a] * ctx->xyz[3 * a + c];
+    return 0;
+}
 
 extern "C" int myqc_int1e(myqc_ctx *ctx, double *S, double *H) {
     if (!ctx || !S || !H) return MYQC_E_ARG;

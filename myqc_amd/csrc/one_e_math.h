@@ -11,9 +11,10 @@
 
 namespace myqc {
 
-// E^{ij}_0 for i <= 1, j <= 3 (1-D overlap factor without (pi/p)^{1/2})
-MYQC_HD double e1d(int i, int j, double PA, double PB, double p) {
-    if (i < 0 || j < 0) return 0.0;
+// E^{ij}_t for i <= 1, j <= 3, t <= 4 (Hermite expansion coefficients; t = 0
+// is the 1-D overlap factor without (pi/p)^{1/2})
+MYQC_HD double e1d_t(int i, int j, int t, double PA, double PB, double p) {
+    if (i < 0 || j < 0 || t < 0 || t > 4) return 0.0;
     // E[t][i][j] by the Hermite recursion on j, then i
     double E[5][2][4];
     for (int t = 0; t < 5; ++t)
@@ -27,7 +28,41 @@ MYQC_HD double e1d(int i, int j, double PA, double PB, double p) {
     for (int b = 0; b <= 3; ++b)
         for (int t = 0; t <= b + 1; ++t)
             E[t][1][b] = (t > 0 ? h * E[t - 1][0][b] : 0.0) + PA * E[t][0][b] + (t + 1 <= 4 ? (t + 1) * E[t + 1][0][b] : 0.0);
-    return E[0][i][j];
+    return E[t][i][j];
+}
+
+MYQC_HD double e1d(int i, int j, double PA, double PB, double p) { return e1d_t(i, j, 0, PA, PB, p); }
+
+// Dipole integrals <a| r_c |b> (c = x, y, z, about the coordinate origin) of
+// one shell pair, as int1p's dipole (src/integrals/int1p.f90:265-355):
+//   (pi/p)^{3/2} K (E^c_1 + P_c E^c_0) prod_{d != c} E^d_0,
+// with single-precision pi and the EIJ < 1e-14 pair screen (:214-218).
+MYQC_HD void dipole_pair(const ShellPair &sp, const PrimPair *__restrict__ prims, double dip[3][3][3]) {
+    const int na = sp.la ? 3 : 1, nb = sp.lb ? 3 : 1;
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) dip[c][a][b] = 0.0;
+    for (int ip = 0; ip < sp.npp; ++ip) {
+        const PrimPair pr = prims[sp.pp0 + ip];
+        if (pr.EIJ < 1.0e-14) continue;
+        const double p = pr.p;
+        double PB[3];
+        for (int d = 0; d < 3; ++d) PB[d] = pr.PA[d] + sp.AB[d];
+        const double pref = pow(PI_SP / p, 1.5) * pr.K;
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < nb; ++b) {
+                int la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
+                if (sp.la) la[a] = 1;
+                if (sp.lb) lb[b] = 1;
+                double E0[3], E1[3];
+                for (int d = 0; d < 3; ++d) {
+                    E0[d] = e1d_t(la[d], lb[d], 0, pr.PA[d], PB[d], p);
+                    E1[d] = e1d_t(la[d], lb[d], 1, pr.PA[d], PB[d], p);
+                }
+                for (int c = 0; c < 3; ++c)
+                    dip[c][a][b] += pref * (E1[c] + pr.P[c] * E0[c]) * E0[(c + 1) % 3] * E0[(c + 2) % 3];
+            }
+    }
 }
 
 // S, H = T + V contributions of one shell pair; atoms[4c..4c+3] = Z, x, y, z

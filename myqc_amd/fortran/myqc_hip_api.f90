@@ -157,6 +157,18 @@ MODULE myqc_hip_api
       INTEGER(c_int), VALUE :: nalpha, nbeta, nvrt_a, nvrt_b, nroots
       REAL(c_double) :: energies(*), vectors(*)
     END FUNCTION myqc_cis_uhf
+
+    INTEGER(c_int) FUNCTION myqc_int1p(ctx, P) BIND(C, name='myqc_int1p')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: P(*)
+    END FUNCTION myqc_int1p
+
+    INTEGER(c_int) FUNCTION myqc_dipole_rhf(ctx, D, mu) BIND(C, name='myqc_dipole_rhf')
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double) :: D(*), mu(3)
+    END FUNCTION myqc_dipole_rhf
   END INTERFACE
 
 CONTAINS

@@ -40,6 +40,8 @@ EXPORTS = {
     "myqc_mp2_rhf": ([_vp, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "myqc_mp2_rhf_records": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
+    "myqc_int1p": ([_vp, _dp], ctypes.c_int),
+    "myqc_dipole_rhf": ([_vp, _dp, _dp], ctypes.c_int),
     "myqc_density_uhf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "myqc_uhf_scf": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_double,
                       ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip], ctypes.c_int),
@@ -203,6 +205,19 @@ class Context:
         H = np.zeros((n, n), order="F")
         self._chk(self.L.myqc_int1e(self.h, _p(S), _p(H)))
         return S, H
+
+    def int1p(self):
+        """int1p: the dipole integral matrices, shape (3, n, n) (x, y, z)."""
+        n = self.n
+        P = np.zeros(3 * n * n)
+        self._chk(self.L.myqc_int1p(self.h, _p(P)))
+        return np.array([P[c * n * n:(c + 1) * n * n].reshape(n, n, order="F") for c in range(3)])
+
+    def dipole(self, D):
+        """cphf_dipole_rhf: the dipole moment (a.u.) of the total density D."""
+        mu = np.zeros(3)
+        self._chk(self.L.myqc_dipole_rhf(self.h, _p(_f(D)), _p(mu)))
+        return mu
 
     def ao2mo(self, C, nocc: int, precision: int = FP64, download: bool = True):
         """ao2mo: returns iajb[i, a, j, b] = (ia|jb) (or None if not downloaded)."""

@@ -442,3 +442,14 @@ def cis_uhf_matrix(streams, eig_a, eig_b, na, nb):
         for b in range(vb):
             A[upA + j * vb + b, upA + j * vb + b] += eig_b[nb + b] - eig_b[j]
     return np.triu(A) + np.triu(A, 1).T
+
+
+def dipole_rhf(P, D, Z, xyz):
+    """cphf_dipole_rhf (src/props/cphf.f90:106-121): mu_c = -sum_uv D(u,v)
+    P_c(v,u) + sum_A Z_A R_A,c, with P the three int1p matrices (x, y, z) and
+    D the total density.  Restated: the reference's cphf cannot run under
+    flang's runtime (it reads basinfo where it means Pxuv, cphf.f90:46-49,97-100)."""
+    mu = np.array([-np.sum(D * P[c].T) for c in range(3)])
+    for z, r in zip(Z, np.asarray(xyz)):
+        mu += z * r
+    return mu

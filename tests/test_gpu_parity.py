@@ -102,6 +102,21 @@ def test_rhf_matches_reference_example_outputs(ctx, name):
         assert abs(r["E"] - float(f["example_E"])) <= 1e-8
 
 
+@pytest.mark.parametrize("mol", ["co2", "h2o", "heh", "h2o_2", "mol2"])
+def test_dipole_integrals_and_moment_match_reference(ctx, mol):
+    """myqc_int1p against the reference's own int1p (Pxuv) and
+    myqc_dipole_rhf against the restated cphf contraction on those
+    integrals, for the fixture's converged density."""
+    g = golden("props_" + mol)
+    Z = [int(z) for z in g["Z"]]
+    ctx.set_system(Z, g["xyz"], build_basis(Z))
+    P = ctx.int1p()
+    scale = max(1.0, np.max(np.abs(g["P"])))
+    assert np.max(np.abs(P - g["P"])) <= 1e-10 * scale
+    mu = ctx.dipole(g["D"])
+    assert np.max(np.abs(mu - g["mu"])) <= 1e-10
+
+
 @pytest.mark.parametrize("name", [m for m in MOLECULES if m not in CONVERGED])
 def test_rhf_converges_where_reference_loop_oscillates(ctx, name):
     """The reference's plain Roothaan loop does not converge on these inputs

@@ -245,3 +245,30 @@ def test_uhf_fock_matches_reference_UHFI2G(ctx, mol, tmp_path):
     scale = max(1.0, np.max(np.abs(GaR)))
     assert np.max(np.abs(Ga - GaR)) <= 1e-10 * scale
     assert np.max(np.abs(Gb - GbR)) <= 1e-10 * scale
+
+
+def test_props_programs_end_to_end(tmp_path):
+    """int1e, scf, int1p and cphf on H2O: Pxuv against the reference's own
+    int1p, and the printed dipole (6 decimals, as cphf prints it) against the
+    restated contraction over the reference's integrals."""
+    if not all(os.path.exists(os.path.join(BIN, p)) for p in ("int1e", "scf", "int1p", "cphf")):
+        pytest.fail("Fortran host programs not built (myqc_amd/build.py)")
+    g = golden("h2o")
+    pr = golden("props_h2o")
+    n = int(g["n"])
+    d = str(tmp_path)
+    _write_inputs(d, g["Z"], g["xyz"])
+    _run(d, "int1e")
+    _run(d, "scf")
+    _run(d, "int1p")
+    vals = np.array(open(os.path.join(d, "Pxuv")).read().split(), dtype=np.float64)
+    assert vals.size == 3 * n * n
+    P = vals.reshape(3, n * n)
+    for c in range(3):
+        assert np.max(np.abs(P[c].reshape(n, n, order="F") - pr["P"][c])) <= 1e-10
+    assert "Reading Dipole integrals" in _run(d, "int1p")
+    out = _run(d, "cphf")
+    lines = out.splitlines()
+    k = next(i for i, ln in enumerate(lines) if "X" in ln and "Y" in ln and "Z" in ln)
+    mu = np.array([float(x) for x in lines[k + 1].split()])
+    assert np.max(np.abs(mu - pr["mu"])) <= 1e-6 + 1e-12

@@ -318,3 +318,25 @@ def test_cis_oracle_oh_matches_cfour():
     st = R.cis_uhf_streams(g["XX_full"], g["Ca"], g["Cb"], na, nb)
     w = np.linalg.eigvalsh(R.cis_uhf_matrix(st, g["eig_a"], g["eig_b"], na, nb))
     assert np.max(np.abs(w[:3] - [0.0, 0.228665544103, 0.400333416145])) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["props_co2", "props_h2o", "props_heh", "props_h2o_2", "props_mol2"])
+def test_reference_dipole_integrals_closed_forms(name):
+    """The reference int1p's dipole integrals (fixtures): symmetric, and on
+    the diagonal P_c(u,u) = R_c S(u,u) with R the centre of AO u (an s or p
+    function's density is symmetric about its centre) -- pins the origin and
+    sign.  The restated cphf contraction gives CO2 (centred, symmetric) a zero
+    dipole."""
+    g = golden(name)
+    P, S = g["P"], g["S"]
+    Z = [int(z) for z in g["Z"]]
+    for c in range(3):
+        assert np.max(np.abs(P[c] - P[c].T)) <= 1e-14
+    ao_atom = []   # AO order of buildBasis: 1s for H/He, 1s 2s 2p for Li-Ne
+    for a, z in enumerate(Z):
+        ao_atom += [a] * (1 if z <= 2 else 5)
+    R = np.asarray(g["xyz"])[ao_atom]
+    for c in range(3):
+        assert np.max(np.abs(np.diag(P[c]) - R[:, c] * np.diag(S))) <= 1e-12
+    if name == "props_co2":
+        assert np.max(np.abs(g["mu"])) <= 1e-10
