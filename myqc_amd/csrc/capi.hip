@@ -49,6 +49,7 @@ extern "C" int myqc_finalize(myqc_ctx *ctx) {
     for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_ksoa[c]);
     for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_uhf[c]);
     for (int c = 0; c < 2; ++c) dfree((void **)&ctx->d_vvoo[c]);
+    for (auto &t : ctx->d_tmp) dfree(&t);
     for (auto &e : ctx->events) {
         hipEventDestroy(e.a);
         hipEventDestroy(e.b);

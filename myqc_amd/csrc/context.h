@@ -137,6 +137,12 @@ struct myqc_ctx {
     int scf_purify = 0, scf_diag = 0;
     int scf_sp2_steps = 0;                // steps the last purification took
 
+    // per-call device temporaries of the solve path (int1e, the learned
+    // guess), kept across calls with their capacity: a repeated call makes no
+    // hipMalloc/hipFree (hipFree waits for the whole device, which would
+    // serialise contexts that run side by side on other streams)
+    void *d_tmp[8] = {};
+
     // capacity of each device allocation made through dalloc
     std::unordered_map<void *, size_t> dcap;
 
@@ -180,6 +186,16 @@ hipError_t d2h(myqc_ctx *ctx, void *dst, const void *src, size_t bytes);
 // device allocation helper (frees and reallocates to `bytes`)
 int dalloc(myqc_ctx *ctx, void **p, size_t bytes);
 void dfree(void **p);
+// slots of ctx->d_tmp
+enum TmpSlot { TMP_S = 0, TMP_H, TMP_ATOMS, TMP_GPAIRS, TMP_GATOMS, TMP_GDENS, TMP_GMOL, TMP_NSLOT };
+static_assert(TMP_NSLOT <= 8, "d_tmp slots");
+// a temporary of at least `bytes` in slot `slot`, kept by the context
+template <typename T>
+int dtmp(myqc_ctx *ctx, int slot, T **p, size_t bytes) {
+    MYQC_CHECK(dalloc(ctx, &ctx->d_tmp[slot], bytes));
+    *p = static_cast<T *>(ctx->d_tmp[slot]);
+    return 0;
+}
 
 // timing helpers: bracket a launch sequence on ctx->stream
 int timer_begin(myqc_ctx *ctx, int cat, hipEvent_t *ev);

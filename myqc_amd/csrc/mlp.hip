@@ -622,10 +622,10 @@ extern "C" int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const d
     PairRec *dP = nullptr;
     double *dA = nullptr, *dD = nullptr;
     int *dM = nullptr;
-    int rc = dalloc(ctx, (void **)&dP, sizeof(PairRec) * npair);
-    if (!rc) rc = dalloc(ctx, (void **)&dA, sizeof(double) * atoms.size());
-    if (!rc) rc = dalloc(ctx, (void **)&dD, sizeof(double) * (size_t)n * n);
-    if (!rc) rc = dalloc(ctx, (void **)&dM, sizeof(int) * 2);
+    int rc = dtmp(ctx, TMP_GPAIRS, &dP, sizeof(PairRec) * npair);
+    if (!rc) rc = dtmp(ctx, TMP_GATOMS, &dA, sizeof(double) * atoms.size());
+    if (!rc) rc = dtmp(ctx, TMP_GDENS, &dD, sizeof(double) * (size_t)n * n);
+    if (!rc) rc = dtmp(ctx, TMP_GMOL, &dM, sizeof(int) * 2);
     if (!rc && (hipMemcpyAsync(dP, prs.data(), sizeof(PairRec) * npair, hipMemcpyHostToDevice, ctx->stream) ||
                 hipMemcpyAsync(dA, atoms.data(), sizeof(double) * atoms.size(), hipMemcpyHostToDevice, ctx->stream) ||
                 hipMemcpyAsync(dM, mol_atoms, sizeof(int) * 2, hipMemcpyHostToDevice, ctx->stream)))
@@ -634,9 +634,5 @@ extern "C" int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const d
     if (!rc && hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "myqc_guess_density: launch failed");
     if (!rc && d2h(ctx, D, dD, sizeof(double) * (size_t)n * n) != hipSuccess)
         rc = set_error(ctx, MYQC_E_HIP, "myqc_guess_density: download failed");
-    dfree((void **)&dP);
-    dfree((void **)&dA);
-    dfree((void **)&dD);
-    dfree((void **)&dM);
     return rc;
 }

@@ -132,9 +132,9 @@ extern "C" int myqc_int1e(myqc_ctx *ctx, double *S, double *H) {
     }
     double *dS = nullptr, *dH = nullptr, *dA = nullptr;
     const size_t nn = (size_t)n * n;
-    int rc = dalloc(ctx, (void **)&dS, sizeof(double) * nn);
-    if (rc == 0) rc = dalloc(ctx, (void **)&dH, sizeof(double) * nn);
-    if (rc == 0) rc = dalloc(ctx, (void **)&dA, sizeof(double) * atoms.size());
+    int rc = dtmp(ctx, TMP_S, &dS, sizeof(double) * nn);
+    if (rc == 0) rc = dtmp(ctx, TMP_H, &dH, sizeof(double) * nn);
+    if (rc == 0) rc = dtmp(ctx, TMP_ATOMS, &dA, sizeof(double) * atoms.size());
     if (rc == 0 && hipMemcpyAsync(dA, atoms.data(), sizeof(double) * atoms.size(), hipMemcpyHostToDevice,
                                   ctx->stream) != hipSuccess)
         rc = set_error(ctx, MYQC_E_HIP, "upload failed");
@@ -146,8 +146,5 @@ extern "C" int myqc_int1e(myqc_ctx *ctx, double *S, double *H) {
     if (rc == 0 && (d2h(ctx, S, dS, sizeof(double) * nn) != hipSuccess ||
                     d2h(ctx, H, dH, sizeof(double) * nn) != hipSuccess))
         rc = set_error(ctx, MYQC_E_HIP, "download failed");
-    dfree((void **)&dS);
-    dfree((void **)&dH);
-    dfree((void **)&dA);
     return rc;
 }
