@@ -86,7 +86,10 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
             fq += r[k] * xk * inv_fact[k];
             xk *= x;
         }
-        double e = exp(-T);
+        // e^{-T} only feeds the downward recursion from F_Q: with Q = 0 (sets
+        // of s functions only) there is none, F_0 is the Taylor sum itself
+        const int top = Q < M ? Q : M;
+        const double e = Q > 0 ? exp(-T) : 0.0;
         double t2 = 2.0 * T;
         // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160),
         // with 1/(2j+1) as constants; the reference divides (rounding-level difference)
@@ -97,7 +100,6 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
             if (j >= M && j < Q) fq = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
         // fq now holds F_top, top = min(Q, M); the reference never forms F_j for
         // j > Q, and callers only read F_j for j <= Q
-        const int top = Q < M ? Q : M;
 #pragma unroll
         for (int j = M; j >= 0; --j) {
             if (j > top) F[j] = 0.0;

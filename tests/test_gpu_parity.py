@@ -267,3 +267,41 @@ def test_ao2mo_bf16x3_water_cluster(ctx):
     XX = O.eri_full(O.System(Z, xyz, build_basis(Z), __import__("myqc_amd.basis", fromlist=["x"]).read_ftab()))
     sl = O.ao2mo_ovov(XX, r["C"], nocc)
     assert np.max(np.abs(sl - ref)) <= 1e-10
+
+
+def test_contexts_side_by_side_match_reference():
+    """Four contexts solving four molecules from four host threads at once
+    (one HIP stream each): every energy equals the reference chain's and the
+    same context's solve run alone."""
+    import threading
+
+    from myqc_amd.hip import Context
+    names = ["h2o_2", "h2o_4", "co2", "mol3"]
+    fx = [golden(n) for n in names]
+    ctxs = [Context(0) for _ in names]
+    try:
+        def one(k, out):
+            f = fx[k]
+            Z = [int(z) for z in f["Z"]]
+            c = ctxs[k]
+            c.set_system(Z, f["xyz"], build_basis(Z))
+            c.eri_build(1e-12)
+            S, H = c.int1e()
+            out[k] = c.rhf(H, S, int(f["nocc"]), enuc(Z, f["xyz"]), e_tol=1e-10, d_tol=1e-8)["E"]
+
+        alone = {}
+        for k in range(len(names)):
+            one(k, alone)
+        together = {}
+        ths = [threading.Thread(target=one, args=(k, together)) for k in range(len(names))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        for k, n in enumerate(names):
+            print(f"[side-by-side] {n}: {together[k]:.12f} alone {alone[k]:.12f}", flush=True)
+            assert abs(together[k] - float(fx[k]["E_scf"])) <= 1e-8
+            assert abs(together[k] - alone[k]) <= 1e-12
+    finally:
+        for c in ctxs:
+            c.close()
