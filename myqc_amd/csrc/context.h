@@ -101,6 +101,9 @@ struct myqc_ctx {
     int *d_chunk = nullptr;               // [c] = {offset lo, hi, aoB | LB << 15 | class << 16 | kets << 18, first rank}
     long long *d_unit_off = nullptr;      // [u][3] segment offsets
     int *d_meta[3] = {nullptr, nullptr, nullptr};   // per class, by Schwarz rank: aoC | aoD << 15 | same << 30
+    float *d_qrank[3] = {nullptr, nullptr, nullptr}; // per class, the Schwarz factor of each rank (descending)
+    float *d_chunkq = nullptr;            // [c][4] q_row q_first, q_row q_last, q_row, 0 (rounded up)
+    double *d_dmax = nullptr;             // max |dD| of the current incremental pass (float bits in the first word)
     int *d_shell = nullptr;               // [s] = {ao, L, wg0, wg1}
     int *d_ao_shell = nullptr;            // shell of each AO
     double *d_partial = nullptr;          // n_wg x 3 x n (both precisions accumulate in FP64)

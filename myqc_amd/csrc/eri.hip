@@ -12,6 +12,7 @@
 // Reference: src/integrals/int2e.f90:78-351 (proc2e), :618-726 (clmnew),
 // :540-613 (fillsym); src/integrals/auxilary.f90 (Boys, gtoD, getDk).
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -268,6 +269,8 @@ void free_eri(myqc_ctx *ctx, bool keep_memory) {
     if (!keep_memory) {
         for (auto p : ptrs) dfree(p);
         for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
+        for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_qrank[c]);
+        dfree((void **)&ctx->d_chunkq);
     }
     ctx->n_wg = ctx->n_units = ctx->n_chunks = 0;
     ctx->n_expanded = 0;
@@ -551,6 +554,8 @@ static int build_schedule(myqc_ctx *ctx) {
     ctx->dig_rows = std::max(3, std::min(64, (int)(kLdsBudget / (sizeof(double) * (size_t)n)) - 3));
     std::vector<int> wg, unit, chunk, shell(4 * ns);
     std::vector<long long> unit_off, wg_cost;
+    // per chunk {q_row q_first, q_row q_last, q_row, 0} (the incremental screen)
+    std::vector<float> chunkq;
     {
         long long nchunk = 0, nunit = 0;
         for (size_t seg = 0; seg < ctx->segcnt.size(); ++seg) {
@@ -558,6 +563,7 @@ static int build_schedule(myqc_ctx *ctx) {
             nunit += (seg % 3 == 0) && (ctx->segcnt[seg] + ctx->segcnt[seg + 1] + ctx->segcnt[seg + 2] > 0);
         }
         chunk.reserve(4 * (size_t)nchunk);
+        chunkq.reserve(4 * (size_t)nchunk);
         unit.reserve(4 * (size_t)nunit);
         unit_off.reserve(3 * (size_t)nunit);
     }
@@ -594,6 +600,7 @@ static int build_schedule(myqc_ctx *ctx) {
             const int p = ctx->pair_of[(size_t)X * ns + Y];
             const bool fwd = ctx->pairs[p].sa == X;
             const int cp = ctx->pair_cls[p], rp = ctx->pair_rank[p];
+            const double qrow = ctx->schwarz[p];
             for (int ck = 0; ck < 3; ++ck) {
                 const int start = ctx->segstart[seg + ck], end = start + cnt[ck];
                 const int row = nY * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
@@ -628,6 +635,14 @@ static int build_schedule(myqc_ctx *ctx) {
                         chunk.push_back((int)(off >> 32));
                         chunk.push_back(aoY | (ctx->shells[Y].L << 15) | (ck << 16) | (nk << 18) | (flag[g] << 28));
                         chunk.push_back(r0);
+                        {
+                            const auto &ids = ctx->pair_ids[ck];
+                            const float up = std::numeric_limits<float>::infinity();
+                            chunkq.push_back(std::nextafter((float)(qrow * ctx->schwarz[ids[r0]]), up));
+                            chunkq.push_back(std::nextafter((float)(qrow * ctx->schwarz[ids[r0 + nk - 1]]), up));
+                            chunkq.push_back(std::nextafter((float)qrow, up));
+                            chunkq.push_back(0.0f);
+                        }
                         acc += (long long)nX * row * nk;
                         if (acc >= kTarget) close();
                     }
@@ -665,6 +680,23 @@ static int build_schedule(myqc_ctx *ctx) {
         MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_meta[c], meta[c].data(), sizeof(int) * meta[c].size(),
                                      hipMemcpyHostToDevice, ctx->stream));
     }
+    // Schwarz factors by rank (rounded up, so the screen never drops a
+    // partner that the exact factor keeps)
+    std::vector<float> qrank[3];
+    for (int c = 0; c < 3; ++c) {
+        const auto &ids = ctx->pair_ids[c];
+        if (ids.empty()) continue;
+        qrank[c].resize(ids.size());
+        for (size_t r = 0; r < ids.size(); ++r)
+            qrank[c][r] = std::nextafter((float)ctx->schwarz[ids[r]], std::numeric_limits<float>::infinity());
+        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_qrank[c], sizeof(float) * qrank[c].size()));
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_qrank[c], qrank[c].data(), sizeof(float) * qrank[c].size(),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    }
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_chunkq, sizeof(float) * std::max<size_t>(chunkq.size(), 4)));
+    if (!chunkq.empty())
+        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_chunkq, chunkq.data(), sizeof(float) * chunkq.size(),
+                                     hipMemcpyHostToDevice, ctx->stream));
     std::vector<int> ao_shell(ctx->n);
     for (int X = 0; X < ns; ++X)
         for (int k = 0; k < (ctx->shells[X].L ? 3 : 1); ++k) ao_shell[ctx->shells[X].ao + k] = X;

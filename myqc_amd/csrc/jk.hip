@@ -56,6 +56,13 @@ struct DigArgs {
     double *partial;                          // [w][3][n]
     int n;
     double jscale;                            // 1, or 0 for the exchange part alone
+    // density-weighted screen of the incremental (FP32) passes: a partner of
+    // rank r in class c is skipped when q_row * q_c[r] * dmax < qtau (0: no
+    // screen); q_c are the partners' Schwarz factors by rank (descending)
+    const float *q0, *q1, *q2;
+    const float4 *chunkq;                     // [c] = {q_row q_first, q_row q_last, q_row, 0}, rounded up
+    const unsigned *dmax;                     // max |D| of this pass (device, float bits)
+    double qtau;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -80,13 +87,14 @@ __host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 :
 // kets issued before their arithmetic.
 template <int LA, int LB, int CK, typename TV>
 __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int r0, int nk,
-                                             long long off, int aoB, double wJ, double wK) {
+                                             long long off, int aoB, double wJ, double wK, double thr) {
     constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
     constexpr int ROW = nb * nc * nd, BLK = na * ROW, KPL = kets_per_lane(ROW);
     const int n = a.n;
     const int lane = threadIdx.x & 63;
     const int *meta = CK == 0 ? a.meta0 : CK == 1 ? a.meta1 : a.meta2;
     const TV *dkc = CK == 0 ? a.dk0 : CK == 1 ? a.dk1 : a.dk2;
+    const float *qk = CK == 0 ? a.q0 : CK == 1 ? a.q1 : a.q2;
     bool act[KPL];
     int aoC[KPL], aoD[KPL], same[KPL];
     TV dq[KPL][nc * nd], dbd[KPL][nb * nd], dbc[KPL][nb * nc];
@@ -94,7 +102,9 @@ __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_row
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
         const int k = lane + 64 * j;
-        act[j] = k < nk;
+        // thr > 0: the screen of the incremental passes (partners whose
+        // integrals with this row pair are all below it drop out)
+        act[j] = k < nk && (thr <= 0.0 || (double)qk[r0 + k] >= thr);
         const int r = r0 + (act[j] ? k : 0);
         const int m = meta[r];
         aoC[j] = m & 0x7fff;
@@ -169,15 +179,16 @@ __device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_row
 
 template <int LA, typename TV>
 __device__ __forceinline__ void digest_chunk_la(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int LB, int ck,
-                                                int r0, int nk, long long off, int aoB, double wJ, double wK) {
+                                                int r0, int nk, long long off, int aoB, double wJ, double wK,
+                                                double thr) {
     if (LB == 0) {
-        if (ck == 0) digest_chunk<LA, 0, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
-        else if (ck == 1) digest_chunk<LA, 0, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
-        else digest_chunk<LA, 0, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
+        if (ck == 0) digest_chunk<LA, 0, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
+        else if (ck == 1) digest_chunk<LA, 0, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
+        else digest_chunk<LA, 0, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
     } else {
-        if (ck == 0) digest_chunk<LA, 1, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
-        else if (ck == 1) digest_chunk<LA, 1, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
-        else digest_chunk<LA, 1, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK);
+        if (ck == 0) digest_chunk<LA, 1, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
+        else if (ck == 1) digest_chunk<LA, 1, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
+        else digest_chunk<LA, 1, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
     }
 }
 
@@ -204,13 +215,32 @@ __global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6;
+    const bool screen = a.qtau > 0.0;
+    const double dmax = screen ? (double)__uint_as_float(*a.dmax) : 0.0;
     int c = ch0 + wave;
     int4 d = c < ch1 ? a.chunk[c] : make_int4(0, 0, 0, 0);
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 q = screen && c < ch1 ? a.chunkq[c] : z4;
     while (c < ch1) {
         const int cn = c + 4;
         const int4 dn = cn < ch1 ? a.chunk[cn] : make_int4(0, 0, 0, 0);
+        const float4 qn = screen && cn < ch1 ? a.chunkq[cn] : z4;
         const long long off = (long long)(unsigned)d.x | ((long long)d.y << 32);
         const int aoB = d.z & 0x7fff, LB = (d.z >> 15) & 1, ck = (d.z >> 16) & 3, nk = (d.z >> 18) & 0x3ff;
+        double thr = 0.0;
+        if (screen) {
+            // |integral| <= q_row q_partner, partners in descending order: a
+            // chunk whose first partner is below the screen drops out whole,
+            // one whose last is above it runs whole, and only a chunk that
+            // straddles it tests its partners one by one (thr > 0)
+            if ((double)q.x * dmax < a.qtau) {
+                c = cn;
+                d = dn;
+                q = qn;
+                continue;
+            }
+            if ((double)q.y * dmax < a.qtau) thr = a.qtau / ((double)q.z * dmax);
+        }
         const int role = (d.z >> 28) & 7;   // 1: bra role, 2: forward orientation, 4: diagonal quartet
         // weights of the three-copy split (G = A + A^T): the bra pair's two
         // orientations share its J and each adds its K rows, the ket pair's
@@ -219,10 +249,11 @@ __global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
         const double wJ = a.jscale * ((role & 1) ? ((role & 2) ? half_same : 0.0) : half_same);
         const double wK = (role & 1) ? ((role & 4) ? 0.5 : 1.0) : 0.0;
         const TV *drow = s_drow + (size_t)(aoB - row0) * n;
-        if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK);
-        else digest_chunk_la<1>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK);
+        if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK, thr);
+        else digest_chunk_la<1>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK, thr);
         c = cn;
         d = dn;
+        q = qn;
     }
     __syncthreads();
     double *dst = a.partial + (size_t)slot * 3 * n;
@@ -266,14 +297,25 @@ __global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y
         Z[i] = X[i] + Y[i];
 }
 
-// Ds = scale (D + D^T)
+// Ds = scale (D + D^T); with amax, also max |Ds32| into *amax as the bit
+// pattern of a non-negative float (ordered like the value; *amax starts at 0)
 __global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n,
-                             double scale) {
+                             double scale, unsigned *__restrict__ amax = nullptr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    const double v = scale * (D[i + (size_t)n * j] + D[j + (size_t)n * i]);
-    if (Ds) Ds[i + (size_t)n * j] = v;
-    if (Ds32) Ds32[i + (size_t)n * j] = (float)v;
+    float m = 0.0f;
+    if (i < n) {
+        const double v = scale * (D[i + (size_t)n * j] + D[j + (size_t)n * i]);
+        if (Ds) Ds[i + (size_t)n * j] = v;
+        if (Ds32) {
+            Ds32[i + (size_t)n * j] = (float)v;
+            m = fabsf((float)v);
+        }
+    }
+    if (amax) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(m));
+    }
 }
 
 
@@ -290,13 +332,14 @@ int ensure_work(myqc_ctx *ctx) {
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_G, sizeof(double) * nn));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_D32, sizeof(float) * nn));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_A32, sizeof(float) * nn));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_dmax, sizeof(double)));
     ctx->work_n = ctx->n;
     return 0;
 }
 
 template <typename TV>
 static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double *partial, double *Graw,
-                         double jscale = 1.0) {
+                         double jscale = 1.0, double qtau = 0.0) {
     const int np0 = (int)ctx->pair_ids[0].size(), np1 = (int)ctx->pair_ids[1].size(),
               np2 = (int)ctx->pair_ids[2].size();
     TV *dk0 = dk, *dk1 = dk + np0, *dk2 = dk + np0 + 3 * (size_t)np1;
@@ -325,6 +368,12 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
         a.partial = partial;
         a.n = ctx->n;
         a.jscale = jscale;
+        a.q0 = ctx->d_qrank[0];
+        a.q1 = ctx->d_qrank[1];
+        a.q2 = ctx->d_qrank[2];
+        a.chunkq = reinterpret_cast<const float4 *>(ctx->d_chunkq);
+        a.dmax = reinterpret_cast<const unsigned *>(ctx->d_dmax);
+        a.qtau = qtau;
         const size_t shm = (3 * sizeof(double) + ctx->dig_rows * sizeof(TV)) * (size_t)ctx->n;
         // timed alone: algorithmic bytes per pass are one read of every
         // stored unique integral plus the density in and G out (16 n^2)
@@ -339,8 +388,15 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
     return 0;
 }
 
+// The incremental (FP32) passes skip a partner when every integral it forms
+// with the row pair, times max |dD|, is below this (|integral| <= Q_row
+// Q_partner); the full FP64 builds, which decide convergence and the final
+// energy, never screen.
+constexpr double kIncrementalScreen = 1e-12;
+
 // G(D) into d_Gout (precision FP64), or d_Gout += G32(D) (precision FP32:
-// D is then the density change)
+// D is then the density change; partners below the incremental screen drop
+// out)
 int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) {
     if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "no stored integrals (call myqc_eri_build)");
     MYQC_CHECK(ensure_work(ctx));
@@ -354,8 +410,12 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, nullptr, n, 1.0);
     } else {
         if (!ctx->have_eri32) return set_error(ctx, MYQC_E_STATE, "no FP32 integrals stored");
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n, 0.5);
-        MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial, ctx->d_A));
+        // max |dD| of the pass for the screen, folded into the symmetrisation
+        MYQC_HIP(ctx, hipMemsetAsync(ctx->d_dmax, 0, sizeof(unsigned), ctx->stream));
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n, 0.5,
+                                                        reinterpret_cast<unsigned *>(ctx->d_dmax));
+        MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial, ctx->d_A, 1.0,
+                                        kIncrementalScreen));
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n, 1.0);
         k_add<<<256, 256, 0, ctx->stream>>>(d_Gout, ctx->d_G, d_Gout, nn);
     }
