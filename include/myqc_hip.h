@@ -70,7 +70,9 @@ int myqc_eri_build(myqc_ctx *ctx, double schwarz_tol, long long *n_stored);
 
 /* RHFI2G drop-in: G(i,j) = sum_kl D(k,l)[(ij|kl) - 1/4 (ik|jl) - 1/4 (il|jk)]
  * (src/I2G/RHFI2G.f90:80-90) from the integrals of myqc_eri_build.
- * precision MYQC_FP64 or MYQC_FP32. */
+ * precision MYQC_FP64, or MYQC_FP32: the incremental form for a density
+ * change D, over FP32 copies of the integrals, skipping integrals whose
+ * Schwarz bound times max|D| is below 1e-12. */
 int myqc_fock_G(myqc_ctx *ctx, const double *D, double *G, int precision);
 
 /* UHFI2G (src/I2G/UHFI2G.f90:80-93): the two spin Fock contributions
