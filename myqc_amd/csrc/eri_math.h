@@ -74,6 +74,10 @@ MYQC_HD double rsqrt_hd(double x) {
 // ft is the table transposed to ftT[Tk*23 + m].
 template <int M>
 MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&F)[M + 1]) {
+    // e^{-T} feeds the downward recursion below T = 12 (when Q > 0) and the
+    // asymptotic form up to 2Q + 36; one evaluation serves both branches (the
+    // lanes of a wavefront usually fall on both sides of T = 12)
+    const double e = (Q > 0 || T >= 12.0) && T < 2.0 * Q + 36.0 ? exp(-T) : 0.0;
     if (T < 12.0) {
         int Tk = (int)floor(T * 10.0 + 0.5);          // NINT, T >= 0
         double x = Tk / 10.0 - T;
@@ -86,10 +90,9 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
             fq += r[k] * xk * inv_fact[k];
             xk *= x;
         }
-        // e^{-T} only feeds the downward recursion from F_Q: with Q = 0 (sets
-        // of s functions only) there is none, F_0 is the Taylor sum itself
+        // with Q = 0 (sets of s functions only) there is no recursion: F_0 is
+        // the Taylor sum itself
         const int top = Q < M ? Q : M;
-        const double e = Q > 0 ? exp(-T) : 0.0;
         double t2 = 2.0 * T;
         // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160),
         // with 1/(2j+1) as constants; the reference divides (rounding-level difference)
@@ -120,7 +123,6 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
             else if (T < 18.0) g = 0.4998436875 - 0.24249438 * iT + 0.24642845 * iT * iT;
             else if (T < 24.0) g = 0.499093162 - 0.2152832 * iT;
             else g = 0.490;
-            double e = exp(-T);
             F[0] = half_sqrt_pi * rt - e * g * iT;
 #pragma unroll
             for (int j = 1; j <= M; ++j) F[j] = it2 * ((2.0 * (j - 1) + 1.0) * F[j - 1] - e);
