@@ -405,11 +405,14 @@ static void launch_class(myqc_ctx *ctx, int c, double *XX) {
     }
 }
 
-// Static FP64 operation count per primitive quartet and per contracted
-// quartet of each class (FMA = 2), for the roofline's algorithmic flops.
-// Counted from the recursions in eri_math.h; see eri_opcount() in the tests.
-static const double kPrimFlops[6] = {70, 95, 160, 165, 290, 540};
-static const double kContrFlops[6] = {2, 6, 30, 30, 110, 400};
+// FP64 operations per contracted shell quartet of each class (FMA = 2,
+// transcendental = 1), for the roofline's flops: the FP64 VALU instruction
+// counters (ADD, MUL, FMA, TRANS _F64, each x 64 lanes) of one build of the
+// 40-water STO-3G cluster at Schwarz 1e-12, divided by the class's quartet
+// count.  They include lanes that sit idle while a neighbour computes a
+// primitive quartet that its own pair screen drops, and both sides of a
+// divergent Boys branch, so they count issued rather than strictly useful work.
+static const double kQuartetFlops[6] = {5218.4, 5405.6, 7180.6, 7406.0, 12303.6, 33887.6};
 
 static int build_classes(myqc_ctx *ctx, double tol) {
     std::vector<long long> cum;
@@ -433,8 +436,10 @@ static int build_classes(myqc_ctx *ctx, double tol) {
                                      ctx->stream));
         ctx->n_quartets += qc.nq;
         ctx->n_stored += qc.nq * qc.blk;
-        // every STO-3G shell pair has 9 primitive pairs
-        ctx->eri_flops += (double)qc.nq * (81.0 * kPrimFlops[c] + kContrFlops[c]);
+        ctx->eri_flops += (double)qc.nq * kQuartetFlops[c];
+        if (getenv("MYQC_HOST_TIMING"))
+            fprintf(stderr, "[host] eri class (%d,%d): %lld quartets, %.6e flops by the table\n", qc.cb, qc.ck, qc.nq,
+                    (double)qc.nq * kQuartetFlops[c]);
     }
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
