@@ -54,6 +54,11 @@ extern "C" int myqc_finalize(myqc_ctx *ctx) {
         hipEventDestroy(e.a);
         hipEventDestroy(e.b);
     }
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->side[k]) hipStreamDestroy(ctx->side[k]);
+        if (ctx->join_ev[k]) hipEventDestroy(ctx->join_ev[k]);
+    }
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -46,6 +46,10 @@ struct EventRec {
 struct myqc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // two side streams (default priority) and their fork/join events: the
+    // ERI classes with few waves run beside the large ones
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
     rocblas_handle blas = nullptr;
     std::string err;
 

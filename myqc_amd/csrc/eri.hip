@@ -367,7 +367,7 @@ static void class_counts(myqc_ctx *ctx, int cb, int ck, double tol, std::vector<
 }
 
 template <int CB, int CK, int MODE>
-static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX) {
+static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX, hipStream_t s) {
     EriArgs a;
     a.pairs = ctx->d_pairs;
     a.prims = ctx->d_prims;
@@ -388,20 +388,20 @@ static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX) {
     a.XX = XX;
     a.n = ctx->n;
     const long long nblk = (qc.nq + 255) / 256;
-    k_eri<CB, CK, MODE><<<(unsigned)nblk, 256, 0, ctx->stream>>>(a);
+    k_eri<CB, CK, MODE><<<(unsigned)nblk, 256, 0, s>>>(a);
 }
 
 template <int MODE>
-static void launch_class(myqc_ctx *ctx, int c, double *XX) {
+static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s) {
     const QuartetClass &qc = ctx->cls[c];
     if (qc.nq == 0) return;
     switch (c) {
-        case 0: launch_eri<0, 0, MODE>(ctx, qc, XX); break;
-        case 1: launch_eri<1, 0, MODE>(ctx, qc, XX); break;
-        case 2: launch_eri<1, 1, MODE>(ctx, qc, XX); break;
-        case 3: launch_eri<2, 0, MODE>(ctx, qc, XX); break;
-        case 4: launch_eri<2, 1, MODE>(ctx, qc, XX); break;
-        case 5: launch_eri<2, 2, MODE>(ctx, qc, XX); break;
+        case 0: launch_eri<0, 0, MODE>(ctx, qc, XX, s); break;
+        case 1: launch_eri<1, 0, MODE>(ctx, qc, XX, s); break;
+        case 2: launch_eri<1, 1, MODE>(ctx, qc, XX, s); break;
+        case 3: launch_eri<2, 0, MODE>(ctx, qc, XX, s); break;
+        case 4: launch_eri<2, 1, MODE>(ctx, qc, XX, s); break;
+        case 5: launch_eri<2, 2, MODE>(ctx, qc, XX, s); break;
     }
 }
 
@@ -729,6 +729,17 @@ static int build_schedule(myqc_ctx *ctx) {
     return 0;
 }
 
+// the side streams and fork/join events, created on first use
+static int ensure_side_streams(myqc_ctx *ctx) {
+    if (ctx->fork_ev) return 0;
+    for (int k = 0; k < 2; ++k) {
+        MYQC_HIP(ctx, hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+        MYQC_HIP(ctx, hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+    }
+    MYQC_HIP(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    return 0;
+}
+
 int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     HostClock hc(ctx);
     MYQC_CHECK(eri_schwarz(ctx));
@@ -744,12 +755,24 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     else dfree((void **)&ctx->d_x32);   // a kept buffer may be smaller than this build
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair, sizeof(double) * 9 * ctx->pairs.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_Dpair32, sizeof(float) * 9 * ctx->pairs.size()));
+    MYQC_CHECK(ensure_side_streams(ctx));
     hipEvent_t ev;
     MYQC_CHECK(timer_begin(ctx, T_ERI, &ev));
-    for (int c = 0; c < 6; ++c) {
+    // the classes write disjoint blocks of the store, so they may overlap:
+    // the L-bra classes (few waves, long ones) start first on side stream 0,
+    // (Ls|Ls) on side stream 1, the two large s-ket classes on the context
+    // stream; the context stream then waits for both side streams
+    MYQC_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+    for (int k = 0; k < 2; ++k) MYQC_HIP(ctx, hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
+    static const int kOrder[6] = {5, 4, 3, 2, 0, 1};
+    for (int c : kOrder) {
         if (ctx->cls[c].nq == 0) continue;
-        launch_class<0>(ctx, c, nullptr);
+        launch_class<0>(ctx, c, nullptr, c >= 3 ? ctx->side[0] : c == 2 ? ctx->side[1] : ctx->stream);
         MYQC_HIP(ctx, hipGetLastError());
+    }
+    for (int k = 0; k < 2; ++k) {
+        MYQC_HIP(ctx, hipEventRecord(ctx->join_ev[k], ctx->side[k]));
+        MYQC_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev[k], 0));
     }
     MYQC_CHECK(timer_end(ctx, T_ERI, ev, ctx->eri_flops));
     hc.mark("eri_store: allocations and launches");
@@ -782,7 +805,7 @@ int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX) {
         hipEvent_t ev;
         rc = timer_begin(ctx, T_ERI, &ev);
         for (int c = 0; c < 6 && rc == 0; ++c) {
-            launch_class<1>(ctx, c, d_XX);
+            launch_class<1>(ctx, c, d_XX, ctx->stream);
             if (hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "ERI dense-fill launch failed");
         }
         if (rc == 0) rc = timer_end(ctx, T_ERI, ev, ctx->eri_flops);
