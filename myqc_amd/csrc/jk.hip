@@ -299,7 +299,7 @@ __global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y
 
 // Ds = scale (D + D^T); with amax, also max |Ds32| into *amax as the bit
 // pattern of a non-negative float (ordered like the value; *amax starts at 0)
-__global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n,
+__global__ __launch_bounds__(128) void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n,
                              double scale, unsigned *__restrict__ amax = nullptr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     float m = 0.0f;
@@ -312,9 +312,18 @@ __global__ void k_symmetrize(const double *__restrict__ D, double *__restrict__ 
         }
     }
     if (amax) {
+        // one atomic per workgroup at most, and none once the stored maximum
+        // is already as large (the stored value only grows, so a stale read
+        // can only cost an unneeded atomic)
+        __shared__ float s_m[2];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-        if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(m));
+        if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned b = __float_as_uint(fmaxf(s_m[0], s_m[1]));
+            if (b > __atomic_load_n(amax, __ATOMIC_RELAXED)) atomicMax(amax, b);
+        }
     }
 }
 

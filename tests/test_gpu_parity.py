@@ -142,6 +142,25 @@ def test_rhf_converges_where_reference_loop_oscillates(ctx, name):
     assert np.max(np.abs(comm)) <= 1e-6
 
 
+@pytest.mark.parametrize("name", ["co2", "h2o_4", "mol3"])
+@pytest.mark.parametrize("eps", [1e-2, 1e-5, 1e-8])
+def test_fp32_screened_pass_on_small_densities(ctx, name, eps):
+    """The FP32 pass skips a partner when q_row q_partner max|D| < 1e-12
+    (Schwarz factors q), max|D| being found on the device while D is
+    symmetrised.  G is linear in D, so G32(eps D) must equal eps G(D) of the
+    reference to FP32 rounding plus at most 1e-12 per dropped partner (n^2/2
+    partners bound the count)."""
+    f, _ = _setup(ctx, name)
+    ctx.eri_build(1e-12)
+    n = int(f["n"])
+    G32 = ctx.fock_G(eps * f["D_guess"], FP32)
+    ref = eps * f["G_guess"]
+    tol = 1e-4 * np.max(np.abs(ref)) + 1e-12 * n * n / 2
+    err = np.max(np.abs(G32 - ref))
+    print(f"[screen {name} eps={eps:g}] max|G32 - eps G| = {err:.2e} (tol {tol:.2e})")
+    assert err <= tol
+
+
 @pytest.mark.parametrize("name", ["co2", "h2o_2", "h2o_4"])
 def test_rhf_mixed_precision_energy(ctx, name):
     f, Z = _setup(ctx, name)
