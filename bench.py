@@ -8,8 +8,8 @@ the SCF to |dE| < 1e-10 Eh.  Forms: plain FP64 digestion; mixed
 (incremental FP32 dD Fock builds with FP64 rebuilds and an FP64 finish); and
 both again from the learned guess (the trained network's density, computed on
 the device inside the timed solve, myqc_amd/data/guess_mlp.npz); and the
-learned + mixed form batched, over 8 seeded clusters of the same size solved
-side by side (one context and HIP stream each; one step = one batch of 8).
+learned + mixed form batched, over 16 seeded clusters of the same size solved
+side by side (one context and HIP stream each; one step = one batch of 16).
 `value` is the rate of the fastest form whose energies all agree with plain
 FP64 to 1e-8 Eh.
 Extra legs: RHF-MP2 of 24 waters (ao2mo FP64 vs split-bf16) and the
@@ -297,7 +297,7 @@ def main():
                     help="comma list of fp64, mixed, learned, learned_mixed")
     ap.add_argument("--mp2-waters", type=int, default=24, help="water cluster of the MP2 leg (0: skip)")
     ap.add_argument("--batch", type=int, default=4096, help="molecules in the batched leg (0: skip)")
-    ap.add_argument("--concurrent", type=int, default=8,
+    ap.add_argument("--concurrent", type=int, default=16,
                     help="clusters solved side by side in the concurrent learned+mixed form (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run one device copy of MB MiB (scale for HBM counters)")
