@@ -80,7 +80,11 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
     const double e = (Q > 0 || T >= 12.0) && T < 2.0 * Q + 36.0 ? exp(-T) : 0.0;
     if (T < 12.0) {
         int Tk = (int)floor(T * 10.0 + 0.5);          // NINT, T >= 0
-        double x = Tk / 10.0 - T;
+        // Tk / 10 without a division: the product and one fma correction
+        // equal the correctly rounded quotient for every integer 0..1000
+        // (checked exhaustively; Tk <= 120 here)
+        const double tkd = (double)Tk, q10 = tkd * 0.1;
+        double x = fma(fma(-q10, 10.0, tkd), 0.1, q10) - T;
         const double *r = ftT + Tk * FT_NM + Q;
         // sum_{k=0..6} Ft(Tk,Q+k) x^k / k!   (auxilary.f90:152-155)
         double xk = 1.0, fq = 0.0;
@@ -97,10 +101,19 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
         // downward recursion F_j = (2T F_{j+1} + e^{-T}) / (2j+1)  (auxilary.f90:158-160),
         // with 1/(2j+1) as constants; the reference divides (rounding-level difference)
         // Q <= 12: sets have max L <= 1 (checked when the system is built), so
-        // Q = 3 (sum of four set L's) <= 12
+        // Q = 3 (sum of four set L's) <= 12.  Q is a multiple of 3, so the
+        // steps j = Q-1 .. M run in blocks of three (j = 3b+2, 3b+1, 3b for
+        // Q > 3b), each behind one branch that a wavefront skips when none of
+        // its lanes needs it
 #pragma unroll
-        for (int j = 11; j >= 0; --j)
-            if (j >= M && j < Q) fq = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
+        for (int b = 3; b >= 0; --b) {
+            if (3 * b + 2 < M) continue;
+            if (Q > 3 * b) {
+#pragma unroll
+                for (int j = 3 * b + 2; j >= 3 * b; --j)
+                    if (j >= M) fq = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
+            }
+        }
         // fq now holds F_top, top = min(Q, M); the reference never forms F_j for
         // j > Q, and callers only read F_j for j <= Q
 #pragma unroll

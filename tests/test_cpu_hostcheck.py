@@ -145,3 +145,19 @@ def test_three_copy_digestion_formula(hc, name):
         assert hc.hc_fock_G_3copy(*s.args(), Df.ctypes.data_as(_dp), G.ctypes.data_as(_dp)) == 0
         assert np.max(np.abs(G.reshape(n, n, order="F") - ref)) <= 1e-11
 
+
+
+def test_boys_grid_point_without_division():
+    """The Boys expansion point Tk/10 is formed as q = Tk*0.1 corrected by one
+    fma (no division); for every integer 0..1000 that equals the correctly
+    rounded quotient.  fma is emulated exactly with rationals (float() of a
+    Fraction rounds to nearest even, as IEEE fma does)."""
+    from fractions import Fraction
+
+    def fma(a, b, c):
+        return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+    for tk in range(1001):
+        x = float(tk)
+        q = x * 0.1
+        assert fma(fma(-q, 10.0, x), 0.1, q) == x / 10.0, tk
