@@ -412,7 +412,7 @@ static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s) {
 // count.  They include lanes that sit idle while a neighbour computes a
 // primitive quartet that its own pair screen drops, and both sides of a
 // divergent Boys branch, so they count issued rather than strictly useful work.
-static const double kQuartetFlops[6] = {5218.4, 5405.6, 7180.6, 7406.0, 12303.6, 33887.6};
+static const double kQuartetFlops[6] = {4806.7, 5034.7, 6817.8, 7016.4, 11940.1, 33434.3};
 
 static int build_classes(myqc_ctx *ctx, double tol) {
     std::vector<long long> cum;

@@ -70,6 +70,19 @@ MYQC_HD double rsqrt_hd(double x) {
 #endif
 }
 
+// 1/x for x > 0 in the normal range: the device's reciprocal estimate refined
+// by two Newton-Raphson steps (within about an ulp of the quotient, without
+// the full division sequence), or the host's division
+MYQC_HD double recip_hd(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // Boys F_0..F_M(T) as the reference computes them at order Q (M <= Q <= 16).
 // ft is the table transposed to ftT[Tk*23 + m].
 template <int M>
@@ -205,7 +218,7 @@ MYQC_HD void quartet_prims_f(const ShellPair &bra, const ShellPair &ket, const P
             const PrimPair k = kprim(j);
             if (SCREEN && b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
             const double p = b.p, q = k.p, pq = p + q;
-            const double ipq = 1.0 / pq;
+            const double ipq = recip_hd(pq);
             const double rho = p * q * ipq;
             const double rsq = rsqrt_hd(pq);
             double PQ[3], WP[3], WQ[3], QC[3];
