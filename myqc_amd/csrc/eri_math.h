@@ -212,10 +212,13 @@ MYQC_HD void quartet_prims_f(const ShellPair &bra, const ShellPair &ket, const P
 #pragma unroll
         for (int f = 0; f < A::NF; ++f) acc.v[e][f] = 0.0;
     const double c0 = 2.0 * pow(PI_SP, 2.5);
-    for (int i = 0; i < bra.npp; ++i) {
-        const PrimPair b = ppb[bra.pp0 + i];
-        for (int j = 0; j < ket.npp; ++j) {
-            const PrimPair k = kprim(j);
+    // ket primitives outer: a lane's own ket data is loaded once per quartet,
+    // and the inner loop reads the bra's primitives, whose addresses the lanes
+    // of a wavefront (one bra pair, consecutive kets) mostly share
+    for (int j = 0; j < ket.npp; ++j) {
+        const PrimPair k = kprim(j);
+        for (int i = 0; i < bra.npp; ++i) {
+            const PrimPair b = ppb[bra.pp0 + i];
             if (SCREEN && b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
             const double p = b.p, q = k.p, pq = p + q;
             const double ipq = recip_hd(pq);
