@@ -168,9 +168,22 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     long long rk;
     find_quartet(a.cum, a.nbra, t, a.same_class, rb, rk);
     const int pb = a.sorted_b[rb], pk = a.sorted_k[rk];
-    const ShellPair bra = a.pairs[pb], ket = a.pairs[pk];
+    const ShellPair ket = a.pairs[pk];
     QuartetAcc<LA, LB, LC, LD> acc;
-    quartet_prims_f<LA, LB, LC, LD>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, rk}, s_ft, acc);
+    // A wavefront whose lanes share the bra pair (most do: its lanes take
+    // consecutive kets of one bra) reads the bra's pair and primitive data
+    // through scalar loads; the classes with few kets per bra, and waves that
+    // straddle two bras, load them per lane.
+    constexpr bool kUniformPath = CB + CK <= 2;
+    const int pb0 = __builtin_amdgcn_readfirstlane(pb);
+    ShellPair bra;
+    if (kUniformPath && __all(pb == pb0)) {
+        bra = a.pairs[pb0];
+        quartet_prims_f<LA, LB, LC, LD>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, rk}, s_ft, acc);
+    } else {
+        bra = a.pairs[pb];
+        quartet_prims_f<LA, LB, LC, LD>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, rk}, s_ft, acc);
+    }
     double CD[3] = {ket.AB[0], ket.AB[1], ket.AB[2]};
     double out[BLK];
     quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, CD, out);
