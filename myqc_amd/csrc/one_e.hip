@@ -10,20 +10,38 @@
 
 namespace myqc {
 
-__global__ __launch_bounds__(128) void k_one_e(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
+__global__ __launch_bounds__(256) void k_one_e(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
                                                const double *__restrict__ ftT, const double *__restrict__ atoms,
                                                int nnuc, int npairs, int n, double *__restrict__ S, double *__restrict__ H) {
     __shared__ double s_ft[FT_NT * FT_NM];
     for (int i = threadIdx.x; i < FT_NT * FT_NM; i += blockDim.x) s_ft[i] = ftT[i];
     __syncthreads();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs) return;
+    // one wavefront per shell pair: lane l takes the nuclei l, l + 64, ... of
+    // the attraction integrals and the primitive pairs l, l + 64, ... of S and
+    // T, then the lanes' sums are added across the wavefront
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= npairs) return;   // whole wavefronts
     const ShellPair sp = pairs[t];
     double s[3][3], h[3][3];
-    one_e_pair(sp, prims, s_ft, atoms, nnuc, s, h);
+    one_e_pair(sp, prims, s_ft, atoms, nnuc, s, h, lane, 64);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                h[a][b] += __shfl_xor(h[a][b], off, 64);
+                s[a][b] += __shfl_xor(s[a][b], off, 64);
+            }
+    if (lane != 0) return;
     const int na = sp.la ? 3 : 1, nb = sp.lb ? 3 : 1;
-    for (int a = 0; a < na; ++a)
-        for (int b = 0; b < nb; ++b) {
+    // constant indices into s and h (registers), the sizes as guards
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (a >= na || b >= nb) continue;
             const size_t i = sp.ao_a + a, j = sp.ao_b + b;
             S[i + n * j] = s[a][b];
             S[j + n * i] = s[a][b];
@@ -139,8 +157,8 @@ extern "C" int myqc_int1e(myqc_ctx *ctx, double *S, double *H) {
                                   ctx->stream) != hipSuccess)
         rc = set_error(ctx, MYQC_E_HIP, "upload failed");
     if (rc == 0) {
-        k_one_e<<<(np + 127) / 128, 128, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_prims, ctx->d_ftT, dA, ctx->nnuc, np,
-                                                          n, dS, dH);
+        k_one_e<<<(np + 3) / 4, 256, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_prims, ctx->d_ftT, dA, ctx->nnuc, np, n,
+                                                      dS, dH);
         if (hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "one-electron kernel launch failed");
     }
     if (rc == 0 && (d2h(ctx, S, dS, sizeof(double) * nn) != hipSuccess ||

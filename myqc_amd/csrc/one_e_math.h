@@ -65,10 +65,17 @@ MYQC_HD void dipole_pair(const ShellPair &sp, const PrimPair *__restrict__ prims
     }
 }
 
-// S, H = T + V contributions of one shell pair; atoms[4c..4c+3] = Z, x, y, z
-MYQC_HD void one_e_pair(const ShellPair &sp, const PrimPair *__restrict__ prims, const double *__restrict__ ftT,
-                        const double *__restrict__ atoms, int nnuc, double s[3][3], double h[3][3]) {
-    const int na = sp.la ? 3 : 1, nb = sp.lb ? 3 : 1;
+// S, H = T + V contributions of one shell pair; atoms[4c..4c+3] = Z, x, y, z.
+// Only the nuclei c0, c0 + cstep, ... enter V, and only the primitive pairs
+// c0, c0 + cstep, ... enter S and T, so that cstep callers (the lanes of a
+// wavefront) summing their s and h cover the pair once; the defaults take
+// everything.
+// LA, LB are the pair's angular momenta (the first shell carries the larger).
+template <int LA, int LB>
+MYQC_HD void one_e_pair_t(const ShellPair &sp, const PrimPair *__restrict__ prims, const double *__restrict__ ftT,
+                          const double *__restrict__ atoms, int nnuc, double s[3][3], double h[3][3], int c0,
+                          int cstep) {
+    constexpr int na = LA ? 3 : 1, nb = LB ? 3 : 1;
     double kin[3][3], v[3][3];
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) s[a][b] = kin[a][b] = v[a][b] = 0.0;
@@ -80,25 +87,31 @@ MYQC_HD void one_e_pair(const ShellPair &sp, const PrimPair *__restrict__ prims,
         double PB[3];
         for (int d = 0; d < 3; ++d) PB[d] = pr.PA[d] + sp.AB[d];   // P - B = (P - A) + (A - B)
         const double beta = pr.beta;
-        const double pref = pow(PI_SP / p, 1.5) * pr.K;
-        for (int a = 0; a < na; ++a)
-            for (int b = 0; b < nb; ++b) {
-                int la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
-                if (sp.la) la[a] = 1;
-                if (sp.lb) lb[b] = 1;
-                double Ex[3], kx[3];
-                for (int d = 0; d < 3; ++d) {
-                    const int i = la[d], j = lb[d];
-                    Ex[d] = e1d(i, j, pr.PA[d], PB[d], p);
-                    // int1e.f90:442-462
-                    kx[d] = j * (j - 1) * e1d(i, j - 2, pr.PA[d], PB[d], p) - 2.0 * beta * j * Ex[d] -
-                            2.0 * beta * (j + 1) * Ex[d] + 4.0 * beta * beta * e1d(i, j + 2, pr.PA[d], PB[d], p);
+        const bool st = ip % cstep == c0;
+        const double pref = st ? pow(PI_SP / p, 1.5) * pr.K : 0.0;
+        if (st) {
+#pragma unroll
+            for (int a = 0; a < na; ++a)
+#pragma unroll
+                for (int b = 0; b < nb; ++b) {
+                    int la[3] = {0, 0, 0}, lb[3] = {0, 0, 0};
+                    if (LA) la[a] = 1;
+                    if (LB) lb[b] = 1;
+                    double Ex[3], kx[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int i = la[d], j = lb[d];
+                        Ex[d] = e1d(i, j, pr.PA[d], PB[d], p);
+                        // int1e.f90:442-462
+                        kx[d] = j * (j - 1) * e1d(i, j - 2, pr.PA[d], PB[d], p) - 2.0 * beta * j * Ex[d] -
+                                2.0 * beta * (j + 1) * Ex[d] + 4.0 * beta * beta * e1d(i, j + 2, pr.PA[d], PB[d], p);
+                    }
+                    s[a][b] += pref * Ex[0] * Ex[1] * Ex[2];
+                    kin[a][b] += -0.5 * pref * (kx[0] * Ex[1] * Ex[2] + Ex[0] * kx[1] * Ex[2] + Ex[0] * Ex[1] * kx[2]);
                 }
-                s[a][b] += pref * Ex[0] * Ex[1] * Ex[2];
-                kin[a][b] += -0.5 * pref * (kx[0] * Ex[1] * Ex[2] + Ex[0] * kx[1] * Ex[2] + Ex[0] * Ex[1] * kx[2]);
-            }
+        }
         // nuclear attraction: [a|V|s]^(m) for a up to level la+lb, then HRR
-        for (int c = 0; c < nnuc; ++c) {
+        for (int c = c0; c < nnuc; c += cstep) {
             const double Zc = atoms[4 * c];
             double PC[3];
             for (int d = 0; d < 3; ++d) PC[d] = pr.P[d] - atoms[4 * c + 1 + d];
@@ -112,12 +125,15 @@ MYQC_HD void one_e_pair(const ShellPair &sp, const PrimPair *__restrict__ prims,
             for (int i = 0; i < 3; ++i)
                 for (int m = 0; m < 2; ++m) v1[i][m] = pr.PA[i] * v0[m] - PC[i] * v0[m + 1];
             const double hh = 0.5 / p;
-            if (!sp.la && !sp.lb) {
+            if constexpr (!LA && !LB) {
                 v[0][0] += v0[0];
-            } else if (sp.la && !sp.lb) {
+            } else if constexpr (LA && !LB) {
+#pragma unroll
                 for (int a = 0; a < 3; ++a) v[a][0] += v1[a][0];
             } else {
+#pragma unroll
                 for (int a = 0; a < 3; ++a)
+#pragma unroll
                     for (int b = 0; b < 3; ++b) {
                         // [p_a + 1_b|s] then (p_a|p_b) = [p_a+1_b| + AB_b [p_a|
                         const double d2 = pr.PA[b] * v1[a][0] - PC[b] * v1[a][1] + (a == b ? hh * (v0[0] - v0[1]) : 0.0);
@@ -128,6 +144,15 @@ MYQC_HD void one_e_pair(const ShellPair &sp, const PrimPair *__restrict__ prims,
     }
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) h[a][b] = kin[a][b] + v[a][b];
+}
+
+// S, H of one shell pair with runtime angular momenta (first shell larger)
+MYQC_HD void one_e_pair(const ShellPair &sp, const PrimPair *__restrict__ prims, const double *__restrict__ ftT,
+                        const double *__restrict__ atoms, int nnuc, double s[3][3], double h[3][3], int c0 = 0,
+                        int cstep = 1) {
+    if (sp.la && sp.lb) one_e_pair_t<1, 1>(sp, prims, ftT, atoms, nnuc, s, h, c0, cstep);
+    else if (sp.la) one_e_pair_t<1, 0>(sp, prims, ftT, atoms, nnuc, s, h, c0, cstep);
+    else one_e_pair_t<0, 0>(sp, prims, ftT, atoms, nnuc, s, h, c0, cstep);
 }
 
 }  // namespace myqc
