@@ -425,7 +425,7 @@ static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s) {
 // count.  They include lanes that sit idle while a neighbour computes a
 // primitive quartet that its own pair screen drops, and both sides of a
 // divergent Boys branch, so they count issued rather than strictly useful work.
-static const double kQuartetFlops[6] = {4806.7, 5034.7, 6817.8, 7016.4, 11940.1, 33434.3};
+static const double kQuartetFlops[6] = {4836.7, 4930.0, 6708.2, 6929.1, 11986.4, 33463.8};
 
 static int build_classes(myqc_ctx *ctx, double tol) {
     std::vector<long long> cum;
