@@ -268,7 +268,11 @@ int myqc_mlp_train(myqc_ctx *ctx, const float *X, const float *T, const float *M
  * pointers NULL only *npair is returned. */
 int myqc_batch_pair_data(myqc_ctx *ctx, int *npair, float *features, float *targets, float *mask);
 /* The network's AO density (n x n) for one molecule of H..Ne atoms, as the
- * D_guess of myqc_rhf_scf_density; xyz_bohr is (nnuc, 3) column-major. */
+ * D_guess of myqc_rhf_scf_density; xyz_bohr is (nnuc, 3) column-major.  n is
+ * the molecule's STO-3G size, 1 function per H/He and 5 per Li..Ne atom (the
+ * AO order of buildBasis).  When a system is set on the context the molecule
+ * must be that system (same nnuc and Z, so n is the context's n); otherwise
+ * MYQC_E_ARG. */
 int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const double *xyz_bohr, double *D);
 /* myqc_batch_rhf started from the network's densities, computed on the
  * device inside the same captured graph; molecules that do not converge from

@@ -105,6 +105,13 @@ static int dgemm(myqc_ctx *ctx, rocblas_operation ta, int m, int n, int k, const
     if (precision == MYQC_BF16X3)
         return gemm_bf16x3(ctx, ta == rocblas_operation_transpose, m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
     const double one = 1.0, zero = 0.0;
+    MYQC_CHECK(blas_reserve(ctx, [&] {
+        if (batch == 1)
+            (void)rocblas_dgemm(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+        else
+            (void)rocblas_dgemm_strided_batched(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, sA, B,
+                                                ldb, sB, &zero, C, ldc, sC, batch);
+    }));
     if (batch == 1)
         MYQC_BLAS(ctx, rocblas_dgemm(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc));
     else
@@ -507,3 +514,4 @@ extern "C" int myqc_mp2_uhf_records(myqc_ctx *ctx, const double *ijab_aa, const 
     ctx->have_uhf = true;
     return mp2_uhf_sums(ctx, eig_a, eig_b, e_aa, e_bb, e_ab, e_tot);
 }
+

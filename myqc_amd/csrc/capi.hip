@@ -29,6 +29,13 @@ extern "C" int myqc_init(int device, myqc_ctx **out) {
         delete ctx;
         return MYQC_E_HIP;
     }
+    // pinned staging for small reads, and a context-owned rocBLAS workspace
+    // from the start (rocBLAS then never allocates inside a call)
+    if (hipHostMalloc(&ctx->h_pin, kPinBytes, hipHostMallocDefault) != hipSuccess ||
+        blas_reserve(ctx, [] {}) != 0) {
+        myqc_finalize(ctx);
+        return MYQC_E_HIP;
+    }
     *out = ctx;
     return 0;
 }
@@ -60,6 +67,8 @@ extern "C" int myqc_finalize(myqc_ctx *ctx) {
     }
     if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+    dfree(&ctx->d_blasws);
+    if (ctx->h_pin) hipHostFree(ctx->h_pin);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;

@@ -113,8 +113,12 @@ static int cis_solve(myqc_ctx *ctx, const CisDims &d, double *const *dstreams, c
         k_cis_matrix<<<1024, 256, 0, ctx->stream>>>(A, (int)dim, (int)d.upA(), (int)d.upB(), dstreams[0], dstreams[1],
                                                     dstreams[2], dstreams[3], dstreams[4], eA, eB, d.oA, d.vA, d.oB,
                                                     d.vB);
-        if (rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, (int)dim, A, (int)dim, W, E, info) !=
-            rocblas_status_success)
+        rc = blas_reserve(ctx, [&] {
+            (void)rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, (int)dim, A, (int)dim, W, E,
+                                   info);
+        });
+        if (rc == 0 && rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, (int)dim, A, (int)dim, W,
+                                        E, info) != rocblas_status_success)
             rc = set_error(ctx, MYQC_E_HIP, "cis: rocsolver_dsyevd failed");
     }
     int h_info = 0;

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
+#include <array>
+#include <set>
 #include <cstdio>
 #include <cstdlib>
 #include <ctime>
@@ -150,8 +153,14 @@ struct myqc_ctx {
     // serialise contexts that run side by side on other streams)
     void *d_tmp[8] = {};
 
-    // capacity of each device allocation made through dalloc
-    std::unordered_map<void *, size_t> dcap;
+    // pinned host staging for small device-to-host reads (d2h)
+    void *h_pin = nullptr;
+    // the rocBLAS/rocSOLVER workspace, owned by the context and handed to the
+    // handle with rocblas_set_workspace: the library allocates nothing inside
+    // a call (blas_reserve sizes it before each new problem shape)
+    void *d_blasws = nullptr;
+    size_t blasws_bytes = 0;
+    std::set<std::array<int, 8>> blas_shapes;   // call shapes the workspace has been sized for
 
     // timing
     bool timing = false;
@@ -187,12 +196,45 @@ int set_error(myqc_ctx *ctx, int code, const char *fmt, ...);
         if (rc_ != 0) return rc_; \
     } while (0)
 
+// bytes of the pinned staging buffer (ctx->h_pin)
+constexpr size_t kPinBytes = 4096;
+
 // device-to-host copy ordered on ctx->stream, complete at return
 hipError_t d2h(myqc_ctx *ctx, void *dst, const void *src, size_t bytes);
 
-// device allocation helper (frees and reallocates to `bytes`)
+// device allocation helper: keeps a buffer whose capacity covers `bytes`,
+// otherwise frees it and allocates `bytes`
 int dalloc(myqc_ctx *ctx, void **p, size_t bytes);
 void dfree(void **p);
+// capacity of a dalloc allocation (0 for an unknown address)
+size_t dcapacity(const void *p);
+
+// Grows the context's rocBLAS workspace to what `calls` needs: the calls run
+// in rocBLAS's device-memory size-query mode (nothing executes), and the
+// workspace is reallocated and handed to the handle only when it must grow.
+template <typename F>
+int blas_reserve(myqc_ctx *ctx, F calls);
+template <typename F>
+int blas_reserve(myqc_ctx *ctx, F calls) {
+    if (rocblas_start_device_memory_size_query(ctx->blas) != rocblas_status_success)
+        return set_error(ctx, MYQC_E_HIP, "rocblas_start_device_memory_size_query failed");
+    calls();
+    size_t need = 0;
+    if (rocblas_stop_device_memory_size_query(ctx->blas, &need) != rocblas_status_success)
+        return set_error(ctx, MYQC_E_HIP, "rocblas_stop_device_memory_size_query failed");
+    if (ctx->d_blasws && need <= ctx->blasws_bytes) return 0;
+    // 1 MiB granules, at least 4 MiB; the old workspace may still be read by
+    // queued work, so the stream drains before it is replaced
+    const size_t gran = size_t(1) << 20;
+    size_t bytes = ((std::max(need, size_t(4) << 20) + gran - 1) / gran) * gran;
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MYQC_CHECK(dalloc(ctx, &ctx->d_blasws, bytes));
+    ctx->blasws_bytes = bytes;
+    if (rocblas_set_workspace(ctx->blas, ctx->d_blasws, bytes) != rocblas_status_success)
+        return set_error(ctx, MYQC_E_HIP, "rocblas_set_workspace failed");
+    return 0;
+}
+
 // slots of ctx->d_tmp
 enum TmpSlot { TMP_S = 0, TMP_H, TMP_ATOMS, TMP_GPAIRS, TMP_GATOMS, TMP_GDENS, TMP_GMOL, TMP_NSLOT };
 static_assert(TMP_NSLOT <= 8, "d_tmp slots");

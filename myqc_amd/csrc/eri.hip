@@ -17,7 +17,9 @@
 #include <cstdarg>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <thread>
+#include <unordered_map>
 
 #include "context.h"
 #include "ket_soa.h"
@@ -38,14 +40,30 @@ int set_error(myqc_ctx *ctx, int code, const char *fmt, ...) {
 // Device allocations keep their capacity: a buffer is only replaced when a
 // request outgrows it, so repeated solves of same-sized systems reuse their
 // memory instead of paying hipFree/hipMalloc (and the 1-2 GB integral store)
-// every time.
+// every time.  The capacities are kept per address in one process-wide
+// registry (contexts may run on several host threads), and dfree drops the
+// entry of the buffer it frees, so an address the allocator hands out again
+// never inherits a stale size.
+namespace {
+std::mutex g_cap_mutex;
+std::unordered_map<void *, size_t> g_cap;
+}  // namespace
+
+static size_t cap_lookup(void *p) {
+    std::lock_guard<std::mutex> lk(g_cap_mutex);
+    auto it = g_cap.find(p);
+    return it == g_cap.end() ? 0 : it->second;
+}
+static void cap_set(void *p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_cap_mutex);
+    if (bytes) g_cap[p] = bytes;
+    else g_cap.erase(p);
+}
+
 int dalloc(myqc_ctx *ctx, void **p, size_t bytes) {
     if (*p) {
-        auto it = ctx->dcap.find(*p);
-        if (bytes > 0 && it != ctx->dcap.end() && it->second >= bytes) return 0;
-        if (it != ctx->dcap.end()) ctx->dcap.erase(it);
-        (void)hipFree(*p);
-        *p = nullptr;
+        if (bytes > 0 && cap_lookup(*p) >= bytes) return 0;
+        dfree(p);
     }
     if (bytes == 0) return 0;
     hipError_t e = hipMalloc(p, bytes);
@@ -53,18 +71,32 @@ int dalloc(myqc_ctx *ctx, void **p, size_t bytes) {
         *p = nullptr;
         return set_error(ctx, MYQC_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     }
-    ctx->dcap[*p] = bytes;
+    cap_set(*p, bytes);
     return 0;
 }
 
+size_t dcapacity(const void *p) { return p ? cap_lookup(const_cast<void *>(p)) : 0; }
+
+// Small device-to-host reads (per-iteration scalars, status words) land in
+// the context's pinned staging buffer first; larger ones go straight to the
+// caller's memory.  Complete at return either way.
 hipError_t d2h(myqc_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (ctx->h_pin && bytes <= kPinBytes) {
+        hipError_t e = hipMemcpyAsync(ctx->h_pin, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) memcpy(dst, ctx->h_pin, bytes);
+        return e;
+    }
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(ctx->stream);
 }
 
 void dfree(void **p) {
-    if (*p) (void)hipFree(*p);
+    if (*p) {
+        cap_set(*p, 0);
+        (void)hipFree(*p);
+    }
     *p = nullptr;
 }
 

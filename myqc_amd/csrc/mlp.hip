@@ -609,6 +609,16 @@ extern "C" int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const d
         ao[a] = n;
         n += nf[a];
     }
+    // with a system set, the molecule must be that system, so that D has the
+    // context's basis size (a caller that sizes D by it cannot be overrun)
+    if (ctx->have_system) {
+        if (nnuc != ctx->nnuc || n != ctx->n)
+            return set_error(ctx, MYQC_E_ARG, "myqc_guess_density: molecule (%d atoms, n=%d) is not the system set (%d atoms, n=%d)",
+                             nnuc, n, ctx->nnuc, ctx->n);
+        for (int a = 0; a < nnuc; ++a)
+            if (Z[a] != ctx->Z[a])
+                return set_error(ctx, MYQC_E_ARG, "myqc_guess_density: Z[%d]=%d differs from the system set", a, Z[a]);
+    }
     std::vector<PairRec> prs;
     for (int a = 0; a < nnuc; ++a)
         for (int c = a; c < nnuc; ++c) prs.push_back(PairRec{0, a, c, ao[a], ao[c], nf[a], nf[c], n, 0});

@@ -9,6 +9,7 @@
 // decides convergence from it; the density itself comes from the MFMA
 // purification in sp2.hip.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -192,6 +193,13 @@ static int add_job(myqc_ctx *ctx, RedJobs &J, int mode, const double *x, const d
 
 static int gemm(myqc_ctx *ctx, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha,
                 const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc) {
+    // the first call of each shape sizes the workspace (no allocation inside rocBLAS)
+    const std::array<int, 8> key{0, (int)ta, (int)tb, m, n, k, lda, ldb};
+    if (!ctx->blas_shapes.count(key)) {
+        MYQC_CHECK(blas_reserve(
+            ctx, [&] { (void)rocblas_dgemm(ctx->blas, ta, tb, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc); }));
+        ctx->blas_shapes.insert(key);
+    }
     MYQC_BLAS(ctx, rocblas_dgemm(ctx->blas, ta, tb, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc));
     return 0;
 }
@@ -199,6 +207,13 @@ static const rocblas_operation NN = rocblas_operation_none, TT = rocblas_operati
 
 // eigen-decomposition of the symmetric n x n A (overwritten by eigenvectors)
 static int syevd(myqc_ctx *ctx, ScfBuffers &b, double *A, double *w, int n) {
+    const std::array<int, 8> key{1, n, 0, 0, 0, 0, 0, 0};
+    if (!ctx->blas_shapes.count(key)) {
+        MYQC_CHECK(blas_reserve(ctx, [&] {
+            (void)rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower, n, A, n, w, b.work, b.info);
+        }));
+        ctx->blas_shapes.insert(key);
+    }
     MYQC_BLAS(ctx, rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower, n, A, n, w, b.work, b.info));
     int info = 0;
     MYQC_HIP(ctx, d2h(ctx, &info, b.info, sizeof(int)));

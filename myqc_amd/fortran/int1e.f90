@@ -2,7 +2,8 @@
 ! int1e -- overlap Suv and core Hamiltonian Huv (text, column-major) on the
 ! GPU.  Program contract of src/integrals/int1e.f90: reads envdat/nucpos/
 ! fmem, mybasis, Ftab; writes basinfo/setinfo, Suv, Huv; skips if Suv and
-! Huv exist (int1e.f90:98-111).
+! Huv exist (int1e.f90:98-111).  S and H are taken from the fmem ledger
+! while held.
 !-----------------------------------------------------------------------
 PROGRAM int1e
   USE iso_c_binding
@@ -21,13 +22,16 @@ PROGRAM int1e
   CALL read_env(nnuc, nelcA, nelcB, xyz, atoms, fmem, options)
   CALL check_error_sentinel()
   CALL build_basis(options(2), atoms, bas, basinfo, setv, setinfo)
+  n = basinfo(1)
+  ! S and H from the fmem ledger (int1e.f90:81-96), before the skip check
+  CALL ledger_take(fmem, 2.0D0 * n * n * 8.0D0 / 1.0D6, "int1e", 1)
   INQUIRE(file='Suv', EXIST=ex1)
   INQUIRE(file='Huv', EXIST=ex2)
   IF (ex1 .AND. ex2) THEN
     WRITE(*,*) "Reading one electron integrals from intermediate"
+    CALL ledger_give(fmem, 2.0D0 * n * n * 8.0D0 / 1.0D6)
     STOP
   END IF
-  n = basinfo(1)
   ALLOCATE(S(n,n), H(n,n))
   CALL open_system(ctx, atoms, xyz, bas, basinfo, setv, setinfo)
   rc = myqc_int1e(ctx, S, H)
@@ -36,4 +40,5 @@ PROGRAM int1e
   CALL write_text_matrix('Suv', S)
   CALL write_text_matrix('Huv', H)
   WRITE(*,*) "Overlap and core Hamiltonian written"
+  CALL ledger_give(fmem, 2.0D0 * n * n * 8.0D0 / 1.0D6)
 END PROGRAM int1e

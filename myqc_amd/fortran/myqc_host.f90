@@ -276,4 +276,180 @@ CONTAINS
     END DO
   END FUNCTION nuclear_repulsion
 
+  ! The fmem ledger (src/myQC/env.f90:77-112, nmem): a program takes the
+  ! host memory it allocates from the free-memory figure (MB) before it
+  ! allocates, fails when the figure goes negative, and gives it back when it
+  ! is done; the file is rewritten at each step, as nmem does.
+  SUBROUTINE write_fmem(fmem)
+    REAL(8), INTENT(IN) :: fmem
+    INTEGER :: u, ios
+    OPEN(newunit=u, file='fmem', status='replace', iostat=ios)
+    IF (ios /= 0) CALL host_fail("Couldn't open fmem")
+    WRITE(u,*) fmem
+    CLOSE(u)
+  END SUBROUTINE write_fmem
+
+  ! check: 0 none (int2e only records the figure, int2e.f90:179-180), 1 fail
+  ! below zero (int1e.f90:81-85), 2 fail at or below zero (scf.f90:114-133)
+  SUBROUTINE ledger_take(fmem, mb, what, check)
+    REAL(8), INTENT(INOUT) :: fmem
+    REAL(8), INTENT(IN) :: mb
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    INTEGER, INTENT(IN) :: check
+    WRITE(*,'(1x,A,2x,F15.4)') "Allocating memory for " // what // " (MB)", mb
+    fmem = fmem - mb
+    IF ((check == 1 .AND. fmem < 0.0D0) .OR. (check == 2 .AND. fmem <= 0.0D0)) THEN
+      WRITE(*,*) "You need more memory for ", what, ": ", -fmem, " MB short"
+      CALL host_fail("not enough memory (MEMORY= / fmem) for " // what)
+    END IF
+    CALL write_fmem(fmem)
+  END SUBROUTINE ledger_take
+
+  SUBROUTINE ledger_give(fmem, mb)
+    REAL(8), INTENT(INOUT) :: fmem
+    REAL(8), INTENT(IN) :: mb
+    fmem = fmem + mb
+    CALL write_fmem(fmem)
+  END SUBROUTINE ledger_give
+
+  ! One unformatted sequential record of nbytes bytes at `buf`, in gfortran's
+  ! layout (the compiler of the reference's own build, src/Makefile:11): a
+  ! record longer than maxsub bytes (gfortran: 2147483639 = 2^31 - 9) is
+  ! split into subrecords of at most that many bytes, each framed by 4-byte
+  ! length markers; a head marker is negative when another subrecord follows,
+  ! a tail marker when one precedes.  A record that fits one subrecord is the
+  ! plain single record.
+  SUBROUTINE write_record_subrecords(fname, buf, nbytes, maxsub)
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    TYPE(c_ptr), INTENT(IN) :: buf
+    INTEGER(8), INTENT(IN) :: nbytes, maxsub
+    INTEGER(1), POINTER :: b(:)
+    INTEGER(8) :: pos, ln
+    INTEGER(4) :: head, tail
+    INTEGER :: u, ios
+    CALL c_f_pointer(buf, b, [nbytes])
+    OPEN(newunit=u, file=fname, status='replace', access='stream', form='unformatted', iostat=ios)
+    IF (ios /= 0) CALL host_fail("Couldn't open " // fname)
+    pos = 0
+    DO
+      ln = MIN(nbytes - pos, maxsub)
+      head = INT(ln, 4)
+      tail = INT(ln, 4)
+      IF (pos + ln < nbytes) head = -head
+      IF (pos > 0) tail = -tail
+      WRITE(u) head
+      IF (ln > 0) WRITE(u) b(pos+1:pos+ln)
+      WRITE(u) tail
+      pos = pos + ln
+      IF (pos >= nbytes) EXIT
+    END DO
+    CLOSE(u)
+  END SUBROUTINE write_record_subrecords
+
+  ! The MOLDEN file of makeMOLDEN (src/scf/scf.f90:1343-1466): the atoms in
+  ! bohr, the contracted GTOs of each atom as mybasis lists them (exponent,
+  ! coefficient per primitive), and the orbitals (energy, spin, occupation
+  ! 1.0 below zero energy else 0.0, coefficients); the beta set follows for
+  ! UHF (uhf = .TRUE.).  The fixed-format lines use the reference's formats.
+  SUBROUTINE write_molden(bkey, atoms, xyz, eigA, CA, uhf, eigB, CB)
+    INTEGER, INTENT(IN) :: bkey, atoms(0:)
+    REAL(8), INTENT(IN) :: xyz(0:,0:), eigA(:), CA(:,:)
+    LOGICAL, INTENT(IN) :: uhf
+    REAL(8), INTENT(IN) :: eigB(:), CB(:,:)
+    CHARACTER(LEN=2), PARAMETER :: anames(1:10) = ['H ', 'He', 'Li', 'Be', 'B ', 'C ', 'N ', 'O ', 'F ', 'Ne']
+    CHARACTER(LEN=8), PARAMETER :: bnames(0:3) = ['STO-3G  ', 'tester1 ', 'tester2 ', 'tester3 ']
+    CHARACTER(LEN=1), PARAMETER :: lnames(0:5) = ['s', 'p', 'd', 'f', 'g', 'h']
+    CHARACTER(LEN=256), ALLOCATABLE :: lines(:)
+    CHARACTER(LEN=256) :: buf
+    REAL(8) :: vals(0:15)
+    INTEGER :: u, m, ios, nl, i, k, p, start, j, f, sec, orb, nset, func, ncoef, pri, ang, ori, norb
+    CHARACTER(LEN=*), PARAMETER :: fatom = '(A2,4x,I3,4x,I3,4x,F15.8,4x,F15.8,4x,F15.8)'
+    CHARACTER(LEN=*), PARAMETER :: fene = '(1x,A4,1x,F15.8)', fcoef = '(1x,I4,4x,F15.10)'
+    OPEN(newunit=m, file='mybasis', status='old', iostat=ios)
+    IF (ios /= 0) CALL host_fail("Couldn't open mybasis")
+    nl = 0
+    DO
+      READ(m, '(A)', iostat=ios) buf
+      IF (ios /= 0) EXIT
+      IF (LEN_TRIM(buf) > 0) nl = nl + 1
+    END DO
+    ALLOCATE(lines(nl))
+    REWIND(m)
+    i = 0
+    DO
+      READ(m, '(A)', iostat=ios) buf
+      IF (ios /= 0) EXIT
+      IF (LEN_TRIM(buf) > 0) THEN
+        i = i + 1
+        lines(i) = buf
+      END IF
+    END DO
+    CLOSE(m)
+    start = 0
+    DO i = 1, nl
+      IF (TRIM(first_token(lines(i))) == TRIM(bnames(bkey))) THEN
+        start = i
+        EXIT
+      END IF
+    END DO
+    IF (start == 0) CALL host_fail("basis not found in mybasis")
+    OPEN(newunit=u, file='MOLDEN', status='replace')
+    WRITE(u,*) "[Molden Format]"
+    WRITE(u,*) "[ATOMS] AU"
+    DO i = 0, SIZE(atoms)-1
+      WRITE(u,fatom) anames(atoms(i)), i+1, atoms(i), xyz(i,0:2)
+    END DO
+    WRITE(u,*) "[Molden Format]"
+    WRITE(u,*) "[GTO]"
+    DO i = 0, SIZE(atoms)-1
+      WRITE(u,*) i+1, 0
+      k = 0
+      DO p = start+3, nl
+        IF (TRIM(first_token(lines(p))) == TRIM(anames(atoms(i)))) THEN
+          k = p
+          EXIT
+        END IF
+      END DO
+      IF (k == 0) CALL host_fail("atom not found in mybasis")
+      READ(lines(k+1),*) sec, orb, nset
+      p = k + 2
+      DO j = 1, sec
+        READ(lines(p),*) func, ncoef, pri, ang, ori
+        p = p + 1
+        WRITE(u,*) lnames(ang), func, "1.00"
+        DO f = 1, func
+          READ(lines(p),*) vals(0:ncoef-1)
+          p = p + 1
+          WRITE(u,*) vals(1:ncoef-1), vals(0)
+        END DO
+      END DO
+      WRITE(u,*)
+    END DO
+    WRITE(u,*)
+    WRITE(u,*) "[MO]"
+    norb = SIZE(eigA)
+    CALL molden_set(eigA, CA, "Alpha")
+    IF (uhf) CALL molden_set(eigB, CB, "Beta")
+    CLOSE(u)
+  CONTAINS
+    SUBROUTINE molden_set(eig, C, spin)
+      REAL(8), INTENT(IN) :: eig(:), C(:,:)
+      CHARACTER(LEN=*), INTENT(IN) :: spin
+      INTEGER :: a, b
+      DO a = 1, norb
+        WRITE(u,*) "Sym= A"
+        WRITE(u,fene) "Ene=", eig(a)
+        WRITE(u,*) " Spin= " // spin
+        IF (eig(a) < 0.0D0) THEN
+          WRITE(u,*) "Occup= 1.0"
+        ELSE
+          WRITE(u,*) "Occup= 0.0"
+        END IF
+        DO b = 1, norb
+          WRITE(u,fcoef) b, C(b,a)
+        END DO
+      END DO
+    END SUBROUTINE molden_set
+  END SUBROUTINE write_molden
+
 END MODULE myqc_host

@@ -8,6 +8,9 @@
 ! :1091-1092) and re-reads XX from disk; here the screened unique integrals
 ! stay on the device and D, G, F never leave it.  The density criterion
 ! follows SCF_Conv (options(8), :1186-1258), with DIIS and |dE| < 1e-10 Eh.
+! The working matrices are taken from the fmem ledger while held (:114-129,
+! :210-211; UHF :272-292, :398-399), and the MOLDEN file is written as
+! makeMOLDEN writes it (:194, :372, :1343-1466).
 !-----------------------------------------------------------------------
 PROGRAM scf
   USE iso_c_binding
@@ -16,7 +19,7 @@ PROGRAM scf
   IMPLICIT NONE
   REAL(8), ALLOCATABLE :: xyz(:,:), bas(:), setv(:), S(:,:), H(:,:)
   INTEGER, ALLOCATABLE :: atoms(:), options(:), basinfo(:), setinfo(:)
-  REAL(8) :: fmem, enr, dtol, t0, t1
+  REAL(8) :: fmem, enr, dtol, t0, t1, mb
   INTEGER :: nnuc, nelcA, nelcB, n, rc, k
   INTEGER(c_long_long) :: nst
   TYPE(c_ptr) :: ctx
@@ -31,6 +34,13 @@ PROGRAM scf
   IF (options(3) == 0 .AND. nelcA /= nelcB) CALL host_fail("You cannot use RHF for open shell molecules!")
   CALL build_basis(options(2), atoms, bas, basinfo, setv, setinfo)
   n = basinfo(1)
+  IF (options(3) == 0) THEN
+    mb = 6.0D0 * n * n * 8.0D0 / 1.0D6 + n * 8.0D0 / 1.0D6
+    CALL ledger_take(fmem, mb, "rhf", 2)
+  ELSE
+    mb = 10.0D0 * n * n * 8.0D0 / 1.0D6 + 2.0D0 * n * 8.0D0 / 1.0D6
+    CALL ledger_take(fmem, mb, "uhf", 2)
+  END IF
   ALLOCATE(S(n,n), H(n,n))
   CALL read_text_matrix('Huv', H)
   CALL read_text_matrix('Suv', S)
@@ -46,6 +56,7 @@ PROGRAM scf
   ELSE
     CALL run_uhf()
   END IF
+  CALL ledger_give(fmem, mb)
   CALL CPU_TIME(t1)
   IF (options(3) == 0) THEN
     WRITE(*,'(1x,A16,F8.5)') "rhf ran in (s) :", t1 - t0
@@ -81,6 +92,7 @@ CONTAINS
     WRITE(*,*) "Total SCF energy (a.u)", e
     WRITE(*,*)
     CALL print_levels("Orbital eigenvalues (a.u.)", eig, nelcA)
+    CALL write_molden(options(2), atoms, xyz, eig, C, .FALSE., eig, C)
     CALL write_text_matrix('Cui', C)
     OPEN(newunit=u, file='Da', status='replace', form='unformatted')
     WRITE(u) D
@@ -121,6 +133,7 @@ CONTAINS
     CALL print_levels("Orbital eigenvalues of Alpha (a.u.)", eigA, nelcA)
     WRITE(*,*)
     CALL print_levels("Orbital eigenvalues of Beta (a.u.)", eigB, nelcB)
+    CALL write_molden(options(2), atoms, xyz, eigA, Ca, .TRUE., eigB, Cb)
     CALL write_text_matrix2('Cui', Ca, Cb)
     OPEN(newunit=u, file='Da', status='replace', form='unformatted')
     WRITE(u) Da

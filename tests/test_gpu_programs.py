@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 from conftest import golden
+from molden_cmp import compare_molden
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "myqc_amd", "bin")
@@ -50,13 +51,14 @@ def _text_matrix(path, n):
     return np.array(open(path).read().split(), dtype=np.float64)[: n * n].reshape(n, n, order="F")
 
 
-def _write_inputs(d, Z, xyz_bohr, scf_conv=11):
+def _write_inputs(d, Z, xyz_bohr, scf_conv=11, ref=0, nab=None):
     """The files the reference's parser writes (src/myQC/env.f90:16-73):
     envdat = nnuc / nelcA nelcB / 17 / options, nucpos = Z x y z (bohr)."""
     nelc = int(sum(Z))
-    opts = [1, 1, 0, 0, 0, 1, 1000000, 1, scf_conv, 0, 1, 1, 1, 0, 0, 0, 0]
+    na, nb = nab if nab is not None else (nelc // 2, nelc // 2)
+    opts = [1, 1, 0, ref, 0, 1, 1000000, 1, scf_conv, 0, 1, 1, 1, 0, 0, 0, 0]
     with open(os.path.join(d, "envdat"), "w") as f:
-        f.write(f" {len(Z)}\n {nelc // 2} {nelc // 2}\n 17\n " + " ".join(map(str, opts)) + "\n")
+        f.write(f" {len(Z)}\n {na} {nb}\n 17\n " + " ".join(map(str, opts)) + "\n")
     with open(os.path.join(d, "nucpos"), "w") as f:
         for z, r in zip(Z, xyz_bohr):
             f.write(f" {int(z)} {float(r[0])!r} {float(r[1])!r} {float(r[2])!r}\n")
@@ -134,6 +136,14 @@ def test_programs_end_to_end(mol, tmp_path):
     rec01 = aa[0].reshape(nv, nv, order="F")
     assert np.max(np.abs(rec01 - iajb[0, :, 1, :])) <= 1e-10
 
+    # the fmem ledger is back where the reference's programs leave it
+    # (src/myQC/env.f90:77-112)
+    files = golden("files")
+    fm = float(open(os.path.join(d, "fmem")).read().split()[0])
+    if f"fmem_{mol}_int2e" in files.files:
+        assert abs(fm - float(files[f"fmem_{mol}_int2e"])) <= 1e-6
+    assert abs(fm - 1000000.0) <= 1e-6
+
     out = _run(d, "mp2")
     # the reference's mp2 prints the correlation energies (mp2.f90:136-141)
     for label, key in (("E(AA) =", "E_aa"), ("E(AB) =", "E_ab"), ("E(MBPT2) =", "E_mp2")):
@@ -163,12 +173,12 @@ def _write_record(path, a):
 
 def test_int2e_record_above_2gib(ctx, tmp_path):
     """XX of a system with n >= 128 (19 waters, n = 133: 2.5 GB), which
-    int2e.f90:166,306 writes as one record.  This image's Fortran runtime
-    (flang, the compiler the reference is built with here) writes a record
-    above 2 GiB whole, with 4-byte markers holding its length modulo 2^32 (a
-    gfortran build would split it into subrecords instead).  The file reads back
-    to the C ABI's dense integrals, and when the reference's own RHFI2G has
-    been built from its sources (oracle/_ref), that program reads our XX
+    int2e.f90:166,306 writes as one record: by default in gfortran's subrecord
+    layout (the reference's own build uses gfortran, src/Makefile:11), and
+    with MYQC_XX_RECORD=single as one record the way this image's flang
+    runtime writes it.  Both read back to the C ABI's dense integrals, and
+    when the reference's own RHFI2G has been built from its sources here with
+    flang (oracle/_ref), that program reads the single-record XX
     (src/I2G/RHFI2G.f90:72-74) and returns the G that myqc_fock_G gives."""
     from myqc_amd import molecules as M
     from myqc_amd.basis import build_basis
@@ -178,10 +188,45 @@ def test_int2e_record_above_2gib(ctx, tmp_path):
     assert n >= 128
     d = str(tmp_path)
     _write_inputs(d, Z, xyz)
-    _run(d, "int2e")
-    path = os.path.join(d, "XX")
-    size = os.path.getsize(path)
     nbytes = 8 * n ** 4
+    ctx.set_system(Z, xyz)
+    ref = ctx.eri_full(0.0).ravel(order="F")
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, n ** 4, size=200000)
+    path = os.path.join(d, "XX")
+    # default: gfortran subrecords of at most 2^31 - 9 bytes (the layout of
+    # the reference's gfortran build), head markers negative but on the last,
+    # tail markers negative but on the first
+    _run(d, "int2e")
+    maxsub = 2147483639
+    k = -(-nbytes // maxsub)
+    assert os.path.getsize(path) == nbytes + 8 * k
+    with open(path, "rb") as f:
+        pos = 0
+        for i in range(k):
+            ln = min(maxsub, nbytes - i * maxsub)
+            f.seek(pos)
+            (head,) = struct.unpack("<i", f.read(4))
+            f.seek(pos + 4 + ln)
+            (tail,) = struct.unpack("<i", f.read(4))
+            print(f"[XX] n={n} subrecord {i}: markers {head} {tail}", flush=True)
+            assert head == (ln if i == k - 1 else -ln) and tail == (ln if i == 0 else -ln)
+            pos += 8 + ln
+    # values: the data bytes of the subrecords in order
+    raw = np.memmap(path, dtype=np.uint8, mode="r")
+    data = np.concatenate([raw[4 + i * (maxsub + 8): 4 + i * (maxsub + 8) + min(maxsub, nbytes - i * maxsub)]
+                           for i in range(k)]).view("<f8")
+    assert np.array_equal(data[idx], ref[idx])
+    del raw, data
+    os.remove(path)
+    # MYQC_XX_RECORD=single: one record, as this image's Fortran runtime
+    # (flang, the compiler the reference is built with here) writes it, its
+    # 4-byte markers holding the length modulo 2^32
+    env = dict(os.environ, MYQC_XX_RECORD="single")
+    print(f"[program] int2e (single record) in {d}", flush=True)
+    r = subprocess.run([os.path.join(BIN, "int2e")], cwd=d, capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    size = os.path.getsize(path)
     with open(path, "rb") as f:
         (head,) = struct.unpack("<I", f.read(4))
         f.seek(-4, 2)
@@ -189,10 +234,6 @@ def test_int2e_record_above_2gib(ctx, tmp_path):
     print(f"[XX] n={n} file {size} bytes, markers {head} {tail}", flush=True)
     assert size == nbytes + 8 and head == tail == nbytes % 2 ** 32
     XX = np.memmap(path, dtype="<f8", mode="r", offset=4, shape=(n ** 4,))
-    ctx.set_system(Z, xyz)
-    ref = ctx.eri_full(0.0).ravel(order="F")
-    rng = np.random.default_rng(3)
-    idx = rng.integers(0, n ** 4, size=200000)
     assert np.array_equal(XX[idx], ref[idx])
     del XX, ref
     rhfi2g = os.path.join(ROOT, "oracle", "_ref", "RHFI2G")
@@ -272,3 +313,24 @@ def test_props_programs_end_to_end(tmp_path):
     k = next(i for i, ln in enumerate(lines) if "X" in ln and "Y" in ln and "Z" in ln)
     mu = np.array([float(x) for x in lines[k + 1].split()])
     assert np.max(np.abs(mu - pr["mu"])) <= 1e-6 + 1e-12
+
+
+@pytest.mark.parametrize("key,fixture,ref,e_tol,c_tol", [("be", "ex_be", 0, 2e-8, 1e-6),
+                                                         ("o1", "ex_o1", 0, 2e-8, 1e-6),
+                                                         ("no", "uhf_no", 1, 2e-7, 1e-5)])
+def test_molden_matches_reference_examples(key, fixture, ref, e_tol, c_tol, tmp_path):
+    """The scf program's MOLDEN file against the ones the reference ships
+    (examples/Be, examples/O/singlet, examples/NO; makeMOLDEN,
+    src/scf/scf.f90:1343-1466): atoms, basis blocks, and per spin the orbital
+    energies (printed to 8 decimals), occupations and coefficients (as the
+    projector of each level, so signs and rotations among degenerate orbitals
+    do not matter), at the examples' default SCF_Conv of 7."""
+    g = golden(fixture)
+    d = str(tmp_path)
+    Z = [int(z) for z in g["Z"]]
+    nab = (int(g["nalpha"]), int(g["nbeta"])) if ref == 1 else None
+    _write_inputs(d, Z, g["xyz"], scf_conv=7, ref=ref, nab=nab)
+    _run(d, "int1e")
+    _run(d, "scf")
+    ours = open(os.path.join(d, "MOLDEN")).read()
+    compare_molden(ours, str(golden("files")[f"molden_{key}"]), e_tol, c_tol)
