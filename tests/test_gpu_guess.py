@@ -64,16 +64,23 @@ def test_device_pair_data_matches_host_statement(ctx):
 
 
 def test_device_guess_density_matches_host_statement(ctx):
+    from myqc_amd.hip import MyqcError
     ctx.mlp_create(G.N_FEAT, G.WIDTH, G.N_OUT, 11)
     for seed in (1, 2):
         Z, x = M.random_molecule(np.random.default_rng(seed))
         xyz = M.to_bohr(Z, x)
+        ctx.set_system(Z, xyz)
         D = ctx.guess_density(Z, xyz)
         Dh = G.predict_density_host(ctx, Z, xyz)
         assert np.max(np.abs(D - Dh)) <= 1e-5 * max(1.0, np.max(np.abs(Dh)))
     Z, x = M.water_cluster(6, seed=6)          # beyond the batch limit: one molecule of 42 functions
+    ctx.set_system(Z, M.to_bohr(Z, x))
     D = ctx.guess_density(Z, M.to_bohr(Z, x))
     assert D.shape == (42, 42) and np.allclose(D, D.T)
+    # a molecule other than the system set is refused (its n would not be the context's)
+    Z2, x2 = M.random_molecule(np.random.default_rng(1))
+    with pytest.raises(MyqcError):
+        ctx.guess_density(Z2, M.to_bohr(Z2, x2))
 
 
 def test_training_reduces_loss(ctx):
