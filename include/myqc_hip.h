@@ -166,6 +166,19 @@ int myqc_cis_uhf(myqc_ctx *ctx, const double *ajib_aa, const double *ajbi_aa, co
 int myqc_mp2_rhf_records(myqc_ctx *ctx, const double *iajb_records, const double *eig, int nocc, int nvrt,
                          double *e_aa, double *e_ab, double *e_tot);
 
+/* myqc_mp2_rhf_records for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same sums over the
+ * ijab_AB record stream, with the records (nocc^2 records of nvrt x nvrt, a
+ * fastest), the n = nocc + nvrt orbital energies and the three results
+ * {E(AA), E(AB), E(MBPT2)} all in caller-owned DEVICE memory (d_out: 3
+ * doubles).  The partial sums live in a fixed device scratch of this GPU that
+ * holds at most 65536 records: nocc > 256 returns MYQC_E_ARG.  Nothing is
+ * allocated; complete at return.  Calls that overlap on one GPU share the
+ * scratch and race. */
+int myqc_mp2_rhf_records_noalloc(myqc_ctx *ctx, const double *d_records, const double *d_eig, int nocc, int nvrt,
+                                 double *d_out);
+
 /* Core Hamiltonian and overlap, int1e drop-in (src/integrals/int1e.f90:132-280). */
 int myqc_int1e(myqc_ctx *ctx, double *S, double *H);
 

@@ -100,6 +100,71 @@ __global__ __launch_bounds__(256) void k_mp2(const double *__restrict__ O, const
     }
 }
 
+// MP2 sums straight from the ijab_AB record stream (record (i,j) i-major,
+// (a,b) a fastest; src/mp2/mp2.f90:99-135): one workgroup per record writes
+// its two partial sums to part[2 rec], part[2 rec + 1]
+//   sum_{a<b} ((ia|jb) - (ib|ja))^2 / D  (for i < j only),   sum_{ab} (ia|jb)^2 / D,
+// D = e_i + e_j - e_a - e_b; every slot of [0, 2 o^2) is written.
+__global__ __launch_bounds__(256) void k_mp2_records(const double *__restrict__ R, const double *__restrict__ e, int o,
+                                                     int nv, double *__restrict__ part) {
+    __shared__ double sh1[256], sh2[256];
+    const int rec = blockIdx.x, i = rec / o, j = rec - i * o;
+    const size_t nvv = (size_t)nv * nv;
+    const double *Rr = R + (size_t)rec * nvv;
+    const double eij = e[i] + e[j];
+    double s1 = 0.0, s2 = 0.0;
+    for (size_t q = threadIdx.x; q < nvv; q += blockDim.x) {
+        const int b = (int)(q / nv), a = (int)(q - (size_t)b * nv);
+        const double t = Rr[q];
+        const double den = eij - e[o + a] - e[o + b];
+        s2 += t * t / den;
+        if (i < j && a < b) {
+            const double u = t - Rr[(size_t)a * nv + b];
+            s1 += u * u / den;
+        }
+    }
+    sh1[threadIdx.x] = s1;
+    sh2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh1[threadIdx.x] += sh1[threadIdx.x + w];
+            sh2[threadIdx.x] += sh2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (size_t)rec] = sh1[0];
+        part[2 * (size_t)rec + 1] = sh2[0];
+    }
+}
+
+// out = {E(AA), E(AB), E(MBPT2) = 2 E(AA) + E(AB)} from the nrec record
+// partials, summed in a fixed order (one workgroup)
+__global__ __launch_bounds__(256) void k_mp2_fold(const double *__restrict__ part, int nrec, double *__restrict__ out) {
+    __shared__ double sh1[256], sh2[256];
+    double s1 = 0.0, s2 = 0.0;
+    for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+        s1 += part[2 * (size_t)r];
+        s2 += part[2 * (size_t)r + 1];
+    }
+    sh1[threadIdx.x] = s1;
+    sh2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh1[threadIdx.x] += sh1[threadIdx.x + w];
+            sh2[threadIdx.x] += sh2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh1[0];
+        out[1] = sh2[0];
+        out[2] = 2.0 * sh1[0] + sh2[0];
+    }
+}
+
 static int dgemm(myqc_ctx *ctx, rocblas_operation ta, int m, int n, int k, const double *A, int lda, long long sA,
                  const double *B, int ldb, long long sB, double *C, int ldc, long long sC, int batch, int precision) {
     if (precision == MYQC_BF16X3)
@@ -515,3 +580,27 @@ extern "C" int myqc_mp2_uhf_records(myqc_ctx *ctx, const double *ijab_aa, const 
     return mp2_uhf_sums(ctx, eig_a, eig_b, e_aa, e_bb, e_ab, e_tot);
 }
 
+// myqc_mp2_rhf_records without any allocation: the records and orbital
+// energies are already on the device, the per-record partial sums go to the
+// device scratch of this GPU (ctx->d_scratch, kNoAllocRecords records), and
+// the three energies to d_out.  Work is enqueued on the context stream and
+// complete at return.  Calls that overlap on one GPU share the scratch and
+// race.
+extern "C" int myqc_mp2_rhf_records_noalloc(myqc_ctx *ctx, const double *d_records, const double *d_eig, int nocc,
+                                            int nvrt, double *d_out) {
+    if (!ctx || !d_records || !d_eig || !d_out) return MYQC_E_ARG;
+    if (nocc < 1 || nvrt < 1)
+        return set_error(ctx, MYQC_E_ARG, "myqc_mp2_rhf_records_noalloc: nocc=%d nvrt=%d", nocc, nvrt);
+    const size_t nrec = (size_t)nocc * nocc;
+    if (nrec > kNoAllocRecords)
+        return set_error(ctx, MYQC_E_ARG,
+                         "myqc_mp2_rhf_records_noalloc: nocc=%d gives %zu records, beyond the scratch's %zu "
+                         "(nocc <= 256); use myqc_mp2_rhf_records",
+                         nocc, nrec, kNoAllocRecords);
+    if (!ctx->d_scratch) return set_error(ctx, MYQC_E_STATE, "myqc_mp2_rhf_records_noalloc: no device scratch");
+    k_mp2_records<<<(unsigned)nrec, 256, 0, ctx->stream>>>(d_records, d_eig, nocc, nvrt, ctx->d_scratch);
+    k_mp2_fold<<<1, 256, 0, ctx->stream>>>(ctx->d_scratch, (int)nrec, d_out);
+    MYQC_HIP(ctx, hipGetLastError());
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}

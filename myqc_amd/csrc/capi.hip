@@ -7,6 +7,14 @@
 
 using namespace myqc;
 
+// Device scratch of the entry points that allocate nothing
+// (myqc_mp2_rhf_records_noalloc): 2 doubles per (i,j) record, for at most
+// kNoAllocRecords records.  Each device the library runs on holds one copy
+// until the process ends; every call on that device shares it.  Host code
+// uses only the address hipGetSymbolAddress gives for it (ctx->d_scratch),
+// never the array's name.
+static __device__ __align__(256) double scratch[2 * kNoAllocRecords];
+
 extern "C" int myqc_init(int device, myqc_ctx **out) {
     if (!out) return MYQC_E_ARG;
     *out = nullptr;
@@ -27,6 +35,12 @@ extern "C" int myqc_init(int device, myqc_ctx **out) {
         rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success) {
         hipStreamDestroy(ctx->stream);
         delete ctx;
+        return MYQC_E_HIP;
+    }
+    // this device's copy of the no-allocation scratch (the device is current)
+    if (hipGetSymbolAddress(reinterpret_cast<void **>(&ctx->d_scratch), HIP_SYMBOL(scratch)) != hipSuccess ||
+        !ctx->d_scratch) {
+        myqc_finalize(ctx);
         return MYQC_E_HIP;
     }
     // pinned staging for small reads, and a context-owned rocBLAS workspace

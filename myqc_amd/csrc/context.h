@@ -155,6 +155,9 @@ struct myqc_ctx {
 
     // pinned host staging for small device-to-host reads (d2h)
     void *h_pin = nullptr;
+    // this device's address of the file-scope scratch of the no-allocation
+    // entry points (capi.hip), from hipGetSymbolAddress at myqc_init
+    double *d_scratch = nullptr;
     // the rocBLAS/rocSOLVER workspace, owned by the context and handed to the
     // handle with rocblas_set_workspace: the library allocates nothing inside
     // a call (blas_reserve sizes it before each new problem shape)
@@ -198,6 +201,8 @@ int set_error(myqc_ctx *ctx, int code, const char *fmt, ...);
 
 // bytes of the pinned staging buffer (ctx->h_pin)
 constexpr size_t kPinBytes = 4096;
+// records (i,j) the no-allocation MP2 entry point takes at most (nocc <= 256)
+constexpr size_t kNoAllocRecords = 65536;
 
 // device-to-host copy ordered on ctx->stream, complete at return
 hipError_t d2h(myqc_ctx *ctx, void *dst, const void *src, size_t bytes);

@@ -39,6 +39,7 @@ EXPORTS = {
     "myqc_ao2mo_ovov": ([_vp, _dp, ctypes.c_int, _dp, ctypes.c_int], ctypes.c_int),
     "myqc_mp2_rhf": ([_vp, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "myqc_mp2_rhf_records": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+    "myqc_mp2_rhf_records_noalloc": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "myqc_int1e": ([_vp, _dp, _dp], ctypes.c_int),
     "myqc_int1p": ([_vp, _dp], ctypes.c_int),
     "myqc_dipole_rhf": ([_vp, _dp, _dp], ctypes.c_int),
