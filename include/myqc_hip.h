@@ -203,6 +203,24 @@ int myqc_rhf_scf_density(myqc_ctx *ctx, const double *H, const double *S, int no
                          const double *D_guess, double e_tol, double d_tol, int max_iter, int precision,
                          double *energy, double *C, double *eig, int *iterations);
 
+/* myqc_rhf_scf_density (D_guess[k] given) or myqc_rhf_scf from the core
+ * guess (D_guess NULL or D_guess[k] NULL) for nsys <= 32 systems of one basis
+ * size at once, each on its own context (myqc_set_system and myqc_eri_build
+ * done), solved in lockstep on the first context's stream: the Fock builds one
+ * system after another, every other step of the loop one launch for all
+ * systems, one host synchronisation per iteration for all.  No counterpart in
+ * the reference, which solves one molecule per run; for callers that solve
+ * many molecules of one size (conformers, clusters).  Each system converges
+ * on its own criteria, as in myqc_rhf_scf; outputs per system k: energy[k],
+ * C[k] (n x n), eig[k] (n), iterations[k] (C, eig may be NULL, or hold NULL
+ * entries).  MYQC_E_NOCONV when any system did not converge (the others'
+ * outputs are valid).  The contexts must not be used by other threads during
+ * the call. */
+int myqc_rhf_scf_many(myqc_ctx *const *ctxs, int nsys, const double *const *H, const double *const *S,
+                      const int *nocc, const double *e_nuc, const double *const *D_guess, double e_tol, double d_tol,
+                      int max_iter, int precision, double *energy, double *const *C, double *const *eig,
+                      int *iterations);
+
 /* densUHF drop-in: Da(u,v) = sum_{i<nalpha} Ca(u,i) Ca(v,i), Db likewise over
  * nbeta (src/dens/dens.f90:213-228). */
 int myqc_density_uhf(myqc_ctx *ctx, const double *Ca, const double *Cb, int nalpha, int nbeta, double *Da,

@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-mllvm", "-pragma-unroll-threshold=100000",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
-SOURCES = ["capi.hip", "eri.hip", "jk.hip", "scf.hip", "sp2.hip", "batch.hip", "mlp.hip", "one_e.hip", "ao2mo.hip", "mfma_gemm.hip", "cis.hip"]
+SOURCES = ["capi.hip", "eri.hip", "jk.hip", "scf.hip", "sp2.hip", "batch.hip", "mlp.hip", "one_e.hip", "ao2mo.hip", "mfma_gemm.hip", "mfma_f64.hip", "cis.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -1,11 +1,11 @@
 // ao2mo.hip -- the RHF (ia|jb) four-index transform and the MP2 energy.
 //
 // The transform keeps the reference's four quarter steps
-// (idx1..idx4_trans, src/ao2mo/ao2mo.f90:1306-1439) but runs each as one
-// dense GEMM (or a strided batch of GEMMs) over the device-resident dense AO
-// integrals; the FP64 form uses rocBLAS DGEMM (plain library GEMMs), the
-// split-precision form the hand-written MFMA kernels in mfma_gemm.hip, with
-// steps 1 and 2 fused so that the o n^3 intermediate is never stored.
+// (idx1..idx4_trans, src/ao2mo/ao2mo.f90:1306-1439) but runs each as dense
+// matrix-core products over the device-resident dense AO integrals, in
+// hand-written kernels: FP64 MFMA (mfma_f64.hip) for the plain form, split
+// bf16 MFMA with FP32 accumulation (mfma_gemm.hip) for the mixed one; in both
+// steps 1 and 2 are fused so that the o n^3 intermediate is never stored.
 // The MP2 reduction restates mp2_rhf (src/mp2/mp2.f90:79-144) as one pass
 // with a deterministic two-stage sum.
 #include <cmath>
@@ -18,6 +18,10 @@ int gemm_bf16x3(myqc_ctx *ctx, bool transA, int m, int n, int k, const double *A
                 const double *B, int ldb, long long strideB, double *C, int ldc, long long strideC, int batch);
 int ao2mo12_bf16x3(myqc_ctx *ctx, int np, const double *Cp, int nq, const double *Cq, int n, const double *XX,
                    double *M);
+int ao2mo12_f64(myqc_ctx *ctx, int np, const double *Cp, int nq, const double *Cq, int n, const double *XX,
+                double *M);
+int gemm_f64(myqc_ctx *ctx, int m, int nb, int k, const double *A, long long lda, long long sA, const double *B,
+             long long ldb, double *C, long long ldc, long long sC, int batch);
 }
 
 namespace myqc {
@@ -165,53 +169,29 @@ __global__ __launch_bounds__(256) void k_mp2_fold(const double *__restrict__ par
     }
 }
 
-static int dgemm(myqc_ctx *ctx, rocblas_operation ta, int m, int n, int k, const double *A, int lda, long long sA,
-                 const double *B, int ldb, long long sB, double *C, int ldc, long long sC, int batch, int precision) {
-    if (precision == MYQC_BF16X3)
-        return gemm_bf16x3(ctx, ta == rocblas_operation_transpose, m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
-    const double one = 1.0, zero = 0.0;
-    MYQC_CHECK(blas_reserve(ctx, [&] {
-        if (batch == 1)
-            (void)rocblas_dgemm(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
-        else
-            (void)rocblas_dgemm_strided_batched(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, sA, B,
-                                                ldb, sB, &zero, C, ldc, sC, batch);
-    }));
-    if (batch == 1)
-        MYQC_BLAS(ctx, rocblas_dgemm(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc));
-    else
-        MYQC_BLAS(ctx, rocblas_dgemm_strided_batched(ctx->blas, ta, rocblas_operation_none, m, n, k, &one, A, lda, sA, B,
-                                                     ldb, sB, &zero, C, ldc, sC, batch));
-    return 0;
-}
-
 // Quarter steps 1 and 2 (idx1_trans, idx2_trans) over two column blocks of
-// orbitals, Cp (n x np) and Cq (n x nq), leading dimension n:
+// orbitals, Cp (n x np) and Cq (n x nq), leading dimension n, fused per (l,d)
+// block so that L(p,v,l,d) = sum_u Cp(u,p) XX(u,v,l,d) stays on chip:
 //   M(p,q,l,d) = sum_v [sum_u Cp(u,p) XX(u,v,l,d)] Cq(v,q)      (np x nq x n x n)
 // With release_xx, *dXX is freed once nothing reads it any more.
 static int quarter12(myqc_ctx *ctx, double **dXX, bool release_xx, const double *Cp, int np, const double *Cq, int nq,
                      int precision, double **dM) {
     const int n = ctx->n;
     const size_t N = n, msize = sizeof(double) * (size_t)np * nq * N * N;
-    if (precision == MYQC_BF16X3) {
-        // fused per (l,d) block: L never leaves the workgroup
-        MYQC_CHECK(dalloc(ctx, (void **)dM, msize));
-        MYQC_CHECK(ao2mo12_bf16x3(ctx, np, Cp, nq, Cq, n, *dXX, *dM));
-        if (release_xx) dfree((void **)dXX);   // (hipFree waits for the kernel)
-        return 0;
-    }
-    // 1: L(p,v,l,d) = sum_u Cp(u,p) XX(u,v,l,d)
-    double *dL = nullptr;
-    MYQC_CHECK(dalloc(ctx, (void **)&dL, sizeof(double) * (size_t)np * N * N * N));
-    int rc = dgemm(ctx, rocblas_operation_transpose, np, n * n * n, n, Cp, n, 0, *dXX, n, 0, dL, np, 0, 1, precision);
-    if (rc == 0 && release_xx) dfree((void **)dXX);
-    // 2: M(p,q,l,d) = sum_v L(p,v,l,d) Cq(v,q)
-    if (rc == 0) rc = dalloc(ctx, (void **)dM, msize);
-    if (rc == 0)
-        rc = dgemm(ctx, rocblas_operation_none, np, nq, n, dL, np, (long long)np * n, Cq, n, 0, *dM, np,
-                   (long long)np * nq, n * n, precision);
-    dfree((void **)&dL);
-    return rc;
+    MYQC_CHECK(dalloc(ctx, (void **)dM, msize));
+    if (precision == MYQC_BF16X3) MYQC_CHECK(ao2mo12_bf16x3(ctx, np, Cp, nq, Cq, n, *dXX, *dM));
+    else MYQC_CHECK(ao2mo12_f64(ctx, np, Cp, nq, Cq, n, *dXX, *dM));
+    if (release_xx) dfree((void **)dXX);   // (hipFree waits for the kernel)
+    return 0;
+}
+
+// C = A B with A (m x k, lda, stride sA per problem) and the shared B (k x nb,
+// leading dimension k) for `batch` problems, in the transform's precision
+static int quarter_gemm(myqc_ctx *ctx, int m, int nb, int k, const double *A, long long lda, long long sA,
+                        const double *B, double *C, long long ldc, long long sC, int batch, int precision) {
+    if (precision == MYQC_BF16X3)
+        return gemm_bf16x3(ctx, false, m, nb, k, A, (int)lda, sA, B, k, 0, C, (int)ldc, sC, batch);
+    return gemm_f64(ctx, m, nb, k, A, lda, sA, B, k, C, ldc, sC, batch);
 }
 
 // Quarter steps 3 and 4 (idx3_trans, idx4_trans) with the second pair's
@@ -224,11 +204,10 @@ static int quarter34(myqc_ctx *ctx, const double *dM, int m1, const double *Cr, 
     double *dN = nullptr;
     MYQC_CHECK(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)m1 * nr * N));
     // 3: N(pq,r,d) = sum_l M(pq,l,d) Cr(l,r)
-    int rc = dgemm(ctx, rocblas_operation_none, m1, nr, n, dM, m1, (long long)m1 * n, Cr, n, 0, dN, m1,
-                   (long long)m1 * nr, n, precision);
+    int rc = quarter_gemm(ctx, m1, nr, n, dM, m1, (long long)m1 * n, Cr, dN, m1, (long long)m1 * nr, n, precision);
     // 4: O(pq,r,s) = sum_d N(pq,r,d) Cs(d,s)
     if (rc == 0)
-        rc = dgemm(ctx, rocblas_operation_none, m1 * nr, ns, n, dN, m1 * nr, 0, Cs, n, 0, dO, m1 * nr, 0, 1, precision);
+        rc = quarter_gemm(ctx, m1 * nr, ns, n, dN, (long long)m1 * nr, 0, Cs, dO, (long long)m1 * nr, 0, 1, precision);
     dfree((void **)&dN);
     return rc;
 }

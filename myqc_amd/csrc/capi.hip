@@ -60,6 +60,7 @@ extern "C" int myqc_finalize(myqc_ctx *ctx) {
     hipStreamSynchronize(ctx->stream);
     free_eri(ctx);
     free_scf(ctx);
+    free_many(ctx);
     free_batch(ctx);
     free_mlp(ctx);
     void **ptrs[] = {(void **)&ctx->d_pairs, (void **)&ctx->d_prims, (void **)&ctx->d_ftT, (void **)&ctx->d_schwarz,

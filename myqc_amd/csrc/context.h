@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/myqc_hip.h"
+#include "digest_plan.h"
 #include "eri_math.h"
 #include "system.h"
 
@@ -33,6 +34,7 @@ struct QuartetClass {
 };
 
 struct ScfBuffers;
+struct ManyBuffers;
 struct BatchState;
 struct MlpState;
 
@@ -77,43 +79,40 @@ struct myqc_ctx {
     double *d_schwarz = nullptr;
     bool have_schwarz = false;
 
-    // stored integrals: each unique shell quartet block is kept in up to three
-    // copies (the bra pair in both orientations, the ket pair in its stored
-    // one), each under an ordered shell pair (A,B) that owns rows of G, in
-    // segments [(A*ns + B)*3 + ket class] of kets sorted by Schwarz factor
+    // stored integrals: each unique screened shell quartet once, pre-weighted,
+    // under its bra pair (the layout and schedule of digest_plan.h)
     bool have_eri = false;
     bool have_eri32 = false;
     double eri_tol = 0.0;
     myqc::QuartetClass cls[6];
-    long long n_stored = 0;               // unique integral slots computed
-    long long n_expanded = 0;             // values held for digestion
+    long long n_stored = 0;               // stored values (one per unique integral slot)
     long long n_quartets = 0;
     double eri_flops = 0.0;               // algorithmic FP64 work of the last build
     int ns = 0;                           // number of shells
-    std::vector<long long> segoff;        // ns*ns*3 segment offsets
-    std::vector<int> segcnt;              // ns*ns*3 kets per segment
-    std::vector<int> segstart;            // ns*ns*3 rank of a segment's first ket (reverse segments)
-    std::vector<int> pair_of;             // ns*ns: the shell pair of (X, Y) in either orientation
-    std::vector<int> pair_rank, pair_cls; // rank within its class's sorted list, class
-    long long *d_segoff = nullptr;
+    myqc::DigestPlan plan;                // layout and digestion schedule (host)
+    long long *d_segoff = nullptr;        // [p*3 + ket class]: block of ket rank r at segoff + r*BLK
     int *d_rank = nullptr;                // rank of each pair within its class's sorted list
-    double *d_x64 = nullptr;              // expanded values
+    double *d_x64 = nullptr;              // stored values
     float *d_x32 = nullptr;
-    // digestion schedule: work units (A, B) grouped into workgroups of one
-    // row shell A, each unit's kets cut into chunks of at most 64 of one class
-    int n_wg = 0, n_units = 0, n_chunks = 0;
-    int dig_rows = 0;                     // most D rows a workgroup stages in LDS
-    int *d_wg = nullptr;                  // [w] = {A, chunk0, chunk1, first D row, D rows, partial slot}, longest first
-    int *d_unit = nullptr;                // [u] = {B, count ck=0, count ck=1, count ck=2}
-    int *d_chunk = nullptr;               // [c] = {offset lo, hi, aoB | LB << 15 | class << 16 | kets << 18, first rank}
-    long long *d_unit_off = nullptr;      // [u][3] segment offsets
-    int *d_meta[3] = {nullptr, nullptr, nullptr};   // per class, by Schwarz rank: aoC | aoD << 15 | same << 30
+    // digestion schedule on the device
+    int n_wg = 0;
+    int *d_wg = nullptr;                  // kWgInts per workgroup
+    int *d_unit = nullptr;                // kUnitInts per unit
+    long long *d_unit_off = nullptr;      // [u][3] value offsets
+    int *d_chunk = nullptr;               // {class, first rank, kets, ket-side slot} per column chunk
+    int *d_row_ao = nullptr;              // AO of each partial row
+    int *d_row_list = nullptr, *d_row_list_off = nullptr;   // per AO, its partial rows (CSR)
+    int *d_jk_list = nullptr, *d_jk_list_off = nullptr;     // per rank block, its chunk slots (CSR)
+    int *d_jk_group = nullptr, *d_row_group = nullptr;      // the gathers' groups
+    int *d_jkoff = nullptr;               // [X*ns + Y]: J-ket block of pair (X,Y) if stored so, else -1
+    double *d_jk_part = nullptr;          // ket-side J partial slots (kJkSlot doubles each)
+    double *d_jk = nullptr;               // ket-side J by class and rank
+    int *d_meta[3] = {nullptr, nullptr, nullptr};   // per class, by Schwarz rank: aoC | aoD << 15
     float *d_qrank[3] = {nullptr, nullptr, nullptr}; // per class, the Schwarz factor of each rank (descending)
-    float *d_chunkq = nullptr;            // [c][4] q_row q_first, q_row q_last, q_row, 0 (rounded up)
     double *d_dmax = nullptr;             // max |dD| of the current incremental pass (float bits in the first word)
-    int *d_shell = nullptr;               // [s] = {ao, L, wg0, wg1}
+    int *d_shell = nullptr;               // [s] = {ao, L, 0, 0}
     int *d_ao_shell = nullptr;            // shell of each AO
-    double *d_partial = nullptr;          // n_wg x 3 x n (both precisions accumulate in FP64)
+    double *d_partial = nullptr;          // partial rows x n (both precisions accumulate in FP64)
     double *d_Dpair = nullptr;            // density blocks of the kets, by class and rank (see jk.hip)
     float *d_Dpair32 = nullptr;
 
@@ -141,6 +140,7 @@ struct myqc_ctx {
     // last RHF run: purification steps and diagonalisation fallbacks
     myqc::ScfBuffers *scf = nullptr;       // RHF (and UHF alpha) work arrays, kept across calls
     myqc::ScfBuffers *scf_b = nullptr;     // UHF beta work arrays
+    myqc::ManyBuffers *many = nullptr;     // the many-system SCF's shared buffers (this context leading)
     myqc::BatchState *batch = nullptr;     // batched small-molecule RHF (batch.hip)
     myqc::MlpState *mlp = nullptr;         // learned-guess network (mlp.hip)
     int mlp_gen = 0;                      // bumped whenever the network's buffers move
@@ -283,6 +283,7 @@ int digest_G_uhf(myqc_ctx *ctx, const double *d_Da, const double *d_Db, double *
 int ensure_work(myqc_ctx *ctx);
 void free_eri(myqc_ctx *ctx, bool keep_memory = false);
 void free_scf(myqc_ctx *ctx);
+void free_many(myqc_ctx *ctx);
 void free_batch(myqc_ctx *ctx);
 void free_mlp(myqc_ctx *ctx);
 

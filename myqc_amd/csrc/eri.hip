@@ -138,35 +138,14 @@ struct EriArgs {
     int nbra;
     long long nq;
     int same_class;
-    // MODE 0: expanded copies
-    const long long *segoff;   // [(X*ns + Y)*3 + ket class]
-    const int *rank;           // rank of a pair within its class's sorted list
-    int ns;
+    // MODE 0: the stored (pre-weighted) block of each quartet
+    const long long *segoff;   // [bra pair*3 + ket class]: element e of ket rank r at segoff + r + e*len
     double *x64;
     float *x32;
     // MODE 1: dense XX
     double *XX;
     int n;
 };
-
-// one copy of a quartet block, permuted to the ordered bra (X,Y) with the ket
-// in its stored orientation: dst[((x*ny + y)*nu + u)*nw + w]
-template <int NX, int NY, int NU, int NW, typename F>
-__device__ __forceinline__ void put_copy(double *__restrict__ x64, float *__restrict__ x32, long long base, F &&src) {
-#pragma unroll
-    for (int x = 0; x < NX; ++x)
-#pragma unroll
-        for (int y = 0; y < NY; ++y)
-#pragma unroll
-            for (int u = 0; u < NU; ++u)
-#pragma unroll
-                for (int w = 0; w < NW; ++w) {
-                    const long long e = base + ((x * NY + y) * NU + u) * NW + w;
-                    const double v = src(x, y, u, w);
-                    x64[e] = v;
-                    if (x32) x32[e] = (float)v;
-                }
-}
 
 __device__ __forceinline__ void find_quartet(const long long *__restrict__ cum, int nbra, long long t, int same,
                                              int &rb, long long &rk) {
@@ -219,26 +198,30 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     double CD[3] = {ket.AB[0], ket.AB[1], ket.AB[2]};
     double out[BLK];
     quartet_hrr<LA, LB, LC, LD>(acc, bra.AB, CD, out);
+    const int same_pair = (pb == pk);
     if (MODE == 0) {
-        // copies for the row owners A (bra (A,B)), B (bra (B,A)) and C (ket (C,D))
-        const int A = bra.sa, B = bra.sb, C = ket.sa, D = ket.sb, ns = a.ns;
-        const long long rq = a.rank[pk], rp = a.rank[pb];
-        auto at = [&](int ia, int ib, int ic, int id) { return out[((ia * nb + ib) * nc + ic) * nd + id]; };
-        put_copy<na, nb, nc, nd>(a.x64, a.x32, a.segoff[((long long)A * ns + B) * 3 + CK] + rq * BLK,
-                                 [&](int x, int y, int u, int w) { return at(x, y, u, w); });
-        // the bra's reverse segment holds only the kets of which the bra pair
-        // is the bra: in the same class those from its own rank on
-        if (!bra.same)
-            put_copy<nb, na, nc, nd>(a.x64, a.x32,
-                                     a.segoff[((long long)B * ns + A) * 3 + CK] + (rq - (CB == CK ? rp : 0)) * BLK,
-                                     [&](int x, int y, int u, int w) { return at(y, x, u, w); });
-        // the ket pair's copy, in its stored orientation only
-        if (pb != pk)
-            put_copy<nc, nd, na, nb>(a.x64, a.x32, a.segoff[((long long)C * ns + D) * 3 + CB] + rp * BLK,
-                                     [&](int x, int y, int u, int w) { return at(u, w, x, y); });
+        // one copy under the bra pair, each element times its degeneracy
+        // weight (the digestion then applies the same six updates to all);
+        // element-major over the bra's kets of this class (len of them)
+        const long long base = a.segoff[(long long)pb * 3 + CK] + rk;
+        const long long len = a.cum[rb + 1] - a.cum[rb];
+#pragma unroll
+        for (int ia = 0; ia < na; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < nb; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < nc; ++ic)
+#pragma unroll
+                    for (int id = 0; id < nd; ++id) {
+                        const int e = ((ia * nb + ib) * nc + ic) * nd + id;
+                        const double w = block_weight(ia, ib, ic, id, bra.same, ket.same, same_pair, nb, nd,
+                                                      bra.ao_a + ia, bra.ao_b + ib, ket.ao_a + ic, ket.ao_b + id);
+                        const double v = out[e] * w;
+                        a.x64[base + e * len] = v;
+                        if (a.x32) a.x32[base + e * len] = (float)v;
+                    }
         return;
     }
-    const int same_pair = (pb == pk);
 #pragma unroll
     for (int ia = 0; ia < na; ++ia)
 #pragma unroll
@@ -306,19 +289,20 @@ void free_eri(myqc_ctx *ctx, bool keep_memory) {
         if (!keep_memory) dfree((void **)&c.d_cum);
         c.nq = 0;
     }
-    void **ptrs[] = {(void **)&ctx->d_segoff, (void **)&ctx->d_rank,     (void **)&ctx->d_x64,
-                     (void **)&ctx->d_x32,    (void **)&ctx->d_wg,       (void **)&ctx->d_unit,
-                     (void **)&ctx->d_unit_off, (void **)&ctx->d_shell, (void **)&ctx->d_ao_shell,
-                     (void **)&ctx->d_partial, (void **)&ctx->d_Dpair,
-                     (void **)&ctx->d_Dpair32, (void **)&ctx->d_chunk};
+    void **ptrs[] = {(void **)&ctx->d_segoff,   (void **)&ctx->d_rank,        (void **)&ctx->d_x64,
+                     (void **)&ctx->d_x32,      (void **)&ctx->d_wg,          (void **)&ctx->d_unit,
+                     (void **)&ctx->d_unit_off, (void **)&ctx->d_shell,       (void **)&ctx->d_ao_shell,
+                     (void **)&ctx->d_partial,  (void **)&ctx->d_Dpair,       (void **)&ctx->d_Dpair32,
+                     (void **)&ctx->d_chunk,    (void **)&ctx->d_row_ao,      (void **)&ctx->d_row_list,
+                     (void **)&ctx->d_row_list_off, (void **)&ctx->d_jk_list, (void **)&ctx->d_jk_list_off,
+                     (void **)&ctx->d_jkoff,    (void **)&ctx->d_jk_part,     (void **)&ctx->d_jk,
+                     (void **)&ctx->d_jk_group, (void **)&ctx->d_row_group};
     if (!keep_memory) {
         for (auto p : ptrs) dfree(p);
         for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
         for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_qrank[c]);
-        dfree((void **)&ctx->d_chunkq);
     }
-    ctx->n_wg = ctx->n_units = ctx->n_chunks = 0;
-    ctx->n_expanded = 0;
+    ctx->n_wg = 0;
     ctx->have_eri = false;
     ctx->have_eri32 = false;
 }
@@ -426,8 +410,6 @@ static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX, hipStr
     a.nq = qc.nq;
     a.same_class = (CB == CK);
     a.segoff = ctx->d_segoff;
-    a.rank = ctx->d_rank;
-    a.ns = ctx->ns;
     a.x64 = ctx->d_x64;
     a.x32 = ctx->d_x32;
     a.XX = XX;
@@ -490,286 +472,93 @@ static int build_classes(myqc_ctx *ctx, double tol) {
     return 0;
 }
 
-// Segments of the expanded storage and the digestion schedule.  For every
-// ordered shell pair (X,Y) and ket class ck, the kets of class ck whose
-// Schwarz product with (X,Y) passes `tol` form a prefix of that class's
-// sorted pair list; their blocks (X Y|ket) are stored contiguously.
-// kets per lane of a digestion chunk (jk.hip)
-__host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 : row == 3 ? 4 : row == 9 ? 2 : 1; }
+// copies a host vector to a device buffer kept by the context
+template <typename T>
+static int upload(myqc_ctx *ctx, T **dst, const std::vector<T> &src) {
+    MYQC_CHECK(dalloc(ctx, (void **)dst, sizeof(T) * std::max<size_t>(src.size(), 1)));
+    if (!src.empty())
+        MYQC_HIP(ctx, hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
 
-static int build_segments(myqc_ctx *ctx, double tol) {
+// LDS the digestion kernel may use per workgroup: three workgroups per CU
+constexpr size_t kDigestLds = 52 * 1024;
+
+// The layout of the stored integrals (digest_plan.h): each screened unique
+// quartet's block offset under its bra pair.  The ERI kernels need only this.
+static int build_layout(myqc_ctx *ctx, double tol) {
     const int ns = (int)ctx->shells.size();
     ctx->ns = ns;
-    const int np = (int)ctx->pairs.size();
-    // pair of each unordered shell pair, and its Schwarz factor
-    std::vector<int> pair_of((size_t)ns * ns, -1);
-    for (int p = 0; p < np; ++p) {
-        pair_of[(size_t)ctx->pairs[p].sa * ns + ctx->pairs[p].sb] = p;
-        pair_of[(size_t)ctx->pairs[p].sb * ns + ctx->pairs[p].sa] = p;
-    }
-    std::vector<int> rank(np, 0);
-    std::vector<double> qsorted[3];
-    for (int c = 0; c < 3; ++c) {
-        const auto &ids = ctx->pair_ids[c];
-        qsorted[c].resize(ids.size());
-        for (size_t r = 0; r < ids.size(); ++r) {
-            rank[ids[r]] = (int)r;
-            qsorted[c][r] = ctx->schwarz[ids[r]];
-        }
-    }
-    // screened ket count of every pair in each ket class: kets with
-    // q_P q_Q >= tol form a prefix of the Schwarz-sorted class list
-    // (each class's pairs in descending Schwarz order need ascending
-    // thresholds tol / q_p, so one pointer per ket class walks down once)
-    std::vector<int> pcnt((size_t)np * 3);
-    for (int cb = 0; cb < 3; ++cb) {
-        const auto &ids = ctx->pair_ids[cb];
-        for (int ck = 0; ck < 3; ++ck) {
-            const auto &qk = qsorted[ck];
-            size_t ptr = qk.size();
-            for (size_t r = 0; r < ids.size(); ++r) {
-                const int p = ids[r];
-                const double qp = ctx->schwarz[p];
-                long long cnt;
-                if (tol <= 0.0) cnt = (long long)qk.size();
-                else if (qp <= 0.0) cnt = 0;
-                else {
-                    const double thr = tol / qp;
-                    while (ptr > 0 && !(qk[ptr - 1] >= thr)) --ptr;
-                    cnt = (long long)ptr;
-                }
-                pcnt[(size_t)p * 3 + ck] = (int)cnt;
-            }
-        }
-    }
-    // a pair's forward segment (its stored orientation) holds every screened
-    // partner; its reverse segment only the partners of which it is the bra
-    // (lower classes, and in its own class those from its own rank on)
-    std::vector<int> pcls(np, 0);
-    for (int c = 0; c < 3; ++c)
-        for (int id : ctx->pair_ids[c]) pcls[id] = c;
-    ctx->segoff.assign((size_t)ns * ns * 3, 0);
-    ctx->segcnt.assign((size_t)ns * ns * 3, 0);
-    ctx->segstart.assign((size_t)ns * ns * 3, 0);
-    long long off = 0;
-    for (int X = 0; X < ns; ++X)
-        for (int Y = 0; Y < ns; ++Y) {
-            const int p = pair_of[(size_t)X * ns + Y];
-            const int nxy = (ctx->shells[X].L ? 3 : 1) * (ctx->shells[Y].L ? 3 : 1);
-            const bool fwd = p >= 0 && ctx->pairs[p].sa == X;
-            for (int ck = 0; ck < 3; ++ck) {
-                long long cnt = p >= 0 ? pcnt[(size_t)p * 3 + ck] : 0, start = 0;
-                if (p >= 0 && !fwd) {
-                    if (ck > pcls[p]) cnt = 0;
-                    else if (ck == pcls[p]) {
-                        start = rank[p];
-                        cnt = std::max(0LL, cnt - start);
-                    }
-                }
-                const size_t seg = ((size_t)X * ns + Y) * 3 + ck;
-                ctx->segoff[seg] = off;
-                ctx->segcnt[seg] = (int)cnt;
-                ctx->segstart[seg] = (int)start;
-                off += cnt * nxy * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
-            }
-        }
-    ctx->n_expanded = off;
-    ctx->pair_of.swap(pair_of);
-    ctx->pair_rank = rank;
-    ctx->pair_cls.swap(pcls);
-    const size_t nseg = (size_t)ns * ns * 3;
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_segoff, sizeof(long long) * nseg));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_rank, sizeof(int) * np));
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_segoff, ctx->segoff.data(), sizeof(long long) * nseg, hipMemcpyHostToDevice,
-                                 ctx->stream));
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_rank, ctx->pair_rank.data(), sizeof(int) * np, hipMemcpyHostToDevice,
-                                 ctx->stream));
+    std::vector<int> sao(ns);
+    for (int X = 0; X < ns; ++X) sao[X] = ctx->shells[X].ao;
+    const std::string e = build_digest_layout(ctx->plan, ctx->n, sao, ctx->pairs, ctx->pair_ids, ctx->schwarz, tol);
+    if (!e.empty()) return set_error(ctx, MYQC_E_ARG, "%s", e.c_str());
+    if (ctx->plan.n_values != ctx->n_stored || ctx->plan.n_quartets != ctx->n_quartets)
+        return set_error(ctx, MYQC_E_STATE, "integral layout (%lld values) disagrees with the quartet classes (%lld)",
+                         ctx->plan.n_values, ctx->n_stored);
+    MYQC_CHECK(upload(ctx, &ctx->d_segoff, ctx->plan.off));
+    MYQC_CHECK(upload(ctx, &ctx->d_rank, ctx->plan.rank));
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
-// The digestion schedule over the segments of build_segments.  It is built
-// on the host while the ERI kernels that fill the segments run.
+// The digestion schedule over the layout, built on the host while the ERI
+// kernels that fill the store run, and its device copies.
 static int build_schedule(myqc_ctx *ctx) {
-    const int ns = ctx->ns;
-    // digestion schedule: per row shell X, its units Y (all three ket
-    // classes) packed into workgroups of about kTarget values whose Y shells
-    // span at most dig_rows AOs (the D rows a workgroup stages in LDS: with
-    // the 3 owned rows of G, (3 + dig_rows) n doubles stay within
-    // kLdsBudget so that three workgroups fit a CU); each unit's kets are
-    // cut into chunks of at most 64 of one class, one wavefront's work
-    const long long kTarget = 1 << 15;
-    const size_t kLdsBudget = 52 * 1024;
-    const int n = ctx->n;
-    ctx->dig_rows = std::max(3, std::min(64, (int)(kLdsBudget / (sizeof(double) * (size_t)n)) - 3));
-    std::vector<int> wg, unit, chunk, shell(4 * ns);
-    std::vector<long long> unit_off, wg_cost;
-    // per chunk {q_row q_first, q_row q_last, q_row, 0} (the incremental screen)
-    std::vector<float> chunkq;
-    {
-        long long nchunk = 0, nunit = 0;
-        for (size_t seg = 0; seg < ctx->segcnt.size(); ++seg) {
-            nchunk += (ctx->segcnt[seg] + 63) / 64 + 2;
-            nunit += (seg % 3 == 0) && (ctx->segcnt[seg] + ctx->segcnt[seg + 1] + ctx->segcnt[seg + 2] > 0);
-        }
-        chunk.reserve(4 * (size_t)nchunk);
-        chunkq.reserve(4 * (size_t)nchunk);
-        unit.reserve(4 * (size_t)nunit);
-        unit_off.reserve(3 * (size_t)nunit);
-    }
+    const int ns = ctx->ns, n = ctx->n;
+    std::vector<int> sao(ns), sL(ns);
     for (int X = 0; X < ns; ++X) {
-        shell[4 * X + 0] = ctx->shells[X].ao;
-        shell[4 * X + 1] = ctx->shells[X].L;
-        shell[4 * X + 2] = (int)(wg.size() / 6);
-        const int nX = ctx->shells[X].L ? 3 : 1;
-        long long acc = 0;
-        int ch0 = (int)(chunk.size() / 4), row0 = -1, row1 = -1;
-        auto close = [&]() {
-            const int ch1 = (int)(chunk.size() / 4);
-            if (ch1 > ch0) {
-                wg.push_back(X);
-                wg.push_back(ch0);
-                wg.push_back(ch1);
-                wg.push_back(row0);
-                wg.push_back(row1 - row0);
-                wg.push_back((int)(wg.size() / 6));   // partial slot: the workgroups of X stay contiguous
-                wg_cost.push_back(acc);
-            }
-            ch0 = ch1;
-            row0 = row1 = -1;
-            acc = 0;
-        };
-        for (int Y = 0; Y < ns; ++Y) {
-            const size_t seg = ((size_t)X * ns + Y) * 3;
-            const int cnt[3] = {ctx->segcnt[seg], ctx->segcnt[seg + 1], ctx->segcnt[seg + 2]};
-            if (cnt[0] + cnt[1] + cnt[2] == 0) continue;
-            const int aoY = ctx->shells[Y].ao, nY = ctx->shells[Y].L ? 3 : 1;
-            if (row0 >= 0 && aoY + nY - row0 > ctx->dig_rows) close();
-            unit.push_back(Y);
-            for (int ck = 0; ck < 3; ++ck) unit.push_back(cnt[ck]);
-            const int p = ctx->pair_of[(size_t)X * ns + Y];
-            const bool fwd = ctx->pairs[p].sa == X;
-            const int cp = ctx->pair_cls[p], rp = ctx->pair_rank[p];
-            const double qrow = ctx->schwarz[p];
-            for (int ck = 0; ck < 3; ++ck) {
-                const int start = ctx->segstart[seg + ck], end = start + cnt[ck];
-                const int row = nY * (ck == 0 ? 1 : ck == 1 ? 3 : 9);
-                // offset such that the block of the ket with rank r sits at off + r * block
-                const long long off = ctx->segoff[seg + ck] - (long long)start * nX * row;
-                unit_off.push_back(off);
-                const int step = 64 * kets_per_lane(row);
-                // role ranges of the segment's kets: [r0, r1) with flags
-                // 1 = bra role, 2 = forward orientation, 4 = the diagonal quartet
-                int cut[4] = {start, end, end, end}, flag[3] = {0, 0, 0};
-                const int fo = fwd ? 2 : 0;
-                if (ck < cp) flag[0] = 1 | fo;
-                else if (ck > cp) flag[0] = 0 | fo;
-                else {
-                    // same class: kets below the pair's rank are bras of it,
-                    // the pair itself is the diagonal quartet, the rest its kets
-                    const int d0 = std::min(std::max(rp, start), end);
-                    cut[1] = d0;
-                    cut[2] = std::min(d0 + (rp >= start && rp < end ? 1 : 0), end);
-                    flag[0] = 0 | fo;
-                    flag[1] = 1 | 4 | fo;
-                    flag[2] = 1 | fo;
-                }
-                for (int g = 0; g < 3; ++g)
-                    for (int r0 = cut[g]; r0 < cut[g + 1]; r0 += step) {
-                        // a workgroup closes after about kTarget values, also
-                        // inside a unit: chunks are independent
-                        if (row0 < 0) row0 = aoY;
-                        row1 = aoY + nY;
-                        const int nk = std::min(step, cut[g + 1] - r0);
-                        chunk.push_back((int)(unsigned)(off & 0xffffffffLL));
-                        chunk.push_back((int)(off >> 32));
-                        chunk.push_back(aoY | (ctx->shells[Y].L << 15) | (ck << 16) | (nk << 18) | (flag[g] << 28));
-                        chunk.push_back(r0);
-                        {
-                            const auto &ids = ctx->pair_ids[ck];
-                            const float up = std::numeric_limits<float>::infinity();
-                            chunkq.push_back(std::nextafter((float)(qrow * ctx->schwarz[ids[r0]]), up));
-                            chunkq.push_back(std::nextafter((float)(qrow * ctx->schwarz[ids[r0 + nk - 1]]), up));
-                            chunkq.push_back(std::nextafter((float)qrow, up));
-                            chunkq.push_back(0.0f);
-                        }
-                        acc += (long long)nX * row * nk;
-                        if (acc >= kTarget) close();
-                    }
-            }
-        }
-        close();
-        shell[4 * X + 3] = (int)(wg.size() / 6);
+        sao[X] = ctx->shells[X].ao;
+        sL[X] = ctx->shells[X].L;
     }
-    // launch order: most values first, so that the long workgroups do not
-    // form the tail of the pass
-    {
-        const size_t nw = wg_cost.size();
-        std::vector<int> order(nw);
-        for (size_t i = 0; i < nw; ++i) order[i] = (int)i;
-        std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return wg_cost[p] > wg_cost[q]; });
-        std::vector<int> sorted(wg.size());
-        for (size_t i = 0; i < nw; ++i)
-            for (int k = 0; k < 6; ++k) sorted[6 * i + k] = wg[6 * (size_t)order[i] + k];
-        wg.swap(sorted);
-    }
-    ctx->n_wg = (int)(wg.size() / 6);
-    ctx->n_chunks = (int)(chunk.size() / 4);
-    ctx->n_units = (int)(unit.size() / 4);
-    // ket metadata in Schwarz-rank order per class
-    std::vector<int> meta[3];
+    // LDS rows: the G rows (FP64) and the D rows (FP64 for the plain pass)
+    const int max_rows = std::min(64, (int)(kDigestLds / (2 * sizeof(double) * (size_t)n)));
+    if (max_rows < 6)
+        return set_error(ctx, MYQC_E_ARG, "digestion: n=%d leaves fewer than 6 rows of G and D in %zu bytes of LDS", n,
+                         kDigestLds);
+    DigestPlan &pl = ctx->plan;
+    const std::string e =
+        build_digest_schedule(pl, sao, sL, ctx->pairs, ctx->pair_ids, ctx->schwarz, max_rows, 1LL << 15);
+    if (!e.empty()) return set_error(ctx, MYQC_E_ARG, "%s", e.c_str());
+    ctx->n_wg = (int)(pl.wg.size() / kWgInts);
+    MYQC_CHECK(upload(ctx, &ctx->d_wg, pl.wg));
+    MYQC_CHECK(upload(ctx, &ctx->d_unit, pl.unit));
+    MYQC_CHECK(upload(ctx, &ctx->d_unit_off, pl.unit_off));
+    MYQC_CHECK(upload(ctx, &ctx->d_chunk, pl.chunk));
+    MYQC_CHECK(upload(ctx, &ctx->d_row_ao, pl.row_ao));
+    MYQC_CHECK(upload(ctx, &ctx->d_row_list, pl.row_list));
+    MYQC_CHECK(upload(ctx, &ctx->d_row_list_off, pl.row_list_off));
+    MYQC_CHECK(upload(ctx, &ctx->d_jk_list, pl.jk_list));
+    MYQC_CHECK(upload(ctx, &ctx->d_jk_list_off, pl.jk_list_off));
+    MYQC_CHECK(upload(ctx, &ctx->d_jk_group, pl.jk_group));
+    MYQC_CHECK(upload(ctx, &ctx->d_row_group, pl.row_group));
+    MYQC_CHECK(upload(ctx, &ctx->d_jkoff, pl.jkoff));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_jk_part, sizeof(double) * kJkSlot * std::max<size_t>(pl.n_jk_slots, 1)));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_jk, sizeof(double) * std::max(pl.jk_base[3], 1)));
+    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * (size_t)std::max(pl.n_rows, 1) * n));
+    // ket metadata and Schwarz factors in rank order per class
     for (int c = 0; c < 3; ++c) {
         const auto &ids = ctx->pair_ids[c];
         if (ids.empty()) continue;
-        meta[c].resize(ids.size());
+        std::vector<int> meta(ids.size());
+        std::vector<float> qrank(ids.size());
         for (size_t r = 0; r < ids.size(); ++r) {
             const ShellPair &sp = ctx->pairs[ids[r]];
-            meta[c][r] = sp.ao_a | (sp.ao_b << 15) | (sp.same << 30);
+            meta[r] = sp.ao_a | (sp.ao_b << 15);
+            // rounded up, so that the screen never drops a ket the exact factor keeps
+            qrank[r] = std::nextafter((float)ctx->schwarz[ids[r]], std::numeric_limits<float>::infinity());
         }
-        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_meta[c], sizeof(int) * meta[c].size()));
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_meta[c], meta[c].data(), sizeof(int) * meta[c].size(),
-                                     hipMemcpyHostToDevice, ctx->stream));
+        MYQC_CHECK(upload(ctx, &ctx->d_meta[c], meta));
+        MYQC_CHECK(upload(ctx, &ctx->d_qrank[c], qrank));
     }
-    // Schwarz factors by rank (rounded up, so the screen never drops a
-    // partner that the exact factor keeps)
-    std::vector<float> qrank[3];
-    for (int c = 0; c < 3; ++c) {
-        const auto &ids = ctx->pair_ids[c];
-        if (ids.empty()) continue;
-        qrank[c].resize(ids.size());
-        for (size_t r = 0; r < ids.size(); ++r)
-            qrank[c][r] = std::nextafter((float)ctx->schwarz[ids[r]], std::numeric_limits<float>::infinity());
-        MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_qrank[c], sizeof(float) * qrank[c].size()));
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_qrank[c], qrank[c].data(), sizeof(float) * qrank[c].size(),
-                                     hipMemcpyHostToDevice, ctx->stream));
-    }
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_chunkq, sizeof(float) * std::max<size_t>(chunkq.size(), 4)));
-    if (!chunkq.empty())
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_chunkq, chunkq.data(), sizeof(float) * chunkq.size(),
-                                     hipMemcpyHostToDevice, ctx->stream));
-    std::vector<int> ao_shell(ctx->n);
-    for (int X = 0; X < ns; ++X)
+    std::vector<int> shell(4 * ns, 0), ao_shell(n);
+    for (int X = 0; X < ns; ++X) {
+        shell[4 * X] = ctx->shells[X].ao;
+        shell[4 * X + 1] = ctx->shells[X].L;
         for (int k = 0; k < (ctx->shells[X].L ? 3 : 1); ++k) ao_shell[ctx->shells[X].ao + k] = X;
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_wg, sizeof(int) * std::max<size_t>(wg.size(), 1)));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit, sizeof(int) * std::max<size_t>(unit.size(), 1)));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_unit_off, sizeof(long long) * std::max<size_t>(unit_off.size(), 1)));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_chunk, sizeof(int) * std::max<size_t>(chunk.size(), 1)));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_shell, sizeof(int) * shell.size()));
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_ao_shell, sizeof(int) * ao_shell.size()));
-    if (!wg.empty())
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_wg, wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice, ctx->stream));
-    if (!unit.empty()) {
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_unit, unit.data(), sizeof(int) * unit.size(), hipMemcpyHostToDevice,
-                                     ctx->stream));
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_unit_off, unit_off.data(), sizeof(long long) * unit_off.size(),
-                                     hipMemcpyHostToDevice, ctx->stream));
-        MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_chunk, chunk.data(), sizeof(int) * chunk.size(), hipMemcpyHostToDevice,
-                                     ctx->stream));
     }
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_shell, shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
-    MYQC_HIP(ctx, hipMemcpyAsync(ctx->d_ao_shell, ao_shell.data(), sizeof(int) * ao_shell.size(),
-                                 hipMemcpyHostToDevice, ctx->stream));
+    MYQC_CHECK(upload(ctx, &ctx->d_shell, shell));
+    MYQC_CHECK(upload(ctx, &ctx->d_ao_shell, ao_shell));
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -792,9 +581,9 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     free_eri(ctx, true);
     MYQC_CHECK(build_classes(ctx, tol));
     hc.mark("eri_store: quartet classes");
-    MYQC_CHECK(build_segments(ctx, tol));
-    hc.mark("eri_store: segments");
-    const size_t nx = (size_t)std::max<long long>(ctx->n_expanded, 1);
+    MYQC_CHECK(build_layout(ctx, tol));
+    hc.mark("eri_store: layout");
+    const size_t nx = (size_t)std::max<long long>(ctx->n_stored, 1);
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x64, sizeof(double) * nx));
     if (want32) MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_x32, sizeof(float) * nx));
     else dfree((void **)&ctx->d_x32);   // a kept buffer may be smaller than this build
@@ -823,8 +612,6 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     hc.mark("eri_store: allocations and launches");
     // the digestion schedule is built while the kernels run
     MYQC_CHECK(build_schedule(ctx));
-    const size_t part = (size_t)std::max(ctx->n_wg, 1) * 3 * ctx->n;
-    MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_partial, sizeof(double) * part));
     hc.mark("eri_store: schedule (host, beside the kernels)");
     MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hc.mark("eri_store: kernels");

@@ -3,37 +3,29 @@
 // The reference's two-electron part of the Fock matrix is
 //     G(i,j) = sum_kl D(k,l)[XX(i,j,k,l) - XX(i,k,j,l)/4 - XX(i,l,j,k)/4]
 // (src/I2G/RHFI2G.f90:80-90), which only sees D through D + D^T, so it is
-// evaluated for the symmetrised D.  Each unique shell quartet (AB|CD), with
-// (A,B) its bra pair (the higher class, or the lower Schwarz rank in one
-// class), is stored three times, in segments of kets sorted by Schwarz rank:
-// under the ordered pairs (A,B) and (B,A), and under (C,D) in the ket pair's
-// stored orientation.  A copy under the ordered pair (X,Y) with ket (U,W)
-// updates only rows of shell X:
-//     J: A(i,j) += wJ f D(k,l) (ij|kl),           f = 2 unless U == W
-//     K: A(i,k) -= wK D(j,l) (ij|kl) / 2,  and if U != W  A(i,l) -= wK D(j,k) (ij|kl) / 2
-// for i in X, j in Y, and G = A + A^T.  The weights split the quartet's
-// terms so that each lands once: the bra's forward copy adds J and its K rows,
-// the bra's reverse copy its K rows only, the ket's copy J only (wK = 0);
-// wJ halves for a same-shell row pair, wK for the diagonal quartet (P|P).
-// (tests/hostcheck states this split on the host and checks it against
-// RHFI2G.)  A workgroup owns the rows of one shell A for a run of about 2^15
-// stored values: it stages in LDS the rows of D of the B shells it meets (at
-// most dig_rows AOs), then each wavefront takes chunks of kets of one (A, B,
-// ket class, role) range in turn, with no barrier between chunks.  A lane
-// takes 1-8 kets of a chunk (more for the small classes) and issues all their
-// loads of one bra row before their arithmetic.  J is reduced across the
-// wavefront and K is added per ket, both into FP64 LDS rows of A (FP32
-// values are accumulated in FP64 too); the rows go to the workgroup's own
-// partial slot.  Workgroups launch longest first.  A gather sums the partial
-// rows of each shell in a fixed order: no global atomics; the LDS atomics
-// make the summation order, and so the last bits, vary from run to run.
+// evaluated for the symmetrised D.  Each unique shell quartet is stored once,
+// pre-weighted, under its bra pair (digest_plan.h describes the layout, the
+// schedule and the six updates each stored element makes).  A workgroup
+// holds in LDS the rows of G of its row shell A and of its units' B shells,
+// and the same rows of D; each wavefront takes column chunks (one ket class,
+// W consecutive Schwarz ranks, KPL kets per lane) and, per chunk, every unit
+// that has kets there.  For a unit it streams the bra rows of its lanes'
+// blocks, loads issued before the arithmetic:
+//   J, bra side:  reduced across the wavefront into row a, column b;
+//   J, ket side:  summed per lane over the chunk's units, written once per
+//                 chunk to the chunk's partial slot;
+//   K:            four updates per element into rows a and b (LDS atomics).
+// FP32 values (the incremental passes) are accumulated in FP64.  A gather
+// sums, per AO row, the partial rows of the workgroups holding it, and per
+// ket, the slots of the chunks that cover it; G = A + A^T.  No global
+// atomics; the LDS atomics make the summation order, and so the last bits,
+// vary from run to run.
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
-#include <rocsolver/rocsolver.h>
-
 #include "context.h"
+#include "digest_plan.h"
 
 namespace myqc {
 
@@ -44,238 +36,336 @@ template <> struct PL<2> { static constexpr int a = 1, b = 1; };
 
 template <typename TV>
 struct DigArgs {
-    const int *meta0, *meta1, *meta2;         // ket AOs by class and Schwarz rank
+    const int *meta0, *meta1, *meta2;         // ket AOs by class and Schwarz rank: aoC | aoD << 15
     const TV *dk0, *dk1, *dk2;                // ket density blocks by class and rank (1, 3, 9 values)
-    const int *wg;                            // [w] = {A, chunk0, chunk1, first D row, D rows, partial slot}
-    const int *unit;                          // [u] = {B, c0, c1, c2}
-    const int4 *chunk;                        // [c] = {offset lo, offset hi, aoB | LB << 15 | class << 16 | kets << 18, first rank}
-    const long long *unit_off;                // [u][3]
-    const int *shell;                         // [s] = {ao, L, wg0, wg1}
-    const TV *x;                              // expanded values
+    const int *wg;                            // kWgInts per workgroup (digest_plan.h)
+    const int *unit;                          // kUnitInts per unit
+    const long long *unit_off;                // [u][3] value offsets
+    const int4 *chunk;                        // {class, first rank, kets, partial slot}
+    const int *row_ao;                        // AO of each partial row
+    const TV *x;                              // stored (pre-weighted) values
     const TV *D;                              // symmetric density (or change), n x n
-    double *partial;                          // [w][3][n]
+    double *partial;                          // [row][n]
+    double *jk_part;                          // [slot][kJkSlot]
     int n;
     double jscale;                            // 1, or 0 for the exchange part alone
-    // density-weighted screen of the incremental (FP32) passes: a partner of
-    // rank r in class c is skipped when q_row * q_c[r] * dmax < qtau (0: no
-    // screen); q_c are the partners' Schwarz factors by rank (descending)
+    // density-weighted screen of the incremental (FP32) passes: a unit's ket
+    // of rank r in class c is skipped when q_unit * q_c[r] * dmax < qtau
+    // (0: no screen); q_c are the kets' Schwarz factors by rank (descending)
     const float *q0, *q1, *q2;
-    const float4 *chunkq;                     // [c] = {q_row q_first, q_row q_last, q_row, 0}, rounded up
     const unsigned *dmax;                     // max |D| of this pass (device, float bits)
     double qtau;
 };
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// The schedule's records are read-only for the whole launch: read through
+// the constant address space, so that the wave-uniform reads of a unit,
+// chunk or workgroup record become scalar loads and their values stay in
+// scalar registers.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)p;
 }
 
-// Kets per lane of a chunk: enough that every lane has several loads of one
-// bra row in flight (nb*nc*nd values per ket and row).
-__host__ __device__ constexpr int kets_per_lane(int row) { return row == 1 ? 8 : row == 3 ? 4 : row == 9 ? 2 : 1; }
+// Sum over the 64 lanes of a wavefront (all active), returned wave-uniform:
+// within each row of 16 lanes by four DPP exchanges (lanes xor 1, xor 2,
+// the mirrored half-row, the mirrored row), then the four row sums read from
+// lanes 0, 16, 32, 48.  No LDS traffic.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+    return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <typename T>
+__device__ __forceinline__ T wave_total(T v) {
+    v = dpp_add<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = dpp_add<0x141>(v);   // row_half_mirror
+    v = dpp_add<0x140>(v);   // row_mirror
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
 
-// One chunk of kets of class CK against the bra (A, B): lane l takes kets
-// r0 + l + 64 j, j < KPL.  J is reduced across the wavefront into the LDS
-// rows of A, K added per ket into them.  s_drow holds D's rows of shell B.
-// The bra rows are streamed one at a time, with the loads of all the lane's
-// kets issued before their arithmetic.
-template <int LA, int LB, int CK, typename TV>
-__device__ __forceinline__ void digest_chunk(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int r0, int nk,
-                                             long long off, int aoB, double wJ, double wK, double thr) {
-    constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
-    constexpr int ROW = nb * nc * nd, BLK = na * ROW, KPL = kets_per_lane(ROW);
+// Bra row a = ia of one unit (A, B) over the lanes' kets of one column chunk.
+// s_acc / s_d: the workgroup's LDS rows of G and D (A's rows first, B's from
+// row sb); act[j]: the lane's ket j is in the unit's range (and passes the
+// screen), ri[j]: its rank (an inactive lane reads the unit's first ket, in
+// range, and drops it); dA[j]: D(a, c), D(a, d) at the ket's columns; jk /
+// ka: the lane's ket-side J sums and its row-a K sums, carried over the
+// chunk's units (every unit of a workgroup shares row shell A); xu + r + e
+// len: element e of the unit's block of ket rank r.
+template <int LB, int CK, typename TV, int KPL = dig_kpl(CK), int NCD = dig_ncd(CK),
+          int NK = ncart(PL<CK>::a) + ncart(PL<CK>::b)>
+__device__ __forceinline__ void digest_row(const DigArgs<TV> &a, double *s_acc, const TV *s_d, int ia,
+                                           const int (&ri)[KPL], const bool (&act)[KPL], const int (&aoC)[KPL],
+                                           const int (&aoD)[KPL], const TV (&dq)[KPL][NCD], const TV (&dA)[KPL][NK],
+                                           double (&jk)[KPL][NCD], TV (&ka)[KPL][NK], const TV *xu, int len, int aoB,
+                                           int sb) {
+    constexpr int nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
+    static_assert(nc * nd == NCD && nc + nd == NK, "ket block");
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-    const int *meta = CK == 0 ? a.meta0 : CK == 1 ? a.meta1 : a.meta2;
-    const TV *dkc = CK == 0 ? a.dk0 : CK == 1 ? a.dk1 : a.dk2;
-    const float *qk = CK == 0 ? a.q0 : CK == 1 ? a.q1 : a.q2;
-    bool act[KPL];
-    int aoC[KPL], aoD[KPL], same[KPL];
-    TV dq[KPL][nc * nd], dbd[KPL][nb * nd], dbc[KPL][nb * nc];
-    const TV *v[KPL];
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) {
-        const int k = lane + 64 * j;
-        // thr > 0: the screen of the incremental passes (partners whose
-        // integrals with this row pair are all below it drop out)
-        act[j] = k < nk && (thr <= 0.0 || (double)qk[r0 + k] >= thr);
-        const int r = r0 + (act[j] ? k : 0);
-        const int m = meta[r];
-        aoC[j] = m & 0x7fff;
-        aoD[j] = (m >> 15) & 0x7fff;
-        same[j] = m >> 30;
-        v[j] = a.x + off + (long long)r * BLK;
-#pragma unroll
-        for (int e = 0; e < nc * nd; ++e) dq[j][e] = act[j] ? dkc[(size_t)r * (nc * nd) + e] : (TV)0;
-    }
-#pragma unroll
-    for (int j = 0; j < KPL; ++j)
-#pragma unroll
-        for (int b = 0; b < nb; ++b) {
-#pragma unroll
-            for (int d = 0; d < nd; ++d) dbd[j][b * nd + d] = s_drow[b * n + aoD[j] + d];
-#pragma unroll
-            for (int c = 0; c < nc; ++c) dbc[j][b * nc + c] = s_drow[b * n + aoC[j] + c];
-        }
 #pragma unroll 1
-    for (int ia = 0; ia < na; ++ia) {
-        TV x[KPL][ROW];
+    for (int ib = 0; ib < nb; ++ib) {
+        TV x[KPL][NCD];
+#pragma unroll
+        for (int e = 0; e < NCD; ++e) {
+            const TV *xe = xu + (long long)((ia * nb + ib) * NCD + e) * len;
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) {
+                const TV val = xe[ri[j]];
+                x[j][e] = act[j] ? val : (TV)0;
+            }
+        }
+        // J, bra side: A(a, b) += 2 sum_cd D(c, d) x
+        TV t = 0;
 #pragma unroll
         for (int j = 0; j < KPL; ++j)
 #pragma unroll
-            for (int e = 0; e < ROW; ++e) x[j][e] = act[j] ? v[j][ia * ROW + e] : (TV)0;
-        TV jr[nb];
-#pragma unroll
-        for (int ib = 0; ib < nb; ++ib) jr[ib] = 0;
+            for (int e = 0; e < NCD; ++e) t += dq[j][e] * x[j][e];
+        t = wave_total(t);
+        if (lane == 0 && a.jscale != 0.0) atomicAdd(&s_acc[ia * n + aoB + ib], 2.0 * a.jscale * (double)t);
+        const TV dab = s_d[ia * n + aoB + ib];   // D(a, b), wave-uniform
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
-            const TV fj = same[j] ? (TV)1 : (TV)2;
+            // J, ket side: A(c, d) += 2 D(a, b) x
 #pragma unroll
-            for (int ib = 0; ib < nb; ++ib) {
-                TV js = 0;
+            for (int e = 0; e < NCD; ++e) jk[j][e] += (double)(dab * x[j][e]);
+            // K: A(a, c) -= D(b, d) x / 2, A(a, d) -= D(b, c) x / 2 (into ka),
+            //    A(b, c) -= D(a, d) x / 2, A(b, d) -= D(a, c) x / 2 (LDS)
+            TV dB[NK], kb[NK];
 #pragma unroll
-                for (int c = 0; c < nc; ++c)
+            for (int c = 0; c < nc; ++c) dB[c] = s_d[(sb + ib) * n + aoC[j] + c];
 #pragma unroll
-                    for (int d = 0; d < nd; ++d) js += dq[j][c * nd + d] * x[j][(ib * nc + c) * nd + d];
-                jr[ib] += fj * js;
-            }
-            if (!act[j] || wK == 0.0) continue;   // (wK is uniform across the wavefront)
-            // G(i, k) -= 1/2 sum_{b,d} D(b,d) (i b|k d)
+            for (int d = 0; d < nd; ++d) dB[nc + d] = s_d[(sb + ib) * n + aoD[j] + d];
 #pragma unroll
-            for (int c = 0; c < nc; ++c) {
-                TV ks = 0;
+            for (int k = 0; k < NK; ++k) kb[k] = 0;
 #pragma unroll
-                for (int ib = 0; ib < nb; ++ib)
-#pragma unroll
-                    for (int d = 0; d < nd; ++d) ks += dbd[j][ib * nd + d] * x[j][(ib * nc + c) * nd + d];
-                atomicAdd(&s_rows[ia * n + aoC[j] + c], -0.5 * wK * (double)ks);
-            }
-            // the ket's other orientation: G(i, l) -= 1/2 sum_{b,c} D(b,c) (i b|k l)
-            if (!same[j]) {
+            for (int c = 0; c < nc; ++c)
 #pragma unroll
                 for (int d = 0; d < nd; ++d) {
-                    TV ks = 0;
-#pragma unroll
-                    for (int ib = 0; ib < nb; ++ib)
-#pragma unroll
-                        for (int c = 0; c < nc; ++c) ks += dbc[j][ib * nc + c] * x[j][(ib * nc + c) * nd + d];
-                    atomicAdd(&s_rows[ia * n + aoD[j] + d], -0.5 * wK * (double)ks);
+                    const TV xv = x[j][c * nd + d];
+                    ka[j][c] += dB[nc + d] * xv;
+                    ka[j][nc + d] += dB[c] * xv;
+                    kb[c] += dA[j][nc + d] * xv;
+                    kb[nc + d] += dA[j][c] * xv;
                 }
+            if (act[j]) {
+#pragma unroll
+                for (int c = 0; c < nc; ++c) atomicAdd(&s_acc[(sb + ib) * n + aoC[j] + c], -0.5 * (double)kb[c]);
+#pragma unroll
+                for (int d = 0; d < nd; ++d) atomicAdd(&s_acc[(sb + ib) * n + aoD[j] + d], -0.5 * (double)kb[nc + d]);
             }
         }
+    }
+}
+
+// One column chunk: the lanes' kets; then for each bra row a of the row
+// shell A, every unit of the workgroup whose rank range meets the chunk, and
+// row a's K sums (LDS); then the chunk's ket-side partial slot.
+template <int LA, int CK, typename TV>
+__device__ __forceinline__ void digest_column(const DigArgs<TV> &a, double *s_acc, const TV *s_d, int4 ch, int u0,
+                                              int u1, double dmax, bool screen) {
+    constexpr int na = ncart(LA), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b), NCD = nc * nd, NK = nc + nd;
+    constexpr int KPL = dig_kpl(CK);
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const int r0 = ch.y, nk = ch.z;
+    const int *meta = CK == 0 ? a.meta0 : CK == 1 ? a.meta1 : a.meta2;
+    const TV *dkc = CK == 0 ? a.dk0 : CK == 1 ? a.dk1 : a.dk2;
+    const float *qk = CK == 0 ? a.q0 : CK == 1 ? a.q1 : a.q2;
+    int rk[KPL], aoC[KPL], aoD[KPL];
+    bool valid[KPL];
+    TV dq[KPL][NCD];
+    double jk[KPL][NCD];
 #pragma unroll
-        for (int ib = 0; ib < nb; ++ib) {
-            const TV t = wave_sum(jr[ib]);
-            if (lane == 0 && wJ != 0.0) atomicAdd(&s_rows[ia * n + aoB + ib], wJ * (double)t);
+    for (int j = 0; j < KPL; ++j) {
+        const int k = lane + 64 * j;
+        valid[j] = k < nk;
+        rk[j] = r0 + (valid[j] ? k : 0);
+        const int m = meta[rk[j]];
+        aoC[j] = m & 0x7fff;
+        aoD[j] = (m >> 15) & 0x7fff;
+#pragma unroll
+        for (int e = 0; e < NCD; ++e) {
+            dq[j][e] = valid[j] ? dkc[(size_t)rk[j] * NCD + e] : (TV)0;
+            jk[j][e] = 0.0;
         }
     }
+#pragma unroll 1
+    for (int ia = 0; ia < na; ++ia) {
+        TV dA[KPL][NK], ka[KPL][NK];
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+#pragma unroll
+            for (int c = 0; c < nc; ++c) dA[j][c] = s_d[ia * n + aoC[j] + c];
+#pragma unroll
+            for (int d = 0; d < nd; ++d) dA[j][nc + d] = s_d[ia * n + aoD[j] + d];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) ka[j][k] = 0;
+        }
+#pragma unroll 1
+        for (int u = u0; u < u1; ++u) {
+            const cptr<int> ur = as_const(a.unit) + (size_t)kUnitInts * u;
+            const int lo = ur[3 + 2 * CK], hi = ur[4 + 2 * CK];
+            if (hi <= r0 || lo >= r0 + nk) continue;   // (wave-uniform)
+            double thr = 0.0;
+            if (screen) {
+                // |x| <= q_unit q_ket, kets in descending order: the unit drops
+                // out whole when its first ket in the chunk is below the
+                // screen, runs whole when its last one is above, else tests
+                // ket by ket
+                const double qu = (double)__int_as_float(ur[9]) * dmax;
+                const int rf = max(lo, r0), rl = min(hi, r0 + nk) - 1;
+                if ((double)qk[rf] * qu < a.qtau) continue;
+                if ((double)qk[rl] * qu < a.qtau) thr = a.qtau / qu;
+            }
+            bool act[KPL];
+            int ri[KPL];
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) {
+                act[j] = valid[j] && rk[j] >= lo && rk[j] < hi && (thr <= 0.0 || (double)qk[rk[j]] >= thr);
+                ri[j] = act[j] ? rk[j] : lo;
+            }
+            const int aoB = ur[1] & 0x7fff, LB = ur[1] >> 15, sb = ur[2];
+            const TV *xu = a.x + as_const(a.unit_off)[3 * (size_t)u + CK];
+            if (LB == 0) digest_row<0, CK>(a, s_acc, s_d, ia, ri, act, aoC, aoD, dq, dA, jk, ka, xu, hi - lo, aoB, sb);
+            else digest_row<1, CK>(a, s_acc, s_d, ia, ri, act, aoC, aoD, dq, dA, jk, ka, xu, hi - lo, aoB, sb);
+        }
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            if (!valid[j]) continue;
+#pragma unroll
+            for (int c = 0; c < nc; ++c) atomicAdd(&s_acc[ia * n + aoC[j] + c], -0.5 * (double)ka[j][c]);
+#pragma unroll
+            for (int d = 0; d < nd; ++d) atomicAdd(&s_acc[ia * n + aoD[j] + d], -0.5 * (double)ka[j][nc + d]);
+        }
+    }
+    double *dst = a.jk_part + (size_t)ch.w * kJkSlot;
+    const double js = 2.0 * a.jscale;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j)
+#pragma unroll
+        for (int e = 0; e < NCD; ++e) dst[(lane + 64 * j) * NCD + e] = js * jk[j][e];
 }
 
 template <int LA, typename TV>
-__device__ __forceinline__ void digest_chunk_la(const DigArgs<TV> &a, double *s_rows, const TV *s_drow, int LB, int ck,
-                                                int r0, int nk, long long off, int aoB, double wJ, double wK,
-                                                double thr) {
-    if (LB == 0) {
-        if (ck == 0) digest_chunk<LA, 0, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
-        else if (ck == 1) digest_chunk<LA, 0, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
-        else digest_chunk<LA, 0, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
-    } else {
-        if (ck == 0) digest_chunk<LA, 1, 0>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
-        else if (ck == 1) digest_chunk<LA, 1, 1>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
-        else digest_chunk<LA, 1, 2>(a, s_rows, s_drow, r0, nk, off, aoB, wJ, wK, thr);
-    }
+__device__ __forceinline__ void digest_column_la(const DigArgs<TV> &a, double *s_acc, const TV *s_d, int4 ch, int u0,
+                                                 int u1, double dmax, bool screen) {
+    if (ch.x == 0) digest_column<LA, 0>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
+    else if (ch.x == 1) digest_column<LA, 1>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
+    else digest_column<LA, 2>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
 }
 
-// One workgroup per run of units of one row shell A: it stages the D rows of
-// the units' B shells once, then each wavefront takes chunks in turn (no
-// barrier between them; the next chunk's descriptor is loaded ahead), and
-// the rows of A it accumulated go to its partial slot.
+// One workgroup: stage its D rows, run its column chunks (wavefront w takes
+// chunks w, w + 4, ...), write its rows of A to its partial rows.
 template <typename TV>
 __global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
     extern __shared__ unsigned char smem[];
     const int n = a.n;
-    double *s_rows = reinterpret_cast<double *>(smem);
-    TV *s_drow = reinterpret_cast<TV *>(s_rows + 3 * n);
-    const int w = blockIdx.x;
-    const int A = a.wg[6 * w], ch0 = a.wg[6 * w + 1], ch1 = a.wg[6 * w + 2];
-    const int row0 = a.wg[6 * w + 3], nrows = a.wg[6 * w + 4], slot = a.wg[6 * w + 5];
-    const int LA = a.shell[4 * A + 1], aoA = a.shell[4 * A];
-    const int na = LA ? 3 : 1;
-    for (int i = threadIdx.x; i < na * n; i += blockDim.x) s_rows[i] = 0;
-    // D(row0 + b, j) = D(j, row0 + b): read columns, coalesced
+    const cptr<int> wr = as_const(a.wg) + (size_t)kWgInts * blockIdx.x;
+    const int LA = wr[0] >> 15, u0 = wr[1], u1 = wr[2], c0 = wr[3], c1 = wr[4], nrows = wr[5], row0 = wr[6];
+    double *s_acc = reinterpret_cast<double *>(smem);
+    TV *s_d = reinterpret_cast<TV *>(s_acc + (size_t)nrows * n);
+#pragma unroll 1
     for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) {
-        const int b = i / n, j = i - b * n;
-        s_drow[i] = a.D[j + (size_t)n * (row0 + b)];
+        s_acc[i] = 0.0;
+        const int r = i / n, j = i - r * n;
+        s_d[i] = a.D[j + (size_t)n * a.row_ao[row0 + r]];   // D(row, j) = D(j, row): coalesced
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6;
     const bool screen = a.qtau > 0.0;
     const double dmax = screen ? (double)__uint_as_float(*a.dmax) : 0.0;
-    int c = ch0 + wave;
-    int4 d = c < ch1 ? a.chunk[c] : make_int4(0, 0, 0, 0);
-    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 q = screen && c < ch1 ? a.chunkq[c] : z4;
-    while (c < ch1) {
-        const int cn = c + 4;
-        const int4 dn = cn < ch1 ? a.chunk[cn] : make_int4(0, 0, 0, 0);
-        const float4 qn = screen && cn < ch1 ? a.chunkq[cn] : z4;
-        const long long off = (long long)(unsigned)d.x | ((long long)d.y << 32);
-        const int aoB = d.z & 0x7fff, LB = (d.z >> 15) & 1, ck = (d.z >> 16) & 3, nk = (d.z >> 18) & 0x3ff;
-        double thr = 0.0;
-        if (screen) {
-            // |integral| <= q_row q_partner, partners in descending order: a
-            // chunk whose first partner is below the screen drops out whole,
-            // one whose last is above it runs whole, and only a chunk that
-            // straddles it tests its partners one by one (thr > 0)
-            if ((double)q.x * dmax < a.qtau) {
-                c = cn;
-                d = dn;
-                q = qn;
-                continue;
-            }
-            if ((double)q.y * dmax < a.qtau) thr = a.qtau / ((double)q.z * dmax);
-        }
-        const int role = (d.z >> 28) & 7;   // 1: bra role, 2: forward orientation, 4: diagonal quartet
-        // weights of the three-copy split (G = A + A^T): the bra pair's two
-        // orientations share its J and each adds its K rows, the ket pair's
-        // copy adds J only; halves for a same-shell row pair and the diagonal
-        const double half_same = (aoB == aoA) ? 0.5 : 1.0;
-        const double wJ = a.jscale * ((role & 1) ? ((role & 2) ? half_same : 0.0) : half_same);
-        const double wK = (role & 1) ? ((role & 4) ? 0.5 : 1.0) : 0.0;
-        const TV *drow = s_drow + (size_t)(aoB - row0) * n;
-        if (LA == 0) digest_chunk_la<0>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK, thr);
-        else digest_chunk_la<1>(a, s_rows, drow, LB, ck, d.w, nk, off, aoB, wJ, wK, thr);
-        c = cn;
-        d = dn;
-        q = qn;
+    // the wavefront's index, made visibly uniform so that its chunk records
+    // are scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll 1
+    for (int c = c0 + wave; c < c1; c += 4) {
+        const cptr<int> cr = as_const(reinterpret_cast<const int *>(a.chunk)) + 4 * (size_t)c;
+        const int4 ch = make_int4(cr[0], cr[1], cr[2], cr[3]);
+        if (LA == 0) digest_column_la<0>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
+        else digest_column_la<1>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
     }
     __syncthreads();
-    double *dst = a.partial + (size_t)slot * 3 * n;
-    for (int i = threadIdx.x; i < na * n; i += blockDim.x) dst[i] = s_rows[i];
+    double *dst = a.partial + (size_t)row0 * n;
+#pragma unroll 1
+    for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) dst[i] = s_acc[i];
 }
 
-// G(i,j) = sum over the workgroups of shell(i) of their partial rows
-__global__ void k_gather_rows(const double *__restrict__ partial, const int *__restrict__ shell,
-                              const int *__restrict__ ao_shell, int n, double *__restrict__ Graw) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-    if (j >= n) return;
-    const int A = ao_shell[i];
-    const int ia = i - shell[4 * A], w0 = shell[4 * A + 2], w1 = shell[4 * A + 3];
-    // four independent sums so that the loads of consecutive slots overlap
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-    int w = w0;
-    for (; w + 4 <= w1; w += 4)
+// The ket-side J of every pair: jk[base(c) + r ncd(c) + e] = sum of the
+// partial slots of the chunks that cover rank r of class c.  One workgroup
+// per group of at most kJkGroup slots of one block of W(c) ranks (positions
+// of the block across the threads, coalesced; four independent sums), added
+// into jk (zeroed before the pass) with one atomic per position.
+__global__ __launch_bounds__(256) void k_gather_jk(const double *__restrict__ jk_part, const int *__restrict__ list,
+                                                   const int *__restrict__ group, int3 base, int3 block0, int4 nps,
+                                                   double *__restrict__ jk) {
+    const int g = blockIdx.x;
+    const int blk = group[3 * g], k0 = group[3 * g + 1], k1 = group[3 * g + 2];
+    const int c = blk >= block0.z ? 2 : blk >= block0.y ? 1 : 0;
+    const int ncd = dig_ncd(c), W = dig_width(c), npos = W * ncd;
+    const int r0 = (blk - (c == 0 ? block0.x : c == 1 ? block0.y : block0.z)) * W;
+    const int np = c == 0 ? nps.x : c == 1 ? nps.y : nps.z;
+    const int b0 = c == 0 ? base.x : c == 1 ? base.y : base.z;
+    for (int pos = threadIdx.x; pos < npos; pos += blockDim.x) {
+        if (r0 + pos / ncd >= np) break;
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        int k = k0;
+        for (; k + 4 <= k1; k += 4)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s4[q] += partial[((size_t)(w + q) * 3 + ia) * n + j];
-    for (; w < w1; ++w) s4[0] += partial[((size_t)w * 3 + ia) * n + j];
-    const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-    Graw[i + (size_t)n * j] = s;
+            for (int q = 0; q < 4; ++q) s4[q] += jk_part[(size_t)list[k + q] * kJkSlot + pos];
+        for (; k < k1; ++k) s4[0] += jk_part[(size_t)list[k] * kJkSlot + pos];
+        atomicAdd(&jk[b0 + (size_t)r0 * ncd + pos], (s4[0] + s4[1]) + (s4[2] + s4[3]));
+    }
+}
+
+// A(i, j) += the partial rows of one group holding AO i (A zeroed before the
+// pass), and for the AO's first group the ket-side J of the pair (shell(i),
+// shell(j)) when it is stored in that orientation
+__global__ __launch_bounds__(128) void k_gather_rows(const double *__restrict__ partial, const int4 *__restrict__ group,
+                                                     const int *__restrict__ list, const double *__restrict__ jk,
+                                                     const int *__restrict__ jkoff, const int *__restrict__ ao_shell,
+                                                     const int *__restrict__ shell, int ns, int n,
+                                                     double *__restrict__ Graw) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int4 gr = group[blockIdx.y];
+    const int i = gr.x, k0 = gr.y, k1 = gr.z;
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    int k = k0;
+    for (; k + 4 <= k1; k += 4)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s4[q] += partial[(size_t)list[k + q] * n + j];
+    for (; k < k1; ++k) s4[0] += partial[(size_t)list[k] * n + j];
+    double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+    if (gr.w) {
+        const int X = ao_shell[i], Y = ao_shell[j];
+        const int o = jkoff[X * ns + Y];
+        if (o >= 0) s += jk[o + (i - shell[4 * X]) * (shell[4 * Y + 1] ? 3 : 1) + (j - shell[4 * Y])];
+    }
+    atomicAdd(&Graw[i + (size_t)n * j], s);
+}
+
+__global__ void k_zero(double *__restrict__ p, size_t cnt) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
 }
 
 // the kets' density blocks in Schwarz-rank order: dk[r*(nc*nd) + c*nd + d] =
@@ -349,6 +439,8 @@ int ensure_work(myqc_ctx *ctx) {
 template <typename TV>
 static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double *partial, double *Graw,
                          double jscale = 1.0, double qtau = 0.0) {
+    const DigestPlan &pl = ctx->plan;
+    const int n = ctx->n;
     const int np0 = (int)ctx->pair_ids[0].size(), np1 = (int)ctx->pair_ids[1].size(),
               np2 = (int)ctx->pair_ids[2].size();
     TV *dk0 = dk, *dk1 = dk + np0, *dk2 = dk + np0 + 3 * (size_t)np1;
@@ -357,8 +449,10 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
     for (int c = 0; c < 3; ++c)
         if (nps[c] > 0)
             k_pack_kets<TV><<<(nps[c] + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_sorted[c], nps[c],
-                                                                            c ? 3 : 1, c == 2 ? 3 : 1, D, ctx->n,
-                                                                            dks[c]);
+                                                                            c ? 3 : 1, c == 2 ? 3 : 1, D, n, dks[c]);
+    // the gathers add into A and the ket-side J
+    k_zero<<<128, 256, 0, ctx->stream>>>(Graw, (size_t)n * n);
+    k_zero<<<64, 256, 0, ctx->stream>>>(ctx->d_jk, (size_t)std::max(pl.jk_base[3], 1));
     if (ctx->n_wg > 0) {
         DigArgs<TV> a;
         a.meta0 = ctx->d_meta[0];
@@ -369,30 +463,38 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
         a.dk2 = dk2;
         a.wg = ctx->d_wg;
         a.unit = ctx->d_unit;
-        a.chunk = reinterpret_cast<const int4 *>(ctx->d_chunk);
         a.unit_off = ctx->d_unit_off;
-        a.shell = ctx->d_shell;
+        a.chunk = reinterpret_cast<const int4 *>(ctx->d_chunk);
+        a.row_ao = ctx->d_row_ao;
         a.x = x;
         a.D = D;
         a.partial = partial;
-        a.n = ctx->n;
+        a.jk_part = ctx->d_jk_part;
+        a.n = n;
         a.jscale = jscale;
         a.q0 = ctx->d_qrank[0];
         a.q1 = ctx->d_qrank[1];
         a.q2 = ctx->d_qrank[2];
-        a.chunkq = reinterpret_cast<const float4 *>(ctx->d_chunkq);
         a.dmax = reinterpret_cast<const unsigned *>(ctx->d_dmax);
         a.qtau = qtau;
-        const size_t shm = (3 * sizeof(double) + ctx->dig_rows * sizeof(TV)) * (size_t)ctx->n;
+        const size_t shm = (sizeof(double) + sizeof(TV)) * (size_t)pl.max_rows * n;
         // timed alone: algorithmic bytes per pass are one read of every
         // stored unique integral plus the density in and G out (16 n^2)
         hipEvent_t ev;
         MYQC_CHECK(timer_begin(ctx, T_JK, &ev));
         k_digest<TV><<<ctx->n_wg, 256, shm, ctx->stream>>>(a);
-        MYQC_CHECK(timer_end(ctx, T_JK, ev, (double)sizeof(TV) * (double)ctx->n_stored + 16.0 * ctx->n * ctx->n));
+        MYQC_CHECK(timer_end(ctx, T_JK, ev, (double)sizeof(TV) * (double)ctx->n_stored + 16.0 * n * n));
+        const int ngroups = (int)(pl.jk_group.size() / 3);
+        if (ngroups > 0)
+            k_gather_jk<<<ngroups, 256, 0, ctx->stream>>>(
+                ctx->d_jk_part, ctx->d_jk_list, ctx->d_jk_group, make_int3(pl.jk_base[0], pl.jk_base[1], pl.jk_base[2]),
+                make_int3(pl.jk_block0[0], pl.jk_block0[1], pl.jk_block0[2]),
+                make_int4((int)ctx->pair_ids[0].size(), (int)ctx->pair_ids[1].size(), (int)ctx->pair_ids[2].size(), 0),
+                ctx->d_jk);
     }
-    k_gather_rows<<<dim3((ctx->n + 127) / 128, ctx->n), 128, 0, ctx->stream>>>(partial, ctx->d_shell,
-                                                                                  ctx->d_ao_shell, ctx->n, Graw);
+    k_gather_rows<<<dim3((n + 127) / 128, (unsigned)(pl.row_group.size() / 4)), 128, 0, ctx->stream>>>(
+        partial, reinterpret_cast<const int4 *>(ctx->d_row_group), ctx->d_row_list, ctx->d_jk, ctx->d_jkoff,
+        ctx->d_ao_shell, ctx->d_shell, ctx->ns, n, Graw);
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
 }
@@ -411,8 +513,6 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
     MYQC_CHECK(ensure_work(ctx));
     const int n = ctx->n;
     const size_t nn = (size_t)n * n;
-    if ((size_t)(3 + ctx->dig_rows) * n * sizeof(double) > 64 * 1024)
-        return set_error(ctx, MYQC_E_ARG, "digestion rows for n=%d exceed 64 KB of LDS", n);
     if (precision == MYQC_FP64) {
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, ctx->d_D, nullptr, n, 0.5);
         MYQC_CHECK(launch_digest<double>(ctx, ctx->d_D, ctx->d_x64, ctx->d_Dpair, ctx->d_partial, ctx->d_A));

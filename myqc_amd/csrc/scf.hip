@@ -18,6 +18,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include "context.h"
+#include "many.h"
 
 namespace myqc {
 
@@ -655,6 +656,576 @@ static int uhf_scf(myqc_ctx *ctx, const double *H, const double *S, int na, int 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Many systems of one basis size in lockstep (rhf_scf for each, on the first
+// context's stream): the per-system digestion passes run back to back, and
+// every small step of the loop -- the basis changes, the DIIS error and
+// extrapolation, the level shift, the fused reductions and each purification
+// step -- is one launch for all systems, with one host synchronisation per
+// iteration for all of them.  Each system converges on its own; converged
+// ones drop out of the launches.
+// ---------------------------------------------------------------------------
+
+struct EltMany {
+    const double *X[kMany];
+    const double *Y[kMany];
+    double *Z[kMany];
+    double a[kMany], b[kMany], c[kMany];
+};
+
+// Z_k = a_k X_k + b_k Y_k + c_k I (X or Y may be null); blockIdx.z: system
+__global__ void k_axpbyc_many(EltMany e, int n) {
+    const int k = blockIdx.z, i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    const size_t q = i + (size_t)n * j;
+    double v = (e.X[k] ? e.a[k] * e.X[k][q] : 0.0) + (e.Y[k] ? e.b[k] * e.Y[k][q] : 0.0);
+    if (i == j) v += e.c[k];
+    e.Z[k][q] = v;
+}
+
+// Z_k = X_k - X_k^T
+__global__ void k_antisym_many(EltMany e, int n) {
+    const int k = blockIdx.z, i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    e.Z[k][i + (size_t)n * j] = e.X[k][i + (size_t)n * j] - e.X[k][j + (size_t)n * i];
+}
+
+struct LinMany {
+    const double *hF[kMany];
+    double *Fx[kMany];
+    double c[kMany][ScfBuffers::kDiis];
+    int m[kMany];
+};
+
+// Fx_k = sum_i c_{k,i} hF_k[i]; blockIdx.y: system
+__global__ void k_lincomb_many(LinMany l, size_t nn) {
+    const int k = blockIdx.y;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < l.m[k]; ++i) v += l.c[k][i] * l.hF[k][nn * i + e];
+        l.Fx[k][e] = v;
+    }
+}
+
+struct RedJob {
+    const double *x, *y;
+    int mode, pad;
+};
+
+// the fused reductions of k_reduce_part / k_reduce_final over a job table
+__global__ __launch_bounds__(256) void k_reduce_part_tab(const RedJob *__restrict__ jobs, int n,
+                                                         double *__restrict__ part) {
+    __shared__ double sh[4];
+    const int d = blockIdx.y;
+    const RedJob J = jobs[d];
+    const size_t nn = (size_t)n * n;
+    double acc = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+        if (J.mode == 0)
+            acc += J.x[e] * J.y[e];
+        else if (J.mode == 1)
+            acc = fmax(acc, fabs(J.x[e] - J.y[e]));
+        else
+            acc = fmax(acc, fabs(J.x[e] - ((e % n) == (e / n) ? 1.0 : 0.0)));
+    }
+    acc = wave_fold(acc, J.mode);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = sh[0];
+        for (int w = 1; w < 4; ++w) v = J.mode == 0 ? v + sh[w] : fmax(v, sh[w]);
+        part[(size_t)d * kRedBlocks + blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_reduce_final_tab(const RedJob *__restrict__ jobs,
+                                                         const double *__restrict__ part, double *__restrict__ out) {
+    const int d = blockIdx.x;
+    const double v = wave_fold(part[(size_t)d * kRedBlocks + threadIdx.x], jobs[d].mode);
+    if (threadIdx.x == 0) out[d] = v;
+}
+
+// copies of the systems' orthonormal-basis Fock matrices into one strided
+// buffer (for the batched eigen-decomposition) and back out of it
+__global__ void k_gather_mats(GemmMany g, int n, double *__restrict__ dst) {
+    const int k = blockIdx.z, i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    dst[(size_t)k * n * n + i + (size_t)n * j] = g.A[k][i + (size_t)n * j];
+}
+
+// device buffers of the many-system loop, kept by the first context
+struct ManyBuffers {
+    int cap_jobs = 0, cap_k = 0, n = -1;
+    RedJob *jobs = nullptr;
+    double *part = nullptr, *red = nullptr, *eigA = nullptr, *eigW = nullptr, *eigE = nullptr;
+    int *info = nullptr, *status = nullptr;
+    RedJob *h_jobs = nullptr;   // pinned
+    double *h_red = nullptr;    // pinned
+};
+
+static int many_buffers(myqc_ctx *ctx, int K, int n, int njobs, ManyBuffers *&mb) {
+    if (!ctx->many) ctx->many = new ManyBuffers();
+    mb = ctx->many;
+    if (njobs > mb->cap_jobs) {
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->jobs, sizeof(RedJob) * njobs));
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->part, sizeof(double) * kRedBlocks * njobs));
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->red, sizeof(double) * njobs));
+        if (mb->h_jobs) hipHostFree(mb->h_jobs);
+        if (mb->h_red) hipHostFree(mb->h_red);
+        mb->h_jobs = nullptr;
+        mb->h_red = nullptr;
+        MYQC_HIP(ctx, hipHostMalloc((void **)&mb->h_jobs, sizeof(RedJob) * njobs, hipHostMallocDefault));
+        MYQC_HIP(ctx, hipHostMalloc((void **)&mb->h_red, sizeof(double) * njobs, hipHostMallocDefault));
+        mb->cap_jobs = njobs;
+    }
+    if (K > mb->cap_k || n != mb->n) {
+        const size_t nn = (size_t)n * n;
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->eigA, sizeof(double) * nn * K));
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->eigW, sizeof(double) * (size_t)n * K));
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->eigE, sizeof(double) * (size_t)n * K));
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->info, sizeof(int) * K));
+        MYQC_CHECK(dalloc(ctx, (void **)&mb->status, sizeof(int) * 2 * K));
+        mb->cap_k = K;
+        mb->n = n;
+    }
+    return 0;
+}
+
+void free_many(myqc_ctx *ctx) {
+    if (!ctx->many) return;
+    ManyBuffers *mb = ctx->many;
+    void **ps[] = {(void **)&mb->jobs, (void **)&mb->part, (void **)&mb->red, (void **)&mb->eigA,
+                   (void **)&mb->eigW, (void **)&mb->eigE, (void **)&mb->info, (void **)&mb->status};
+    for (auto p : ps) dfree(p);
+    if (mb->h_jobs) hipHostFree(mb->h_jobs);
+    if (mb->h_red) hipHostFree(mb->h_red);
+    delete mb;
+    ctx->many = nullptr;
+}
+
+// runs a job table (jobs filled in mb->h_jobs[0..m)) and returns the results
+// in mb->h_red (one synchronisation)
+static int reduce_tab(myqc_ctx *ctx, ManyBuffers &mb, int m, int n) {
+    MYQC_HIP(ctx, hipMemcpyAsync(mb.jobs, mb.h_jobs, sizeof(RedJob) * m, hipMemcpyHostToDevice, ctx->stream));
+    k_reduce_part_tab<<<dim3(kRedBlocks, m), 256, 0, ctx->stream>>>(mb.jobs, n, mb.part);
+    k_reduce_final_tab<<<m, 64, 0, ctx->stream>>>(mb.jobs, mb.part, mb.red);
+    MYQC_HIP(ctx, hipGetLastError());
+    MYQC_HIP(ctx, hipMemcpyAsync(mb.h_red, mb.red, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+    MYQC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// C_k = alpha A_k B_k (n x n) over the listed systems
+static int gemm_many(myqc_ctx *lead, const std::vector<int> &sys, int n, const std::vector<const double *> &A,
+                     const std::vector<const double *> &B, const std::vector<double *> &C, double alpha) {
+    GemmMany g{};
+    for (size_t z = 0; z < sys.size(); ++z) {
+        g.A[z] = A[sys[z]];
+        g.B[z] = B[sys[z]];
+        g.C[z] = C[sys[z]];
+    }
+    if (sys.empty()) return 0;
+    return gemm_f64_many(lead, n, n, n, g, (int)sys.size(), alpha);
+}
+
+// per-system state of the loop (the locals of rhf_scf)
+struct ManyState {
+    myqc_ctx *ctx = nullptr;
+    ScfBuffers *b = nullptr;
+    int nocc = 0;
+    double e_nuc = 0.0;
+    bool guessed = false;
+    std::vector<double> Bm;
+    int hist = 0, head = 0;
+    double E = 0.0, E_old = 0.0, dE_prev = 1e300, err_prev = 1e300;
+    bool prev64 = false, final_fp64 = false, conv = false, done = false;
+    int since_full = 0, it = 1;
+    double *fslot = nullptr;
+    const double *fuse = nullptr;
+    bool full = true;
+    int jdd = -1, jb0 = 0, j0 = 0, hn = 0;
+    std::vector<double> cdiis;
+};
+
+static int rhf_scf_many(myqc_ctx *const *ctxs, int K, const double *const *H, const double *const *S, const int *nocc,
+                        const double *e_nuc, const double *const *D_guess, double e_tol, double d_tol, int max_iter,
+                        int precision, double *energy, double *const *C_out, double *const *eig_out,
+                        int *iters_out) {
+    myqc_ctx *lead = ctxs[0];
+    const int n = lead->n;
+    const size_t nn = (size_t)n * n;
+    const int MD = ScfBuffers::kDiis;
+    const bool incr = (precision == MYQC_FP32);
+    ManyBuffers *mbp = nullptr;
+    MYQC_CHECK(many_buffers(lead, K, n, K * (3 + MD) + 8, mbp));
+    ManyBuffers &mb = *mbp;
+    std::vector<ManyState> st(K);
+    // every context's work goes to the lead stream for the call (restored at return)
+    std::vector<hipStream_t> own(K);
+    for (int k = 0; k < K; ++k) {
+        MYQC_HIP(lead, hipStreamSynchronize(ctxs[k]->stream));
+        own[k] = ctxs[k]->stream;
+    }
+    struct Restore {
+        myqc_ctx *const *c;
+        std::vector<hipStream_t> &s;
+        ~Restore() {
+            (void)hipStreamSynchronize(c[0]->stream);
+            for (size_t k = 0; k < s.size(); ++k) {
+                c[k]->stream = s[k];
+                rocblas_set_stream(c[k]->blas, s[k]);
+            }
+        }
+    } restore{ctxs, own};
+    for (int k = 0; k < K; ++k) {
+        ManyState &q = st[k];
+        q.ctx = ctxs[k];
+        q.ctx->stream = lead->stream;
+        MYQC_BLAS(lead, rocblas_set_stream(q.ctx->blas, lead->stream));
+        if (!q.ctx->scf) q.ctx->scf = new ScfBuffers();
+        MYQC_CHECK(q.ctx->scf->ensure(q.ctx, n));
+        q.b = q.ctx->scf;
+        q.nocc = nocc[k];
+        q.e_nuc = e_nuc[k];
+        q.guessed = D_guess && D_guess[k];
+        q.Bm.assign((size_t)MD * MD, 0.0);
+        q.ctx->scf_purify = q.ctx->scf_diag = q.ctx->scf_sp2_steps = 0;
+        MYQC_HIP(lead, hipMemcpyAsync(q.b->H, H[k], sizeof(double) * nn, hipMemcpyHostToDevice, lead->stream));
+        MYQC_HIP(lead, hipMemcpyAsync(q.b->S, S[k], sizeof(double) * nn, hipMemcpyHostToDevice, lead->stream));
+        if (q.guessed)
+            MYQC_HIP(lead, hipMemcpyAsync(q.b->D, D_guess[k], sizeof(double) * nn, hipMemcpyHostToDevice, lead->stream));
+    }
+    std::vector<int> all(K);
+    for (int k = 0; k < K; ++k) all[k] = k;
+    auto ptrs = [&](auto f) {
+        std::vector<decltype(f(st[0]))> v(K);
+        for (int k = 0; k < K; ++k) v[k] = f(st[k]);
+        return v;
+    };
+    const dim3 g2((n + 127) / 128, n);
+    auto elt = [&](const std::vector<int> &sys, auto fill, bool antisym) {
+        EltMany e{};
+        for (size_t z = 0; z < sys.size(); ++z) fill(e, (int)z, st[sys[z]]);
+        if (sys.empty()) return;
+        if (antisym) k_antisym_many<<<dim3(g2.x, g2.y, (unsigned)sys.size()), 128, 0, lead->stream>>>(e, n);
+        else k_axpbyc_many<<<dim3(g2.x, g2.y, (unsigned)sys.size()), 128, 0, lead->stream>>>(e, n);
+    };
+    // X = S^-1/2, Xi = S^1/2 by the coupled Newton-Schulz iteration of
+    // inv_sqrt_ns for all systems at once (the eigen-decomposition path for
+    // any that does not converge in its step budget)
+    {
+        Sp2Many sp{};
+        for (int k = 0; k < K; ++k) {
+            sp.F[k] = st[k].b->S;
+            sp.lohi[k] = st[k].b->lohi;
+            sp.slots[k] = st[k].b->dsc;
+        }
+        MYQC_CHECK(gershgorin_many(lead, sp, K, n));
+        for (int k = 0; k < K; ++k)
+            MYQC_HIP(lead, hipMemcpyAsync(mb.red + k, st[k].b->dsc + 7, sizeof(double), hipMemcpyDeviceToDevice,
+                                          lead->stream));
+        MYQC_HIP(lead, hipMemcpyAsync(mb.h_red, mb.red, sizeof(double) * K, hipMemcpyDeviceToHost, lead->stream));
+        MYQC_HIP(lead, hipStreamSynchronize(lead->stream));
+        std::vector<double> c(mb.h_red, mb.h_red + K);
+        for (int k = 0; k < K; ++k)
+            if (!(c[k] > 0.0)) return set_error(lead, MYQC_E_ARG, "rhf_many: system %d has no positive overlap bound", k);
+        // Y = S / c (in Xi), Z = I (in X); Yn, Zn in T2, P2
+        std::vector<double *> Y(K), Z(K), Yn(K), Zn(K), T1(K);
+        for (int k = 0; k < K; ++k) {
+            Y[k] = st[k].b->Xi;
+            Z[k] = st[k].b->X;
+            Yn[k] = st[k].b->T2;
+            Zn[k] = st[k].b->P2;
+            T1[k] = st[k].b->T1;
+        }
+        elt(all, [&](EltMany &e, int z, ManyState &q) {
+                const int k = (int)(&q - &st[0]);
+                e.X[z] = q.b->S; e.Z[z] = Y[k]; e.a[z] = 1.0 / c[k]; }, false);
+        elt(all, [&](EltMany &e, int z, ManyState &q) {
+                const int k = (int)(&q - &st[0]);
+                e.Z[z] = Z[k]; e.c[z] = 1.0; }, false);
+        auto cv = [](const std::vector<double *> &v) { return std::vector<const double *>(v.begin(), v.end()); };
+        std::vector<int> live = all, extra(K, -1), okk(K, 0);
+        for (int step = 0; step < 40 && !live.empty(); ++step) {
+            MYQC_CHECK(gemm_many(lead, live, n, cv(Z), cv(Y), T1, 1.0));
+            if (step >= 6 && (step & 1) == 0) {
+                int m = 0;
+                std::vector<int> who;
+                for (int k : live)
+                    if (extra[k] < 0) {
+                        mb.h_jobs[m++] = RedJob{T1[k], nullptr, 2, 0};
+                        who.push_back(k);
+                    }
+                if (m) {
+                    MYQC_CHECK(reduce_tab(lead, mb, m, n));
+                    // quadratic convergence: one more step takes 1e-9 below round-off
+                    for (int j = 0; j < m; ++j)
+                        if (mb.h_red[j] < 1e-9) extra[who[j]] = 1;
+                }
+            }
+            std::vector<int> next;
+            for (int k : live) {
+                if (extra[k] == 0) okk[k] = 1;
+                else next.push_back(k);
+            }
+            live.swap(next);
+            if (live.empty()) break;
+            elt(live, [&](EltMany &e, int z, ManyState &q) {
+                    const int k = (int)(&q - &st[0]);
+                    e.X[z] = T1[k]; e.Z[z] = T1[k]; e.a[z] = -0.5; e.c[z] = 1.5; }, false);
+            MYQC_CHECK(gemm_many(lead, live, n, cv(Y), cv(T1), Yn, 1.0));
+            MYQC_CHECK(gemm_many(lead, live, n, cv(T1), cv(Z), Zn, 1.0));
+            for (int k : live) {
+                std::swap(Y[k], Yn[k]);
+                std::swap(Z[k], Zn[k]);
+                if (extra[k] > 0) --extra[k];
+            }
+        }
+        // X = Z / sqrt(c), Xi = Y sqrt(c) (through T1 and Don, as Z and Y may
+        // sit in X and Xi), or the eigen-decomposition path
+        for (int k = 0; k < K; ++k) {
+            ScfBuffers &b = *st[k].b;
+            if (!okk[k]) {
+                MYQC_CHECK(inv_sqrt_eig(st[k].ctx, b, n));
+                continue;
+            }
+            const double rc = sqrt(c[k]);
+            k_axpbyc<<<g2, 128, 0, lead->stream>>>(Z[k], nullptr, b.T1, 1.0 / rc, 0.0, 0.0, n);
+            k_axpbyc<<<g2, 128, 0, lead->stream>>>(Y[k], nullptr, b.Don, rc, 0.0, 0.0, n);
+            MYQC_HIP(lead, hipMemcpyAsync(b.X, b.T1, sizeof(double) * nn, hipMemcpyDeviceToDevice, lead->stream));
+            MYQC_HIP(lead, hipMemcpyAsync(b.Xi, b.Don, sizeof(double) * nn, hipMemcpyDeviceToDevice, lead->stream));
+        }
+    }
+    auto Xs = ptrs([](ManyState &q) { return (const double *)q.b->X; });
+    auto Xis = ptrs([](ManyState &q) { return (const double *)q.b->Xi; });
+    auto T1s = ptrs([](ManyState &q) { return q.b->T1; });
+    auto T2s = ptrs([](ManyState &q) { return q.b->T2; });
+    auto cT1s = ptrs([](ManyState &q) { return (const double *)q.b->T1; });
+    // purification of the listed systems' Fock matrices Fin[k] into Pout[k]
+    int sp2_steps = 0;
+    auto purify_many = [&](const std::vector<int> &sys, const std::vector<const double *> &Fin,
+                           const std::vector<double *> &Pout) -> int {
+        if (sys.empty()) return 0;
+        Sp2Many sp{};
+        for (size_t z = 0; z < sys.size(); ++z) {
+            ScfBuffers &b = *st[sys[z]].b;
+            sp.F[z] = Fin[sys[z]];
+            sp.P[z] = Pout[sys[z]];
+            sp.W[z] = b.P2;
+            sp.lohi[z] = b.lohi;
+            sp.slots[z] = b.dsc;
+            sp.tp[z] = b.sp2t;
+            sp.nocc[z] = st[sys[z]].nocc;
+        }
+        std::vector<int> ok;
+        MYQC_CHECK(sp2_purify_many(lead, sp, (int)sys.size(), n, mb.status, ok, &sp2_steps));
+        for (size_t z = 0; z < sys.size(); ++z) {
+            ManyState &q = st[sys[z]];
+            ++q.ctx->scf_purify;
+            if (!ok[z]) {
+                ++q.ctx->scf_diag;
+                MYQC_CHECK(diag_orth(q.ctx, *q.b, Fin[sys[z]], n));
+                MYQC_CHECK(gemm(q.ctx, NN, TT, n, n, q.nocc, 1.0, q.b->T2, n, q.b->T2, n, 0.0, Pout[sys[z]], n));
+            }
+        }
+        return 0;
+    };
+    // the starting densities
+    {
+        std::vector<int> g, core;
+        for (int k = 0; k < K; ++k) (st[k].guessed ? g : core).push_back(k);
+        // Do = S^1/2 D S^1/2 / 2
+        auto Ds = ptrs([](ManyState &q) { return (const double *)q.b->D; });
+        auto Dos = ptrs([](ManyState &q) { return q.b->Do; });
+        MYQC_CHECK(gemm_many(lead, g, n, Ds, Xis, T1s, 1.0));
+        MYQC_CHECK(gemm_many(lead, g, n, Xis, cT1s, Dos, 0.5));
+        // core Hamiltonian guess: Fo = X H X, purified, D = 2 X Do X
+        auto Hs = ptrs([](ManyState &q) { return (const double *)q.b->H; });
+        auto Fs = ptrs([](ManyState &q) { return q.b->F; });
+        auto cFs = ptrs([](ManyState &q) { return (const double *)q.b->F; });
+        MYQC_CHECK(gemm_many(lead, core, n, Hs, Xs, T1s, 1.0));
+        MYQC_CHECK(gemm_many(lead, core, n, Xs, cT1s, Fs, 1.0));
+        MYQC_CHECK(purify_many(core, cFs, Dos));
+        auto cDos = ptrs([](ManyState &q) { return (const double *)q.b->Do; });
+        auto DsW = ptrs([](ManyState &q) { return q.b->D; });
+        MYQC_CHECK(gemm_many(lead, core, n, cDos, Xs, T1s, 1.0));
+        MYQC_CHECK(gemm_many(lead, core, n, Xs, cT1s, DsW, 2.0));
+    }
+    std::vector<int> active = all;
+    while (!active.empty()) {
+        // Fock builds, one system after another (each pass fills the GPU)
+        for (int k : active) {
+            ManyState &q = st[k];
+            ScfBuffers &b = *q.b;
+            q.full = !incr || q.final_fp64 || q.since_full == 0 || q.since_full >= 8;
+            if (q.full) {
+                if (digest_G(q.ctx, b.D, b.G, MYQC_FP64)) return set_error(lead, MYQC_E_HIP, "%s", q.ctx->err.c_str());
+                q.since_full = 1;
+            } else {
+                k_axpbyc<<<g2, 128, 0, lead->stream>>>(b.D, b.Dref, b.T1, 1.0, -1.0, 0.0, n);
+                if (digest_G(q.ctx, b.T1, b.G, MYQC_FP32)) return set_error(lead, MYQC_E_HIP, "%s", q.ctx->err.c_str());
+                ++q.since_full;
+            }
+            if (incr) MYQC_HIP(lead, hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, lead->stream));
+            q.fslot = b.hF + nn * q.head;
+        }
+        // F = H + G; Fo = X F X into the DIIS slot; error Fo Do - Do Fo
+        elt(active, [&](EltMany &e, int z, ManyState &q) { e.X[z] = q.b->H; e.Y[z] = q.b->G; e.Z[z] = q.b->F; e.a[z] = 1.0; e.b[z] = 1.0; }, false);
+        auto cF = ptrs([](ManyState &q) { return (const double *)q.b->F; });
+        auto fsl = ptrs([](ManyState &q) { return q.fslot; });
+        auto cfsl = ptrs([](ManyState &q) { return (const double *)q.fslot; });
+        auto cDo = ptrs([](ManyState &q) { return (const double *)q.b->Do; });
+        MYQC_CHECK(gemm_many(lead, active, n, cF, Xs, T1s, 1.0));
+        MYQC_CHECK(gemm_many(lead, active, n, Xs, cT1s, fsl, 1.0));
+        MYQC_CHECK(gemm_many(lead, active, n, cfsl, cDo, T2s, 1.0));
+        elt(active, [&](EltMany &e, int z, ManyState &q) { e.X[z] = q.b->T2; e.Z[z] = q.b->hE + nn * q.head; }, true);
+        // one reduction table for all: D.F, D.H, max|D - D_prev|, the DIIS row
+        int m = 0;
+        for (int k : active) {
+            ManyState &q = st[k];
+            ScfBuffers &b = *q.b;
+            q.hn = std::min(q.hist + 1, MD);
+            q.j0 = m;
+            mb.h_jobs[m++] = RedJob{b.D, b.F, 0, 0};
+            mb.h_jobs[m++] = RedJob{b.D, b.H, 0, 0};
+            q.jdd = -1;
+            if (q.it > 1) {
+                q.jdd = m;
+                mb.h_jobs[m++] = RedJob{b.D, b.Dn, 1, 0};
+            }
+            q.jb0 = m;
+            const double *es = b.hE + nn * q.head;
+            for (int j = 0; j < q.hn; ++j) mb.h_jobs[m++] = RedJob{es, b.hE + nn * j, 0, 0};
+        }
+        MYQC_CHECK(reduce_tab(lead, mb, m, n));
+        std::vector<int> go, diis, shift;
+        for (int k : active) {
+            ManyState &q = st[k];
+            const double *r = mb.h_red;
+            q.E = 0.5 * (r[q.j0] + r[q.j0 + 1]) + q.e_nuc;
+            bool stop = false;
+            if (q.jdd >= 0 && q.it > 2 && q.dE_prev < e_tol && r[q.jdd] < d_tol && q.err_prev < 100 * d_tol) {
+                if (!incr || (q.prev64 && q.full)) {
+                    q.conv = true;
+                    stop = true;
+                } else if (!q.final_fp64) {
+                    q.final_fp64 = true;
+                    if (!q.full) {
+                        // confirm with an FP64 build of this same density
+                        q.since_full = 0;
+                        go.push_back(-1 - k);
+                        continue;
+                    }
+                }
+            }
+            if (stop || q.it > max_iter) {
+                q.done = true;
+                continue;
+            }
+            for (int j = 0; j < q.hn; ++j) {
+                q.Bm[q.head * MD + j] = r[q.jb0 + j];
+                q.Bm[j * MD + q.head] = r[q.jb0 + j];
+            }
+            q.hist = q.hn;
+            q.fuse = q.fslot;
+            if (q.hist >= 2) {
+                std::vector<double> Bs((size_t)q.hist * q.hist), cc;
+                for (int i = 0; i < q.hist; ++i)
+                    for (int j = 0; j < q.hist; ++j) Bs[i * q.hist + j] = q.Bm[i * MD + j];
+                if (diis_coeffs(Bs, q.hist, cc)) {
+                    diis.push_back(k);
+                    q.fuse = q.b->Fx;
+                    q.cdiis = cc;
+                }
+            }
+            q.err_prev = sqrt(fabs(q.Bm[q.head * MD + q.head]));
+            q.dE_prev = fabs(q.E - q.E_old);
+            q.E_old = q.E;
+            q.prev64 = q.full;
+            q.head = (q.head + 1) % MD;
+            if (q.guessed && q.err_prev > kShiftErr) shift.push_back(k);
+            go.push_back(k);
+        }
+        // DIIS extrapolation and level shift, all systems at once
+        if (!diis.empty()) {
+            LinMany l{};
+            for (size_t z = 0; z < diis.size(); ++z) {
+                ManyState &q = st[diis[z]];
+                l.hF[z] = q.b->hF;
+                l.Fx[z] = q.b->Fx;
+                l.m[z] = q.hist;
+                for (int i = 0; i < q.hist; ++i) l.c[z][i] = q.cdiis[i];
+            }
+            k_lincomb_many<<<dim3(64, (unsigned)diis.size()), 256, 0, lead->stream>>>(l, nn);
+        }
+        elt(shift, [&](EltMany &e, int z, ManyState &q) {
+                e.X[z] = q.fuse; e.Y[z] = q.b->Do; e.Z[z] = q.b->T1; e.a[z] = 1.0; e.b[z] = -kShift; e.c[z] = kShift; }, false);
+        for (int k : shift) st[k].fuse = st[k].b->T1;
+        std::vector<int> pur, redo;
+        for (int x : go) (x >= 0 ? pur : redo).push_back(x >= 0 ? x : -1 - x);
+        // purify, swap, D = 2 X Do X for the systems that go on
+        {
+            std::vector<const double *> Fin(K);
+            std::vector<double *> Pout(K);
+            for (int k : pur) {
+                Fin[k] = st[k].fuse;
+                Pout[k] = st[k].b->Don;
+            }
+            MYQC_CHECK(purify_many(pur, Fin, Pout));
+            for (int k : pur) {
+                ScfBuffers &b = *st[k].b;
+                std::swap(b.D, b.Dn);
+                std::swap(b.Do, b.Don);
+            }
+            auto cDo2 = ptrs([](ManyState &q) { return (const double *)q.b->Do; });
+            auto Dw = ptrs([](ManyState &q) { return q.b->D; });
+            MYQC_CHECK(gemm_many(lead, pur, n, cDo2, Xs, T1s, 1.0));
+            MYQC_CHECK(gemm_many(lead, pur, n, Xs, cT1s, Dw, 2.0));
+            for (int k : pur) ++st[k].it;
+        }
+        std::vector<int> next;
+        for (int k : active)
+            if (!st[k].done) next.push_back(k);
+        active.swap(next);
+        (void)redo;
+    }
+    // orbitals: eigenvectors of each system's final orthonormal-basis Fock
+    // matrix, one strided batch
+    {
+        GemmMany g{};
+        for (int k = 0; k < K; ++k) g.A[k] = st[k].fslot;
+        k_gather_mats<<<dim3(g2.x, g2.y, K), 128, 0, lead->stream>>>(g, n, mb.eigA);
+        MYQC_CHECK(blas_reserve(lead, [&] {
+            (void)rocsolver_dsyevd_strided_batched(lead->blas, rocblas_evect_original, rocblas_fill_lower, n, mb.eigA, n,
+                                                   (rocblas_stride)nn, mb.eigW, n, mb.eigE, n, mb.info, K);
+        }));
+        MYQC_BLAS(lead, rocsolver_dsyevd_strided_batched(lead->blas, rocblas_evect_original, rocblas_fill_lower, n,
+                                                         mb.eigA, n, (rocblas_stride)nn, mb.eigW, n, mb.eigE, n,
+                                                         mb.info, K));
+        std::vector<const double *> V(K);
+        for (int k = 0; k < K; ++k) V[k] = mb.eigA + nn * k;
+        auto Cs = ptrs([](ManyState &q) { return q.b->C; });
+        MYQC_CHECK(gemm_many(lead, all, n, Xs, V, Cs, 1.0));
+        std::vector<int> info(K);
+        MYQC_HIP(lead, hipMemcpyAsync(info.data(), mb.info, sizeof(int) * K, hipMemcpyDeviceToHost, lead->stream));
+        for (int k = 0; k < K; ++k) {
+            if (C_out && C_out[k]) MYQC_HIP(lead, hipMemcpyAsync(C_out[k], st[k].b->C, sizeof(double) * nn, hipMemcpyDeviceToHost, lead->stream));
+            if (eig_out && eig_out[k]) MYQC_HIP(lead, hipMemcpyAsync(eig_out[k], mb.eigW + (size_t)n * k, sizeof(double) * n, hipMemcpyDeviceToHost, lead->stream));
+        }
+        MYQC_HIP(lead, hipStreamSynchronize(lead->stream));
+        for (int k = 0; k < K; ++k)
+            if (info[k] != 0) return set_error(lead, MYQC_E_HIP, "rocsolver_dsyevd_strided_batched info=%d (system %d)", info[k], k);
+    }
+    int nconv = 0;
+    for (int k = 0; k < K; ++k) {
+        if (energy) energy[k] = st[k].E;
+        if (iters_out) iters_out[k] = st[k].it - 1;
+        nconv += st[k].conv;
+    }
+    if (nconv < K) return set_error(lead, MYQC_E_NOCONV, "rhf_many: %d of %d systems did not converge in %d iterations", K - nconv, K, max_iter);
+    return 0;
+}
+
 }  // namespace myqc
 
 using namespace myqc;
@@ -755,4 +1326,32 @@ extern "C" int myqc_uhf_scf(myqc_ctx *ctx, const double *H, const double *S, int
     ctx->scf_purify = ctx->scf_diag = ctx->scf_sp2_steps = 0;
     return uhf_scf(ctx, H, S, nalpha, nbeta, e_nuc, Ca_guess, Cb_guess, e_tol, d_tol, max_iter, energy, Ca, Cb, eig_a,
                    eig_b, iterations);
+}
+
+extern "C" int myqc_rhf_scf_many(myqc_ctx *const *ctxs, int nsys, const double *const *H, const double *const *S,
+                                 const int *nocc, const double *e_nuc, const double *const *D_guess, double e_tol,
+                                 double d_tol, int max_iter, int precision, double *energy, double *const *C,
+                                 double *const *eig, int *iterations) {
+    if (!ctxs || nsys < 1 || !H || !S || !nocc || !e_nuc) return MYQC_E_ARG;
+    myqc_ctx *lead = ctxs[0];
+    if (!lead) return MYQC_E_ARG;
+    if (nsys > kMany) return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: %d systems (at most %d)", nsys, kMany);
+    if (max_iter < 1 || e_tol <= 0 || d_tol <= 0) return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: invalid tolerances");
+    if (precision != MYQC_FP64 && precision != MYQC_FP32)
+        return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: precision %d", precision);
+    for (int k = 0; k < nsys; ++k) {
+        myqc_ctx *c = ctxs[k];
+        if (!c || !H[k] || !S[k]) return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: system %d: null argument", k);
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == c) return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: context of system %d repeated", k);
+        if (!c->have_system || !c->have_eri || (precision == MYQC_FP32 && !c->have_eri32))
+            return set_error(lead, MYQC_E_STATE, "myqc_rhf_scf_many: system %d: call myqc_eri_build first", k);
+        if (c->n != lead->n || c->device != lead->device)
+            return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: system %d: n=%d on device %d (system 0: n=%d, %d)", k,
+                             c->n, c->device, lead->n, lead->device);
+        if (nocc[k] < 1 || nocc[k] >= c->n)
+            return set_error(lead, MYQC_E_ARG, "myqc_rhf_scf_many: system %d: nocc=%d", k, nocc[k]);
+    }
+    return rhf_scf_many(ctxs, nsys, H, S, nocc, e_nuc, D_guess, e_tol, d_tol, max_iter, precision, energy, C, eig,
+                        iterations);
 }

@@ -14,8 +14,10 @@
 // launches without host synchronisation.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "context.h"
+#include "many.h"
 
 
 namespace myqc {
@@ -122,13 +124,13 @@ __global__ __launch_bounds__(256) void k_sp2_start(const double *__restrict__ F,
 // the projector is final and the remaining launched steps return at once.
 constexpr int KS = 8, KU = 9;
 
-__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
-                                                          int n, double nocc, const double *__restrict__ tin,
-                                                          int nin, double *__restrict__ tout, int step, int first,
-                                                          int *__restrict__ ctrl) {
+__device__ __forceinline__ void sp2_step_tile(const double *__restrict__ P, double *__restrict__ Pn, int n,
+                                              double nocc, const double *__restrict__ tin, int nin,
+                                              double *__restrict__ tout, int step, int first,
+                                              int *__restrict__ ctrl, int tile) {
     __shared__ double s_acc[KS - 1][4][64];
     __shared__ double s_t[KS], s_f[KS];
-    const int tiles = (n + 15) >> 4, tile = blockIdx.x;
+    const int tiles = (n + 15) >> 4;
     const int ti = tile % tiles, tj = tile / tiles;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
     const int ia = ti * 16 + r;     // row of P fed as A
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restr
     const bool crit = step >= first && fabs(tr - fro) < 1e-8 && fabs(tr - nocc) < 1e-6;
     const int cs = (seen >= 0 && seen < step) ? seen : (crit ? step : -1);
     if (cs >= 0 && step >= cs + 2) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && seen < 0 && crit) ctrl[0] = step;
+    if (tile == 0 && threadIdx.x == 0 && seen < 0 && crit) ctrl[0] = step;
     if (w != 0) return;
 #pragma unroll
     for (int q = 0; q < KS - 1; ++q)
@@ -208,6 +210,25 @@ __global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restr
     }
 }
 
+__global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restrict__ P, double *__restrict__ Pn,
+                                                          int n, double nocc, const double *__restrict__ tin,
+                                                          int nin, double *__restrict__ tout, int step, int first,
+                                                          int *__restrict__ ctrl) {
+    sp2_step_tile(P, Pn, n, nocc, tin, nin, tout, step, first, ctrl, blockIdx.x);
+}
+
+// The same step for up to kMany systems of one size at once (blockIdx.y:
+// the system); each system has its own buffers, traces and control word.
+__global__ __launch_bounds__(64 * KS) void k_sp2_step_many(Sp2Many a, int n, int tt, int step, int first) {
+    const int k = blockIdx.y;
+    const double *cur = (step & 1) ? a.W[k] : a.P[k];
+    double *nxt = (step & 1) ? a.P[k] : a.W[k];
+    const double *tin = a.tp[k] + 2 * (size_t)tt * ((step + 1) & 1);
+    double *tout = a.tp[k] + 2 * (size_t)tt * (step & 1);
+    sp2_step_tile(cur, nxt, n, a.nocc[k], tin, tt, tout, step, first, reinterpret_cast<int *>(a.slots[k] + 8),
+                  blockIdx.x);
+}
+
 // After the launched steps: status = {converged, final step}; the final
 // projector (the input of step ctrl[0] + 2) is moved into P when it sits in W.
 __global__ void k_sp2_finish(double *__restrict__ P, const double *__restrict__ W, int n, int *__restrict__ ctrl) {
@@ -220,6 +241,140 @@ __global__ void k_sp2_finish(double *__restrict__ P, const double *__restrict__ 
         ctrl[1] = cs >= 0 ? 1 : 0;
         ctrl[2] = cs >= 0 ? cs + 2 : -1;
     }
+}
+
+// --- the same for up to kMany systems of one size (blockIdx.y: system) ---
+
+__global__ __launch_bounds__(64) void k_gersh_rows_many(Sp2Many a, int n) {
+    const int k = blockIdx.y, i = blockIdx.x, lane = threadIdx.x;
+    const double *F = a.F[k];
+    double r = 0.0;
+    for (int j = lane; j < n; j += 64)
+        if (j != i) r += fabs(F[j + (size_t)n * i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    if (lane == 0) {
+        const double c = F[i + (size_t)n * i];
+        a.lohi[k][2 * i] = c - r;
+        a.lohi[k][2 * i + 1] = c + r;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_gersh_reduce_many(Sp2Many a, int n) {
+    __shared__ double smin[1024], smax[1024];
+    const int k = blockIdx.x;
+    const double *lohi = a.lohi[k];
+    double lo = 1e300, hi = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        lo = fmin(lo, lohi[2 * i]);
+        hi = fmax(hi, lohi[2 * i + 1]);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *slots = a.slots[k];
+        slots[6] = smin[0];
+        slots[7] = smax[0];
+        for (int s = 0; s < 6; ++s) slots[s] = 0.0;
+        reinterpret_cast<int *>(slots + 8)[0] = -1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sp2_start_many(Sp2Many a, int n, int tt) {
+    __shared__ double s_t[4], s_f[4];
+    const int k = blockIdx.y, tiles = (n + 15) >> 4, tile = blockIdx.x;
+    const int i = (tile % tiles) * 16 + (threadIdx.x & 15), j = (tile / tiles) * 16 + (threadIdx.x >> 4);
+    const double lo = a.slots[k][6], hi = a.slots[k][7];
+    const double s = 1.0 / (hi - lo);
+    double t = 0.0, f = 0.0;
+    if (i < n && j < n) {
+        const size_t e = i + (size_t)n * j;
+        const double v = ((i == j ? hi : 0.0) - a.F[k][e]) * s;
+        a.P[k][e] = v;
+        f = v * v;
+        if (i == j) t = v;
+    }
+    t = wsum(t);
+    f = wsum(f);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_t[w] = t;
+        s_f[w] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double *tout = a.tp[k] + 2 * (size_t)tt;   // plays step -1
+        tout[2 * tile] = s_t[0] + s_t[1] + s_t[2] + s_t[3];
+        tout[2 * tile + 1] = s_f[0] + s_f[1] + s_f[2] + s_f[3];
+    }
+}
+
+// status[2k] = converged, status[2k+1] = final step of system k
+__global__ void k_sp2_finish_many(Sp2Many a, int n, int *__restrict__ status) {
+    const int k = blockIdx.y;
+    int *ctrl = reinterpret_cast<int *>(a.slots[k] + 8);
+    const int cs = ctrl[0];
+    const size_t nn = (size_t)n * n;
+    if (cs >= 0 && ((cs + 2) & 1))
+        for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x)
+            a.P[k][e] = a.W[k][e];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl[1] = cs >= 0 ? 1 : 0;
+        ctrl[2] = cs >= 0 ? cs + 2 : -1;
+        status[2 * k] = ctrl[1];
+        status[2 * k + 1] = ctrl[2];
+    }
+}
+
+// Gershgorin bounds of K systems' a.F into their a.slots[6..7] (a.lohi: 2 n each)
+int gershgorin_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n) {
+    if (K < 1 || K > kMany) return set_error(ctx, MYQC_E_ARG, "gershgorin_many: %d systems", K);
+    k_gersh_rows_many<<<dim3(n, K), 64, 0, ctx->stream>>>(a, n);
+    k_gersh_reduce_many<<<K, 1024, 0, ctx->stream>>>(a, n);
+    MYQC_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+// Purification of K systems of order n at once (the projectors into a.P):
+// one launch per step for all of them; status (device, 2 K ints) and host
+// ok[k] say which converged; *steps carries the step count in and out as in
+// sp2_purify.
+int sp2_purify_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n, int *status, std::vector<int> &ok, int *steps) {
+    if (K < 1 || K > kMany) return set_error(ctx, MYQC_E_ARG, "sp2_purify_many: %d systems", K);
+    const int tiles = (n + 15) / 16, tt = tiles * tiles;
+    k_gersh_rows_many<<<dim3(n, K), 64, 0, ctx->stream>>>(a, n);
+    k_gersh_reduce_many<<<K, 1024, 0, ctx->stream>>>(a, n);
+    k_sp2_start_many<<<dim3(tt, K), 256, 0, ctx->stream>>>(a, n, tt);
+    const int first = 8, max_steps = 120;
+    int launched = 0, target = *steps > 0 ? std::min(*steps + 6, max_steps) : 48;
+    std::vector<int> st(2 * K);
+    ok.assign(K, 0);
+    while (true) {
+        for (; launched < target; ++launched)
+            k_sp2_step_many<<<dim3(tt, K), 64 * KS, 0, ctx->stream>>>(a, n, tt, launched, first);
+        k_sp2_finish_many<<<dim3(64, K), 256, 0, ctx->stream>>>(a, n, status);
+        MYQC_HIP(ctx, hipGetLastError());
+        MYQC_HIP(ctx, d2h(ctx, st.data(), status, sizeof(int) * 2 * K));
+        bool all = true;
+        for (int k = 0; k < K; ++k) all = all && st[2 * k];
+        if (all || launched >= max_steps) break;
+        target = std::min(launched + 8, max_steps);
+    }
+    int mx = 0;
+    for (int k = 0; k < K; ++k) {
+        ok[k] = st[2 * k];
+        if (ok[k]) mx = std::max(mx, st[2 * k + 1]);
+    }
+    *steps = mx;
+    return 0;
 }
 
 // Gershgorin bounds of the symmetric A into slots[6..7]; clears slots[0..5]

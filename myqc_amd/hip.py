@@ -58,6 +58,9 @@ EXPORTS = {
     "myqc_rhf_scf_density": ([_vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip], ctypes.c_int),
     "myqc_scf_info": ([_vp, _ip, _ip], ctypes.c_int),
+    "myqc_rhf_scf_many": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_dp), ctypes.POINTER(_dp), _ip, _dp,
+                           ctypes.POINTER(_dp), ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
+                           ctypes.POINTER(_dp), ctypes.POINTER(_dp), _ip], ctypes.c_int),
     "myqc_batch_set": ([_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _ip, _dp, _ip, _ip, _dp], ctypes.c_int),
     "myqc_batch_rhf": ([_vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip], ctypes.c_int),
     "myqc_batch_density": ([_vp, _dp], ctypes.c_int),
@@ -537,3 +540,31 @@ def enuc(Z, xyz) -> float:
         for b in range(a):
             e += Z[a] * Z[b] / np.sqrt(((xyz[a] - xyz[b]) ** 2).sum())
     return e
+
+
+def rhf_many(ctxs, Hs, Ss, noccs, e_nucs, D_guesses=None, e_tol=1e-10, d_tol=1e-8, max_iter=200,
+             precision: int = FP64, want_orbitals: bool = True):
+    """RHF of several systems of one basis size in one lockstep solve
+    (myqc_rhf_scf_many): ctxs[k] holds system k (set_system and eri_build
+    done); D_guesses[k] an AO density or None for the core guess.  Returns a
+    list of dict(E, C, eig, iterations); raises MyqcError when a system did
+    not converge."""
+    K = len(ctxs)
+    L = ctxs[0].L
+    n = ctxs[0].n
+    hs = (_vp * K)(*[c.h for c in ctxs])
+    Hf = [_f(H) for H in Hs]
+    Sf = [_f(S) for S in Ss]
+    Dg = [None if (D_guesses is None or D_guesses[k] is None) else _f(D_guesses[k]) for k in range(K)]
+    Cs = [np.zeros((n, n), order="F") for _ in range(K)] if want_orbitals else None
+    eigs = [np.zeros(n) for _ in range(K)] if want_orbitals else None
+    arr = lambda xs: (_dp * K)(*[_p(x) if x is not None else None for x in xs])
+    E = np.zeros(K)
+    it = np.zeros(K, dtype=np.int32)
+    occ = np.ascontiguousarray(noccs, dtype=np.int32)
+    en = np.ascontiguousarray(e_nucs, dtype=np.float64)
+    rc = L.myqc_rhf_scf_many(hs, K, arr(Hf), arr(Sf), _p(occ, _ip), _p(en), arr(Dg), e_tol, d_tol, max_iter,
+                             precision, _p(E), arr(Cs) if Cs else None, arr(eigs) if eigs else None, _p(it, _ip))
+    ctxs[0]._chk(rc)
+    return [dict(E=float(E[k]), C=Cs[k] if Cs else None, eig=eigs[k] if eigs else None, iterations=int(it[k]))
+            for k in range(K)]

@@ -308,3 +308,198 @@ extern "C" int hc_fock_G_3copy(int nnuc, const double *xyz, int nset, const doub
         for (size_t j = 0; j < n; ++j) G[i + n * j] = A[i + n * j] + A[j + n * i];
     return 0;
 }
+
+// The one-copy digestion of jk.hip over the layout and schedule of
+// digest_plan.h, restated on the host: the stored values (one pre-weighted
+// copy per unique screened quartet, at the plan's offsets), then every
+// workgroup's column chunks and units with the kernel's six updates, the
+// ket-side partial slots, and the two gathers.  Every stored element must be
+// read exactly once (returns -2 otherwise).  G = A + A^T.
+#include "../../myqc_amd/csrc/digest_plan.h"
+
+namespace {
+
+template <int LA, int LB>
+double schwarz_pair(const Sys &s, int p) {
+    QuartetAcc<LA, LB, LA, LB> acc;
+    const ShellPair &sp = s.pairs[p];
+    quartet_prims<LA, LB, LA, LB>(sp, sp, s.prims.data(), s.prims.data(), s.ftT.data(), acc);
+    constexpr int na = ncart(LA), nb = ncart(LB);
+    double out[na * nb * na * nb];
+    quartet_hrr<LA, LB, LA, LB>(acc, sp.AB, sp.AB, out);
+    double m = 0.0;
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) m = std::max(m, fabs(out[((a * nb + b) * na + a) * nb + b]));
+    return sqrt(m);
+}
+
+template <typename Fn>
+void block_of(const Sys &s, int pb, int pk, Fn &&fn) {
+    const int cb = pclass(s.pairs[pb]), ck = pclass(s.pairs[pk]);
+    if (cb == 0 && ck == 0) one<0, 0, 0, 0>(s, pb, pk, fn);
+    else if (cb == 1 && ck == 0) one<1, 0, 0, 0>(s, pb, pk, fn);
+    else if (cb == 1 && ck == 1) one<1, 0, 1, 0>(s, pb, pk, fn);
+    else if (cb == 2 && ck == 0) one<1, 1, 0, 0>(s, pb, pk, fn);
+    else if (cb == 2 && ck == 1) one<1, 1, 1, 0>(s, pb, pk, fn);
+    else if (cb == 2 && ck == 2) one<1, 1, 1, 1>(s, pb, pk, fn);
+}
+
+}  // namespace
+
+extern "C" int hc_fock_G_plan(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                              const double *bas, int norb, const int *basinfo, const double *Ftab, const double *D,
+                              double tol, int max_rows, long long target, double *G, long long *stats) {
+    Sys s;
+    if (!make(s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ftab).empty()) return -1;
+    const int n = norb, np = (int)s.pairs.size(), ns = (int)s.shells.size();
+    std::vector<double> q(np);
+    for (int p = 0; p < np; ++p) {
+        const int c = pclass(s.pairs[p]);
+        q[p] = c == 0 ? schwarz_pair<0, 0>(s, p) : c == 1 ? schwarz_pair<1, 0>(s, p) : schwarz_pair<1, 1>(s, p);
+    }
+    std::vector<int> ids[3];
+    for (int c = 0; c < 3; ++c) {
+        ids[c] = s.ids[c];
+        std::vector<std::pair<double, int>> key;
+        for (int p : ids[c]) key.push_back({-q[p], p});
+        std::sort(key.begin(), key.end());
+        for (size_t r = 0; r < key.size(); ++r) ids[c][r] = key[r].second;
+    }
+    std::vector<int> sao(ns), sL(ns);
+    for (int X = 0; X < ns; ++X) {
+        sao[X] = s.shells[X].ao;
+        sL[X] = s.shells[X].L;
+    }
+    DigestPlan pl;
+    if (!build_digest_plan(pl, n, sao, sL, s.pairs, ids, q, tol, max_rows, target).empty()) return -1;
+    // stored values
+    std::vector<double> x(std::max<long long>(pl.n_values, 1), 0.0);
+    std::vector<int> seen(x.size(), 0);
+    for (int p = 0; p < np; ++p) {
+        const ShellPair &bp = s.pairs[p];
+        const int nb = bp.lb ? 3 : 1;
+        for (int ck = 0; ck < 3; ++ck)
+            for (int r = pl.lo[(size_t)p * 3 + ck]; r < pl.hi[(size_t)p * 3 + ck]; ++r) {
+                const int k = ids[ck][r];
+                const ShellPair &kp = s.pairs[k];
+                const int nc = kp.la ? 3 : 1, nd = kp.lb ? 3 : 1;
+                const long long base = pl.off[(size_t)p * 3 + ck] + r;
+                const long long len = pl.hi[(size_t)p * 3 + ck] - pl.lo[(size_t)p * 3 + ck];
+                block_of(s, p, k, [&](int i, int j, int kk, int l, double v, double w) {
+                    const int a = i - bp.ao_a, b = j - bp.ao_b, c = kk - kp.ao_a, d = l - kp.ao_b;
+                    x[base + (((a * nb + b) * nc + c) * nd + d) * len] = v * w;
+                });
+            }
+    }
+    // the symmetrised density and the workgroups
+    std::vector<double> Ds((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) Ds[i + (size_t)n * j] = 0.5 * (D[i + (size_t)n * j] + D[j + (size_t)n * i]);
+    std::vector<double> partial((size_t)pl.n_rows * n, 0.0), jkp((size_t)std::max(pl.n_jk_slots, 1) * kJkSlot, 0.0);
+    const int nwg = (int)(pl.wg.size() / kWgInts);
+    for (int w = 0; w < nwg; ++w) {
+        const int *wr = &pl.wg[(size_t)w * kWgInts];
+        const int aoA = wr[0] & 0x7fff, LA = wr[0] >> 15, u0 = wr[1], u1 = wr[2], c0 = wr[3], c1 = wr[4];
+        const int nrows = wr[5], row0 = wr[6];
+        const int na = LA ? 3 : 1;
+        if (nrows > pl.max_rows) return -3;
+        std::vector<double> acc((size_t)nrows * n, 0.0), drow((size_t)nrows * n);
+        for (int rr = 0; rr < nrows; ++rr)
+            for (int j = 0; j < n; ++j) drow[(size_t)rr * n + j] = Ds[j + (size_t)n * pl.row_ao[row0 + rr]];
+        for (int c = c0; c < c1; ++c) {
+            const int ck = pl.chunk[(size_t)c * 4], r0 = pl.chunk[(size_t)c * 4 + 1], nk = pl.chunk[(size_t)c * 4 + 2];
+            const int slot = pl.chunk[(size_t)c * 4 + 3], ncd = dig_ncd(ck);
+            if (nk > dig_width(ck)) return -3;
+            for (int u = u0; u < u1; ++u) {
+                const int *ur = &pl.unit[(size_t)u * kUnitInts];
+                const int lo = ur[3 + 2 * ck], hi = ur[4 + 2 * ck];
+                const int aoB = ur[1] & 0x7fff, LB = ur[1] >> 15, sb = ur[2];
+                const int nb = LB ? 3 : 1;
+                const long long off = pl.unit_off[(size_t)u * 3 + ck];
+                for (int kk = 0; kk < nk; ++kk) {
+                    const int r = r0 + kk;
+                    if (r < lo || r >= hi) continue;
+                    const ShellPair &kp = s.pairs[ids[ck][r]];
+                    const int nc = kp.la ? 3 : 1, nd = kp.lb ? 3 : 1, aoC = kp.ao_a, aoD = kp.ao_b;
+                    for (int a = 0; a < na; ++a)
+                        for (int b = 0; b < nb; ++b)
+                            for (int cc = 0; cc < nc; ++cc)
+                                for (int d = 0; d < nd; ++d) {
+                                    const long long e = off + r + (long long)(((a * nb + b) * nc + cc) * nd + d) * (hi - lo);
+                                    if (e < 0 || e >= (long long)x.size()) return -3;
+                                    ++seen[e];
+                                    const double v = x[e];
+                                    auto DA = [&](int row, int col) { return drow[(size_t)row * n + col]; };
+                                    acc[(size_t)a * n + aoB + b] += 2.0 * Ds[(aoC + cc) + (size_t)n * (aoD + d)] * v;
+                                    jkp[(size_t)slot * kJkSlot + (size_t)kk * ncd + cc * nd + d] +=
+                                        2.0 * DA(a, aoB + b) * v;
+                                    acc[(size_t)a * n + aoC + cc] -= 0.5 * DA(sb + b, aoD + d) * v;
+                                    acc[(size_t)a * n + aoD + d] -= 0.5 * DA(sb + b, aoC + cc) * v;
+                                    acc[(size_t)(sb + b) * n + aoC + cc] -= 0.5 * DA(a, aoD + d) * v;
+                                    acc[(size_t)(sb + b) * n + aoD + d] -= 0.5 * DA(a, aoC + cc) * v;
+                                }
+                }
+            }
+        }
+        for (int rr = 0; rr < nrows; ++rr)
+            for (int j = 0; j < n; ++j) partial[(size_t)(row0 + rr) * n + j] = acc[(size_t)rr * n + j];
+        (void)aoA;
+    }
+    // gathers, by the kernels' groups (each group's sum added in)
+    const int np_c[3] = {(int)ids[0].size(), (int)ids[1].size(), (int)ids[2].size()};
+    std::vector<double> jk(std::max(pl.jk_base[3], 1), 0.0);
+    std::vector<int> jk_cover(pl.jk_list.size(), 0);
+    for (size_t g = 0; g < pl.jk_group.size() / 3; ++g) {
+        const int b = pl.jk_group[3 * g], k0 = pl.jk_group[3 * g + 1], k1 = pl.jk_group[3 * g + 2];
+        if (k1 - k0 > kJkGroup || k0 < pl.jk_list_off[b] || k1 > pl.jk_list_off[b + 1]) return -4;
+        const int c = b >= pl.jk_block0[2] ? 2 : b >= pl.jk_block0[1] ? 1 : 0;
+        const int ncd = dig_ncd(c), W = dig_width(c), r0 = (b - pl.jk_block0[c]) * W;
+        for (int pos = 0; pos < W * ncd && r0 + pos / ncd < np_c[c]; ++pos) {
+            double sum = 0.0;
+            for (int k = k0; k < k1; ++k) sum += jkp[(size_t)pl.jk_list[k] * kJkSlot + pos];
+            jk[pl.jk_base[c] + (size_t)r0 * ncd + pos] += sum;
+        }
+        for (int k = k0; k < k1; ++k) ++jk_cover[k];
+    }
+    for (int v : jk_cover)
+        if (v != 1) return -4;
+    std::vector<int> ao_shell(n);
+    for (int X = 0; X < ns; ++X)
+        for (int k = 0; k < (sL[X] ? 3 : 1); ++k) ao_shell[sao[X] + k] = X;
+    std::vector<double> A((size_t)n * n, 0.0);
+    std::vector<int> row_cover(pl.row_list.size(), 0), firsts(n, 0);
+    for (size_t g = 0; g < pl.row_group.size() / 4; ++g) {
+        const int i = pl.row_group[4 * g], k0 = pl.row_group[4 * g + 1], k1 = pl.row_group[4 * g + 2];
+        const int first = pl.row_group[4 * g + 3];
+        if (k1 - k0 > kRowGroup || k0 < pl.row_list_off[i] || k1 > pl.row_list_off[i + 1]) return -4;
+        for (int k = k0; k < k1; ++k) ++row_cover[k];
+        firsts[i] += first;
+        for (int j = 0; j < n; ++j) {
+            double sum = 0.0;
+            for (int k = k0; k < k1; ++k) sum += partial[(size_t)pl.row_list[k] * n + j];
+            if (first) {
+                const int X = ao_shell[i], Y = ao_shell[j];
+                const int o = pl.jkoff[(size_t)X * ns + Y];
+                if (o >= 0) sum += jk[o + (i - sao[X]) * (sL[Y] ? 3 : 1) + (j - sao[Y])];
+            }
+            A[i + (size_t)n * j] += sum;
+        }
+    }
+    for (int v : row_cover)
+        if (v != 1) return -4;
+    for (int v : firsts)
+        if (v != 1) return -4;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) G[i + (size_t)n * j] = A[i + (size_t)n * j] + A[j + (size_t)n * i];
+    if (stats) {
+        stats[0] = pl.n_values;
+        stats[1] = pl.n_quartets;
+        stats[2] = nwg;
+        stats[3] = (long long)(pl.chunk.size() / 4);
+        stats[4] = pl.n_rows;
+        stats[5] = pl.n_jk_slots;
+    }
+    for (size_t e = 0; e < seen.size() && pl.n_values > 0; ++e)
+        if (seen[e] != 1) return -2;
+    return 0;
+}

@@ -161,3 +161,55 @@ def test_boys_grid_point_without_division():
         x = float(tk)
         q = x * 0.1
         assert fma(fma(-q, 10.0, x), 0.1, q) == x / 10.0, tk
+
+
+def _plan_G(hc, s, D, tol, max_rows, target):
+    n = s.n
+    Df = np.asfortranarray(D).ravel(order="F").copy()
+    G = np.zeros(n * n)
+    st = np.zeros(6, dtype=np.int64)
+    fn = hc.hc_fock_G_plan
+    fn.argtypes = None
+    rc = fn(*s.args(), Df.ctypes.data_as(_dp), ctypes.c_double(tol), ctypes.c_int(max_rows),
+            ctypes.c_longlong(target), G.ctypes.data_as(_dp), st.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    return rc, G.reshape(n, n, order="F"), st
+
+
+@pytest.mark.parametrize("name,max_rows,target", [("co2", 64, 1 << 15), ("h2o_4", 6, 200), ("mol1", 7, 500),
+                                                  ("mol3", 11, 1 << 15), ("heh", 6, 1)])
+def test_host_one_copy_digestion_matches_RHFI2G(hc, name, max_rows, target):
+    """The one-copy layout and schedule of digest_plan.h with the digestion
+    kernel's updates, restated on the host, against the reference's RHFI2G
+    (the fixture's G of its first density); small LDS row budgets and
+    workgroup targets force units split over workgroups.  Every stored
+    element is read exactly once (rc == 0)."""
+    f, Z, s = _sys(name)
+    rc, G, st = _plan_G(hc, s, f["D_guess"], 0.0, max_rows, target)
+    print(f"[plan] {name}: values {st[0]} quartets {st[1]} workgroups {st[2]} chunks {st[3]} rows {st[4]}")
+    assert rc == 0
+    assert np.max(np.abs(G - f["G_guess"])) <= 1e-11
+
+
+def test_host_one_copy_digestion_water_cluster(hc):
+    """Eight waters (n = 56): several column chunks per ket class, units
+    split over workgroups; unscreened against the unique-integral G build
+    (itself pinned to RHFI2G above), and with a Schwarz screen every stored
+    element is still read exactly once."""
+    from myqc_amd import molecules as M
+    Z, x = M.water_cluster(8, seed=8)
+    xyz = M.to_bohr(Z, x)
+    s = O.System(Z, xyz, build_basis(Z), FT)
+    n = s.n
+    rng = np.random.default_rng(8)
+    D = rng.uniform(-1.0, 1.0, (n, n))
+    Df = np.asfortranarray(D).ravel(order="F").copy()
+    G0 = np.zeros(n * n)
+    assert hc.hc_fock_G(*s.args(), Df.ctypes.data_as(_dp), G0.ctypes.data_as(_dp)) == 0
+    G0 = G0.reshape(n, n, order="F")
+    rc, G, st = _plan_G(hc, s, D, 0.0, 7, 3000)
+    print(f"[plan] 8 waters: values {st[0]} workgroups {st[2]} chunks {st[3]}")
+    assert rc == 0 and st[3] > st[2]
+    assert np.max(np.abs(G - G0)) <= 1e-11 * max(1.0, np.max(np.abs(G0)))
+    rc, G8, st8 = _plan_G(hc, s, D, 1e-8, 11, 1 << 15)
+    assert rc == 0 and st8[0] < st[0]
+    assert np.max(np.abs(G8 - G0)) <= 1e-5
