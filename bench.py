@@ -136,65 +136,56 @@ def concurrent_leg(waters, k, steps, warmup):
 
 def lockstep_leg(waters, k, steps, warmup, forms):
     """Batches of k seeded clusters of the same size (seeds waters ..
-    waters+k-1, the headline cluster first) solved together: each cluster's
-    set-up (one-electron integrals, the screened two-electron integrals and,
-    in the learned forms, the network's density) on a host thread and HIP
-    stream of its own, then one lockstep SCF of all k (myqc_rhf_scf_many) on
-    one stream, where every step but the digestion passes is one launch for
-    all clusters.  One step is one batch of k full solves from the molecules.
-    Every energy is checked against the same cluster's plain FP64
-    single-system solve, run beforehand outside the timed region."""
-    from concurrent.futures import ThreadPoolExecutor
-
+    waters+k-1, the headline cluster first) solved together by
+    myqc_amd.many.ManySolver: one lockstep SCF of the k clusters
+    (myqc_rhf_scf_many) on one stream, where every step but the digestion
+    passes is one launch for all clusters, while host threads set up the next
+    batch (one-electron integrals, the screened two-electron integrals and,
+    in the learned form, the network's density) on a second set of contexts
+    and streams.  One step is one batch of k full solves from the molecules;
+    the timed steps start with the first batch's set-up and end with the last
+    batch's SCF.  Every energy is checked against the same cluster's plain
+    FP64 single-system solve, run beforehand outside the timed region."""
     from myqc_amd import guess as G
     from myqc_amd import molecules as M
     from myqc_amd.basis import build_basis, read_ftab
-    from myqc_amd.hip import FP32, FP64, Context, enuc, rhf_many
+    from myqc_amd.hip import FP32, FP64, Context, enuc
+    from myqc_amd.many import ManySolver
 
     ftab = read_ftab()
     mols = []
     for s in range(k):
         Z, x = M.water_cluster(waters, seed=waters + s)
-        xyz = M.to_bohr(Z, x)
-        mols.append((Z, xyz, build_basis(Z), sum(Z) // 2, enuc(Z, xyz)))
-    ctxs = [Context(0) for _ in range(k)]
-    have_net = all(G.load(c) for c in ctxs)
-    e64 = [solve(c, *m[:3], ftab, m[3], m[4], FP64)["E"] for c, m in zip(ctxs, mols)]
-    pool = ThreadPoolExecutor(max_workers=min(k, 16))
+        mols.append((Z, M.to_bohr(Z, x)))
+    ref = Context(0)
+    have_net = G.load(ref)
+    e64 = [solve(ref, Z, xyz, build_basis(Z), ftab, sum(Z) // 2, enuc(Z, xyz), FP64)["E"] for Z, xyz in mols]
+    ref.close()
     out = {}
-
-    def setup(i, learned):
-        c, m = ctxs[i], mols[i]
-        c.set_system(m[0], m[1], m[2], ftab)
-        S, H = c.int1e()
-        c.eri_build(1e-12)
-        return H, S, (c.guess_density(m[0], m[1]) if learned else None)
-
-    def batch(prec, learned):
-        hs = list(pool.map(lambda i: setup(i, learned), range(k)))
-        return rhf_many(ctxs, [h[0] for h in hs], [h[1] for h in hs], [m[3] for m in mols], [m[4] for m in mols],
-                        D_guesses=[h[2] for h in hs] if learned else None, e_tol=1e-10, d_tol=1e-8, max_iter=200,
-                        precision=prec, want_orbitals=False)
-
     for name, prec, learned in (("fp64", FP64, False), ("learned_mixed", FP32, True)):
         if name not in forms or (learned and not have_net):
             continue
-        for _ in range(warmup):
-            batch(prec, learned)
+        solver = ManySolver(k, learned=learned, precision=prec, workers=8)
+        # at least two warm-up batches, so that both context sets hold their
+        # buffers before the timed batches
+        for _ in solver.solve([mols] * max(warmup, 2)):
+            pass
         t0 = time.perf_counter()
-        rs = [batch(prec, learned) for _ in range(steps)]
+        rs = list(solver.solve([mols] * steps))
         dt = time.perf_counter() - t0
+        tim = np.array(solver.timings)
+        solver.close()
+        log(f"[bench] lockstep {name}: per batch set-up {1e3 * tim[:, 0].mean():.1f} ms (beside the previous "
+            f"batch's SCF), lockstep SCF {1e3 * tim[:, 1].mean():.1f} ms")
         dmax = max(abs(r[i]["E"] - e64[i]) for r in rs for i in range(k))
         its = float(np.mean([r[i]["iterations"] for r in rs for i in range(k)]))
         log(f"[bench] lockstep {name}: {k} clusters x {steps} batches in {dt * 1e3:.1f} ms "
             f"({k * steps / dt:.1f} solves/s), max |E - E_fp64| = {dmax:.2e}, mean iterations {its:.2f}")
         out[name] = dict(sec=dt / (steps * k), ms_per_batch=dt / steps * 1e3, max_dE_vs_fp64=dmax,
-                         mean_iterations=its, E_first=rs[-1][0]["E"], E_first_fp64=e64[0])
-    pool.shutdown()
-    for c in ctxs:
-        c.close()
+                         mean_iterations=its, E_first=rs[-1][0]["E"], E_first_fp64=e64[0],
+                         setup_ms=1e3 * float(tim[:, 0].mean()), scf_ms=1e3 * float(tim[:, 1].mean()))
     return dict(workload=f"{k} RHF/STO-3G {waters}-water clusters (seeds {waters}-{waters + k - 1}) per batch, "
-                         f"set up side by side, one lockstep SCF", clusters=k, forms=out)
+                         f"one lockstep SCF per batch, the next batch set up meanwhile", clusters=k, forms=out)
 
 
 def cpu_baseline(Z, xyz, basis, ftab, n_iter, budget_s=20.0):
@@ -388,6 +379,12 @@ def main():
     ctx = Context(0)
     from myqc_amd import guess as G
     have_net = G.load(ctx)
+    # the ERI launch shapes, searched once on the headline cluster (untimed;
+    # kept for the process, so every later build uses them)
+    ctx.set_system(Z, xyz, basis, ftab)
+    ctx.eri_build(1e-12)
+    eri_shapes, eri_shape_ms = ctx.eri_tune(reps=2)
+    log(f"[bench] ERI launch shapes (lanes, threads) per class: {eri_shapes}")
     res = {}
     all_forms = (("fp64", FP64, False), ("mixed", FP32, False), ("learned", FP64, True),
                  ("learned_mixed", FP32, True))
@@ -463,7 +460,11 @@ def main():
         "forms": {name: {"solves_per_s": 1.0 / r["sec"], "E": r["E"], "iterations": r["iters"],
                          "dE_vs_fp64": r["E"] - p["E"]} for name, r in res.items()},
         "eri": {"Gint_per_s": gint, "stored_integrals": p["n_stored"], "ms_per_build": eri_avg_ms,
-                "TFLOPs": eri_tflops},
+                "TFLOPs": eri_tflops,
+                "shapes": {["ss|ss", "ps|ss", "ps|ps", "pp|ss", "pp|ps", "pp|pp"][c]: list(v)
+                           for c, v in eri_shapes.items()},
+                "class_ms_tuned": {["ss|ss", "ps|ss", "ps|ps", "pp|ss", "pp|ps", "pp|pp"][c]: v
+                                   for c, v in eri_shape_ms.items()}},
         "jk": {"GB_per_s": jk_gbs, "ms_per_pass": jk_ms / max(jk_n, 1), "passes": jk_n},
         "roofline": {"bound": "hbm", "achieved": jk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": jk_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(args.waters),

@@ -68,6 +68,20 @@ int myqc_eri_full(myqc_ctx *ctx, double schwarz_tol, double *XX);
  * of stored integral slots. */
 int myqc_eri_build(myqc_ctx *ctx, double schwarz_tol, long long *n_stored);
 
+/* Launch-shape search of myqc_eri_build's kernels: for each of the six
+ * quartet classes (ss|ss, ps|ss, ps|ps, pp|ss, pp|ps, pp|pp) present in the
+ * system of the last myqc_eri_build, every candidate of lanes per shell
+ * quartet (1, 2, 4, 8) x threads per workgroup (64, 128, 256) is timed (best
+ * of `reps`), and the fastest is kept for all later builds in the process.
+ * shapes (may be NULL): 12 ints, {lanes, threads} per class; ms (may be
+ * NULL): 6 doubles, the chosen shape's time.  No counterpart in the
+ * reference (whose int2e loop, src/integrals/int2e.f90:192-284, has one
+ * fixed order); for callers that build many systems of one kind. */
+int myqc_eri_tune(myqc_ctx *ctx, int reps, int *shapes, double *ms);
+/* Sets the shapes myqc_eri_tune would choose (12 ints as above; a class
+ * with 0 lanes keeps its shape). */
+int myqc_eri_set_shapes(myqc_ctx *ctx, const int *shapes);
+
 /* RHFI2G drop-in: G(i,j) = sum_kl D(k,l)[(ij|kl) - 1/4 (ik|jl) - 1/4 (il|jk)]
  * (src/I2G/RHFI2G.f90:80-90) from the integrals of myqc_eri_build.
  * precision MYQC_FP64, or MYQC_FP32: the incremental form for a density

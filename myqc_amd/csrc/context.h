@@ -103,7 +103,7 @@ struct myqc_ctx {
     int *d_row_ao = nullptr;              // AO of each partial row
     int *d_row_list = nullptr, *d_row_list_off = nullptr;   // per AO, its partial rows (CSR)
     int *d_jk_list = nullptr, *d_jk_list_off = nullptr;     // per rank block, its chunk slots (CSR)
-    int *d_jk_group = nullptr, *d_row_group = nullptr;      // the gathers' groups
+    int *d_jk_group = nullptr;                              // the ket-side gather's groups
     int *d_jkoff = nullptr;               // [X*ns + Y]: J-ket block of pair (X,Y) if stored so, else -1
     double *d_jk_part = nullptr;          // ket-side J partial slots (kJkSlot doubles each)
     double *d_jk = nullptr;               // ket-side J by class and rank

@@ -57,9 +57,9 @@ constexpr int kUnitInts = 10;
 constexpr int kWgInts = 7;
 // chunk record, ints: class, first rank, kets, ket-side partial slot
 constexpr int kChunkInts = 4;
-// gather groups: at most this many partial slots (rank-block groups) or
-// partial rows (AO groups) are summed by one workgroup of the gathers
-constexpr int kJkGroup = 32, kRowGroup = 16;
+// the ket-side gather's groups: at most this many partial slots of one rank
+// block are summed by one workgroup
+constexpr int kJkGroup = 32;
 
 struct DigestPlan {
     int n = 0, ns = 0, np = 0;
@@ -80,7 +80,6 @@ struct DigestPlan {
     std::vector<int> jk_base;            // first J-ket value of each class (3 + 1 entries)
     std::vector<int> jkoff;              // [X*ns + Y]: J-ket block of pair (X,Y) if stored so, else -1
     std::vector<int> jk_group;           // {rank block, first, end} into jk_list, at most kJkGroup slots
-    std::vector<int> row_group;          // {AO, first, end, first group of the AO} into row_list
 };
 
 inline int pair_nf(const ShellPair &p) { return (p.la ? 3 : 1) * (p.lb ? 3 : 1); }
@@ -319,22 +318,12 @@ inline std::string build_digest_schedule(DigestPlan &pl, const std::vector<int> 
         pl.jk_list_off[b + 1] = pl.jk_list_off[b] + (int)jk_slots[b].size();
         pl.jk_list.insert(pl.jk_list.end(), jk_slots[b].begin(), jk_slots[b].end());
     }
-    // the gathers' work split: runs of at most kJkGroup slots of one rank
-    // block, and of at most kRowGroup partial rows of one AO (every AO has at
-    // least one group, whose first carries the AO's ket-side J)
+    // the ket-side gather's work split: runs of at most kJkGroup slots of
+    // one rank block
     pl.jk_group.clear();
     for (size_t b = 0; b < jk_slots.size(); ++b)
         for (int k = pl.jk_list_off[b]; k < pl.jk_list_off[b + 1]; k += kJkGroup)
             pl.jk_group.insert(pl.jk_group.end(), {(int)b, k, std::min(k + kJkGroup, pl.jk_list_off[b + 1])});
-    pl.row_group.clear();
-    for (int i = 0; i < n; ++i) {
-        int k = pl.row_list_off[i];
-        do {
-            const int e = std::min(k + kRowGroup, pl.row_list_off[i + 1]);
-            pl.row_group.insert(pl.row_group.end(), {i, k, e, k == pl.row_list_off[i] ? 1 : 0});
-            k = e;
-        } while (k < pl.row_list_off[i + 1]);
-    }
     return "";
 }
 

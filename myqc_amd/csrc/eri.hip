@@ -22,6 +22,7 @@
 #include <unordered_map>
 
 #include "context.h"
+#include "wave.h"
 #include "ket_soa.h"
 #include "system.h"
 
@@ -165,12 +166,16 @@ __device__ __forceinline__ void load_ft(const double *__restrict__ ftT, double *
     __syncthreads();
 }
 
-// MODE 0: store the block's copies for digestion;  MODE 1: scatter into dense XX (8-fold)
-template <int CB, int CK, int MODE>
+// MODE 0: store the block's copies for digestion;  MODE 1: scatter into dense XX (8-fold).
+// SPLIT lanes per shell quartet: they share its primitive quartets, add their
+// sums (group_sum), and each stores every SPLIT-th element of the block.
+template <int CB, int CK, int MODE, int SPLIT>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     __shared__ double s_ft[FT_NT * FT_NM];
     load_ft(a.ftT, s_ft);
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = tid / SPLIT;
+    const int sub = (int)(tid % SPLIT);
     if (t >= a.nq) return;
     constexpr int LA = PairL<CB>::a, LB = PairL<CB>::b, LC = PairL<CK>::a, LD = PairL<CK>::b;
     constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(LC), nd = ncart(LD);
@@ -188,7 +193,15 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     constexpr bool kUniformPath = CB + CK <= 2;
     const int pb0 = __builtin_amdgcn_readfirstlane(pb);
     ShellPair bra;
-    if (kUniformPath && __all(pb == pb0)) {
+    if constexpr (SPLIT > 1) {
+        bra = a.pairs[pb];
+        quartet_prims_split<LA, LB, LC, LD, SPLIT>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, rk}, s_ft, acc, sub);
+        using AQ = QuartetAcc<LA, LB, LC, LD>;
+#pragma unroll
+        for (int e = 0; e < AQ::NE; ++e)
+#pragma unroll
+            for (int f = 0; f < AQ::NF; ++f) acc.v[e][f] = group_sum<SPLIT>(acc.v[e][f]);
+    } else if (kUniformPath && __all(pb == pb0)) {
         bra = a.pairs[pb0];
         quartet_prims_f<LA, LB, LC, LD>(bra, ket, a.prims, KetSoa{a.ksoa, a.nket, rk}, s_ft, acc);
     } else {
@@ -214,6 +227,7 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
 #pragma unroll
                     for (int id = 0; id < nd; ++id) {
                         const int e = ((ia * nb + ib) * nc + ic) * nd + id;
+                        if (SPLIT > 1 && e % SPLIT != sub) continue;
                         const double w = block_weight(ia, ib, ic, id, bra.same, ket.same, same_pair, nb, nd,
                                                       bra.ao_a + ia, bra.ao_b + ib, ket.ao_a + ic, ket.ao_b + id);
                         const double v = out[e] * w;
@@ -231,6 +245,7 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
 #pragma unroll
                 for (int id = 0; id < nd; ++id) {
                     const int e = ((ia * nb + ib) * nc + ic) * nd + id;
+                    if (SPLIT > 1 && e % SPLIT != sub) continue;
                     const int i = bra.ao_a + ia, j = bra.ao_b + ib, k = ket.ao_a + ic, l = ket.ao_b + id;
                     const double w = block_weight(ia, ib, ic, id, bra.same, ket.same, same_pair, nb, nd, i, j, k, l);
                     if (w != 0.0) {
@@ -296,7 +311,7 @@ void free_eri(myqc_ctx *ctx, bool keep_memory) {
                      (void **)&ctx->d_chunk,    (void **)&ctx->d_row_ao,      (void **)&ctx->d_row_list,
                      (void **)&ctx->d_row_list_off, (void **)&ctx->d_jk_list, (void **)&ctx->d_jk_list_off,
                      (void **)&ctx->d_jkoff,    (void **)&ctx->d_jk_part,     (void **)&ctx->d_jk,
-                     (void **)&ctx->d_jk_group, (void **)&ctx->d_row_group};
+                     (void **)&ctx->d_jk_group};
     if (!keep_memory) {
         for (auto p : ptrs) dfree(p);
         for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_meta[c]);
@@ -395,8 +410,23 @@ static void class_counts(myqc_ctx *ctx, int cb, int ck, double tol, std::vector<
     }
 }
 
+// Launch shape of each quartet class in the stored build: lanes per shell
+// quartet (1, 2, 4, 8) and threads per workgroup (64, 128, 256).  The
+// defaults hold until myqc_eri_tune measures the candidates on a system and
+// keeps the fastest of each class for the process.
+struct EriShape {
+    int split = 1, tpb = 256;
+};
+static std::mutex g_shape_mu;
+static EriShape g_shape[6];
+
+static EriShape class_shape(int c) {
+    std::lock_guard<std::mutex> g(g_shape_mu);
+    return g_shape[c];
+}
+
 template <int CB, int CK, int MODE>
-static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX, hipStream_t s) {
+static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX, hipStream_t s, EriShape sh = EriShape{}) {
     EriArgs a;
     a.pairs = ctx->d_pairs;
     a.prims = ctx->d_prims;
@@ -414,21 +444,27 @@ static void launch_eri(myqc_ctx *ctx, const QuartetClass &qc, double *XX, hipStr
     a.x32 = ctx->d_x32;
     a.XX = XX;
     a.n = ctx->n;
-    const long long nblk = (qc.nq + 255) / 256;
-    k_eri<CB, CK, MODE><<<(unsigned)nblk, 256, 0, s>>>(a);
+    if (MODE == 1) sh = EriShape{};   // the dense fill (small systems) keeps one lane per quartet
+    const long long nblk = (qc.nq * sh.split + sh.tpb - 1) / sh.tpb;
+    switch (sh.split) {
+        case 2: k_eri<CB, CK, MODE, 2><<<(unsigned)nblk, sh.tpb, 0, s>>>(a); break;
+        case 4: k_eri<CB, CK, MODE, 4><<<(unsigned)nblk, sh.tpb, 0, s>>>(a); break;
+        case 8: k_eri<CB, CK, MODE, 8><<<(unsigned)nblk, sh.tpb, 0, s>>>(a); break;
+        default: k_eri<CB, CK, MODE, 1><<<(unsigned)nblk, sh.tpb, 0, s>>>(a); break;
+    }
 }
 
 template <int MODE>
-static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s) {
+static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s, EriShape sh) {
     const QuartetClass &qc = ctx->cls[c];
     if (qc.nq == 0) return;
     switch (c) {
-        case 0: launch_eri<0, 0, MODE>(ctx, qc, XX, s); break;
-        case 1: launch_eri<1, 0, MODE>(ctx, qc, XX, s); break;
-        case 2: launch_eri<1, 1, MODE>(ctx, qc, XX, s); break;
-        case 3: launch_eri<2, 0, MODE>(ctx, qc, XX, s); break;
-        case 4: launch_eri<2, 1, MODE>(ctx, qc, XX, s); break;
-        case 5: launch_eri<2, 2, MODE>(ctx, qc, XX, s); break;
+        case 0: launch_eri<0, 0, MODE>(ctx, qc, XX, s, sh); break;
+        case 1: launch_eri<1, 0, MODE>(ctx, qc, XX, s, sh); break;
+        case 2: launch_eri<1, 1, MODE>(ctx, qc, XX, s, sh); break;
+        case 3: launch_eri<2, 0, MODE>(ctx, qc, XX, s, sh); break;
+        case 4: launch_eri<2, 1, MODE>(ctx, qc, XX, s, sh); break;
+        case 5: launch_eri<2, 2, MODE>(ctx, qc, XX, s, sh); break;
     }
 }
 
@@ -531,7 +567,6 @@ static int build_schedule(myqc_ctx *ctx) {
     MYQC_CHECK(upload(ctx, &ctx->d_jk_list, pl.jk_list));
     MYQC_CHECK(upload(ctx, &ctx->d_jk_list_off, pl.jk_list_off));
     MYQC_CHECK(upload(ctx, &ctx->d_jk_group, pl.jk_group));
-    MYQC_CHECK(upload(ctx, &ctx->d_row_group, pl.row_group));
     MYQC_CHECK(upload(ctx, &ctx->d_jkoff, pl.jkoff));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_jk_part, sizeof(double) * kJkSlot * std::max<size_t>(pl.n_jk_slots, 1)));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_jk, sizeof(double) * std::max(pl.jk_base[3], 1)));
@@ -601,7 +636,8 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     static const int kOrder[6] = {5, 4, 3, 2, 0, 1};
     for (int c : kOrder) {
         if (ctx->cls[c].nq == 0) continue;
-        launch_class<0>(ctx, c, nullptr, c >= 3 ? ctx->side[0] : c == 2 ? ctx->side[1] : ctx->stream);
+        launch_class<0>(ctx, c, nullptr, c >= 3 ? ctx->side[0] : c == 2 ? ctx->side[1] : ctx->stream,
+                        class_shape(c));
         MYQC_HIP(ctx, hipGetLastError());
     }
     for (int k = 0; k < 2; ++k) {
@@ -621,6 +657,78 @@ int eri_store(myqc_ctx *ctx, double tol, bool want32) {
     return 0;
 }
 
+// The launch-shape search of the stored build: every (lanes per quartet,
+// threads per workgroup) candidate of every class with quartets in the
+// current system is timed on the context stream (the fastest of `reps`
+// launches; each launch rewrites the same values into the store), and the
+// fastest shape of each class is kept for the process.
+static const int kSplits[4] = {1, 2, 4, 8}, kTpbs[3] = {64, 128, 256};
+
+int eri_tune(myqc_ctx *ctx, int reps, int *shape_out, double *ms_out) {
+    if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "myqc_eri_tune: call myqc_eri_build first");
+    if (reps < 1 || reps > 100) return set_error(ctx, MYQC_E_ARG, "myqc_eri_tune: reps=%d (1..100)", reps);
+    hipEvent_t e0, e1;
+    MYQC_HIP(ctx, hipEventCreate(&e0));
+    MYQC_HIP(ctx, hipEventCreate(&e1));
+    int rc = 0;
+    EriShape best[6];
+    double best_ms[6];
+    for (int c = 0; c < 6 && rc == 0; ++c) {
+        best[c] = class_shape(c);
+        best_ms[c] = 0.0;
+        if (ctx->cls[c].nq == 0) continue;
+        best_ms[c] = 1e300;
+        for (int sp : kSplits)
+            for (int tpb : kTpbs) {
+                const EriShape sh{sp, tpb};
+                double t = 1e300;
+                for (int r = 0; r < reps && rc == 0; ++r) {
+                    float ms = 0.0f;
+                    if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "event");
+                    launch_class<0>(ctx, c, nullptr, ctx->stream, sh);
+                    if (!rc && (hipGetLastError() != hipSuccess || hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                        rc = set_error(ctx, MYQC_E_HIP, "myqc_eri_tune: class %d launch failed", c);
+                    t = std::min(t, (double)ms);
+                }
+                if (t < best_ms[c]) {
+                    best_ms[c] = t;
+                    best[c] = sh;
+                }
+            }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> g(g_shape_mu);
+        for (int c = 0; c < 6; ++c)
+            if (ctx->cls[c].nq > 0) g_shape[c] = best[c];
+    }
+    for (int c = 0; c < 6; ++c) {
+        if (shape_out) {
+            shape_out[2 * c] = best[c].split;
+            shape_out[2 * c + 1] = best[c].tpb;
+        }
+        if (ms_out) ms_out[c] = best_ms[c];
+    }
+    return 0;
+}
+
+// sets the launch shapes directly (a shape of 0 lanes keeps that class's)
+int eri_set_shapes(myqc_ctx *ctx, const int *shape) {
+    for (int c = 0; c < 6; ++c) {
+        const int sp = shape[2 * c], tpb = shape[2 * c + 1];
+        if (sp == 0) continue;
+        if ((sp != 1 && sp != 2 && sp != 4 && sp != 8) || (tpb != 64 && tpb != 128 && tpb != 256))
+            return set_error(ctx, MYQC_E_ARG, "myqc_eri_set_shapes: class %d: %d lanes, %d threads", c, sp, tpb);
+    }
+    std::lock_guard<std::mutex> g(g_shape_mu);
+    for (int c = 0; c < 6; ++c)
+        if (shape[2 * c] != 0) g_shape[c] = EriShape{shape[2 * c], shape[2 * c + 1]};
+    return 0;
+}
+
 int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX) {
     MYQC_CHECK(eri_schwarz(ctx));
     const size_t n = ctx->n;
@@ -637,7 +745,7 @@ int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX) {
         hipEvent_t ev;
         rc = timer_begin(ctx, T_ERI, &ev);
         for (int c = 0; c < 6 && rc == 0; ++c) {
-            launch_class<1>(ctx, c, d_XX, ctx->stream);
+            launch_class<1>(ctx, c, d_XX, ctx->stream, EriShape{});
             if (hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "ERI dense-fill launch failed");
         }
         if (rc == 0) rc = timer_end(ctx, T_ERI, ev, ctx->eri_flops);
@@ -720,4 +828,14 @@ extern "C" int myqc_eri_build(myqc_ctx *ctx, double schwarz_tol, long long *n_st
     MYQC_CHECK(eri_store(ctx, schwarz_tol, true));
     if (n_stored) *n_stored = ctx->n_stored;
     return 0;
+}
+
+extern "C" int myqc_eri_tune(myqc_ctx *ctx, int reps, int *shapes, double *ms) {
+    if (!ctx) return MYQC_E_ARG;
+    return eri_tune(ctx, reps, shapes, ms);
+}
+
+extern "C" int myqc_eri_set_shapes(myqc_ctx *ctx, const int *shapes) {
+    if (!ctx || !shapes) return MYQC_E_ARG;
+    return eri_set_shapes(ctx, shapes);
 }

@@ -196,6 +196,94 @@ struct QuartetAcc {
     double v[NE][NF];
 };
 
+// One primitive quartet (b k) of a shell quartet added into acc: the Boys
+// values, the bra vertical recursion [e0|00]^(m), the ket vertical recursion
+// [e0|f0]^(m), and the targets.  Q: the Boys order rule of the shell
+// quartet; c0 = 2 pi^2.5.
+template <int LA, int LB, int LC, int LD>
+MYQC_HD __forceinline__ void prim_quartet(const PrimPair &b, const PrimPair &k, int Q, double c0,
+                                          const double *__restrict__ ftT, QuartetAcc<LA, LB, LC, LD> &acc) {
+    using A = QuartetAcc<LA, LB, LC, LD>;
+    constexpr int EB = A::EB, EK = A::EK, M = A::M;
+    constexpr int NB = nupto(EB);
+    const double p = b.p, q = k.p, pq = p + q;
+    const double ipq = recip_hd(pq);
+    const double rho = p * q * ipq;
+    const double rsq = rsqrt_hd(pq);
+    double PQ[3], WP[3], WQ[3], QC[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        PQ[d] = b.P[d] - k.P[d];
+        // W = (pP + qQ)/(p+q):  W - P = -q/(p+q) PQ,  W - Q = p/(p+q) PQ
+        WP[d] = -q * ipq * PQ[d];
+        WQ[d] = p * ipq * PQ[d];
+        QC[d] = k.PA[d];
+    }
+    const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+    const double pref = c0 * b.ip * k.ip * rsq * b.K * k.K;
+    double F[M + 1];
+    boys_ref<M>(Q, T, ftT, F);
+    // bra vertical recursion [e0|00]^(m)
+    double Vb[NB][M + 1];
+#pragma unroll
+    for (int m = 0; m <= M; ++m) Vb[0][m] = pref * F[m];
+    if constexpr (EB > 0) {
+        const double i2p = 0.5 * b.ip, rp = q * ipq;
+#pragma unroll
+        for (int idx = 1; idx < NB; ++idx) {
+            const int d = bdir(idx), lo = lower(idx, d);
+            const int am1 = cexp(lo, d);
+#pragma unroll
+            for (int m = 0; m <= M - lvl(idx); ++m) {
+                double v = b.PA[d] * Vb[lo][m] + WP[d] * Vb[lo][m + 1];
+                if (am1 > 0) {
+                    const int lo2 = lower(lo, d);
+                    v += am1 * i2p * (Vb[lo2][m] - rp * Vb[lo2][m + 1]);
+                }
+                Vb[idx][m] = v;
+            }
+        }
+    }
+    // ket vertical recursion [e0|f0]^(m), then accumulate the targets
+    if constexpr (EK == 0) {
+#pragma unroll
+        for (int e = A::E0; e < NB; ++e) acc.v[e - A::E0][0] += Vb[e][0];
+    } else {
+        constexpr int NK = nupto(EK);
+        const double i2q = 0.5 * k.ip, rq = p * ipq, i2pq = 0.5 * ipq;
+        double W[NB][NK][M + 1];
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+#pragma unroll
+            for (int m = 0; m <= M - lvl(e); ++m) W[e][0][m] = Vb[e][m];
+#pragma unroll
+        for (int f = 1; f < NK; ++f) {
+            const int d = bdir(f), lo = lower(f, d);
+            const int cm1 = cexp(lo, d);
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+#pragma unroll
+                for (int m = 0; m <= M - lvl(e) - lvl(f); ++m) {
+                    double v = QC[d] * W[e][lo][m] + WQ[d] * W[e][lo][m + 1];
+                    if (cm1 > 0) {
+                        const int lo2 = lower(lo, d);
+                        v += cm1 * i2q * (W[e][lo2][m] - rq * W[e][lo2][m + 1]);
+                    }
+                    if (cexp(e, d) > 0) {
+                        const int el = lower(e, d);
+                        v += cexp(e, d) * i2pq * W[el][lo][m + 1];
+                    }
+                    W[e][f][m] = v;
+                }
+            }
+        }
+#pragma unroll
+        for (int e = A::E0; e < NB; ++e)
+#pragma unroll
+            for (int f = A::F0; f < NK; ++f) acc.v[e - A::E0][f - A::F0] += W[e][f][0];
+    }
+}
+
 // SCREEN = false skips the reference's primitive-pair screen: used for the
 // Schwarz factors, since a primitive pair dropped from (ab|ab) by
 // EIJ^2 < 1e-14 can survive EIJ*EGH >= 1e-14 against a compact ket pair.
@@ -204,8 +292,6 @@ template <int LA, int LB, int LC, int LD, bool SCREEN = true, typename KF>
 MYQC_HD void quartet_prims_f(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
                              KF &&kprim, const double *__restrict__ ftT, QuartetAcc<LA, LB, LC, LD> &acc) {
     using A = QuartetAcc<LA, LB, LC, LD>;
-    constexpr int EB = A::EB, EK = A::EK, M = A::M;
-    constexpr int NB = nupto(EB);
     const int Q = 3 * (bra.bl + ket.bl);
 #pragma unroll
     for (int e = 0; e < A::NE; ++e)
@@ -220,82 +306,39 @@ MYQC_HD void quartet_prims_f(const ShellPair &bra, const ShellPair &ket, const P
         for (int i = 0; i < bra.npp; ++i) {
             const PrimPair b = ppb[bra.pp0 + i];
             if (SCREEN && b.EIJ * k.EIJ < 1.0e-14) continue;          // int2e.f90:257
-            const double p = b.p, q = k.p, pq = p + q;
-            const double ipq = recip_hd(pq);
-            const double rho = p * q * ipq;
-            const double rsq = rsqrt_hd(pq);
-            double PQ[3], WP[3], WQ[3], QC[3];
+            prim_quartet<LA, LB, LC, LD>(b, k, Q, c0, ftT, acc);
+        }
+    }
+}
+
+// The same sum split over SPLIT lanes: lane `sub` of the quartet's group
+// takes the primitive quartets t = j npp_bra + i with t = sub (mod SPLIT);
+// the group's partial sums are added afterwards (split_reduce on the device).
+template <int LA, int LB, int LC, int LD, int SPLIT, typename KF>
+MYQC_HD void quartet_prims_split(const ShellPair &bra, const ShellPair &ket, const PrimPair *__restrict__ ppb,
+                                 KF &&kprim, const double *__restrict__ ftT, QuartetAcc<LA, LB, LC, LD> &acc,
+                                 int sub) {
+    using A = QuartetAcc<LA, LB, LC, LD>;
+    const int Q = 3 * (bra.bl + ket.bl);
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                PQ[d] = b.P[d] - k.P[d];
-                // W = (pP + qQ)/(p+q):  W - P = -q/(p+q) PQ,  W - Q = p/(p+q) PQ
-                WP[d] = -q * ipq * PQ[d];
-                WQ[d] = p * ipq * PQ[d];
-                QC[d] = k.PA[d];
-            }
-            const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-            const double pref = c0 * b.ip * k.ip * rsq * b.K * k.K;
-            double F[M + 1];
-            boys_ref<M>(Q, T, ftT, F);
-            // bra vertical recursion [e0|00]^(m)
-            double Vb[NB][M + 1];
+    for (int e = 0; e < A::NE; ++e)
 #pragma unroll
-            for (int m = 0; m <= M; ++m) Vb[0][m] = pref * F[m];
-            if constexpr (EB > 0) {
-                const double i2p = 0.5 * b.ip, rp = q * ipq;
-#pragma unroll
-                for (int idx = 1; idx < NB; ++idx) {
-                    const int d = bdir(idx), lo = lower(idx, d);
-                    const int am1 = cexp(lo, d);
-#pragma unroll
-                    for (int m = 0; m <= M - lvl(idx); ++m) {
-                        double v = b.PA[d] * Vb[lo][m] + WP[d] * Vb[lo][m + 1];
-                        if (am1 > 0) {
-                            const int lo2 = lower(lo, d);
-                            v += am1 * i2p * (Vb[lo2][m] - rp * Vb[lo2][m + 1]);
-                        }
-                        Vb[idx][m] = v;
-                    }
-                }
-            }
-            // ket vertical recursion [e0|f0]^(m), then accumulate the targets
-            if constexpr (EK == 0) {
-#pragma unroll
-                for (int e = A::E0; e < NB; ++e) acc.v[e - A::E0][0] += Vb[e][0];
-            } else {
-                constexpr int NK = nupto(EK);
-                const double i2q = 0.5 * k.ip, rq = p * ipq, i2pq = 0.5 * ipq;
-                double W[NB][NK][M + 1];
-#pragma unroll
-                for (int e = 0; e < NB; ++e)
-#pragma unroll
-                    for (int m = 0; m <= M - lvl(e); ++m) W[e][0][m] = Vb[e][m];
-#pragma unroll
-                for (int f = 1; f < NK; ++f) {
-                    const int d = bdir(f), lo = lower(f, d);
-                    const int cm1 = cexp(lo, d);
-#pragma unroll
-                    for (int e = 0; e < NB; ++e) {
-#pragma unroll
-                        for (int m = 0; m <= M - lvl(e) - lvl(f); ++m) {
-                            double v = QC[d] * W[e][lo][m] + WQ[d] * W[e][lo][m + 1];
-                            if (cm1 > 0) {
-                                const int lo2 = lower(lo, d);
-                                v += cm1 * i2q * (W[e][lo2][m] - rq * W[e][lo2][m + 1]);
-                            }
-                            if (cexp(e, d) > 0) {
-                                const int el = lower(e, d);
-                                v += cexp(e, d) * i2pq * W[el][lo][m + 1];
-                            }
-                            W[e][f][m] = v;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int e = A::E0; e < NB; ++e)
-#pragma unroll
-                    for (int f = A::F0; f < NK; ++f) acc.v[e - A::E0][f - A::F0] += W[e][f][0];
-            }
+        for (int f = 0; f < A::NF; ++f) acc.v[e][f] = 0.0;
+    const double c0 = 2.0 * pow(PI_SP, 2.5);
+    const int nbp = bra.npp, total = nbp * ket.npp;
+    int j = 0, i = sub;
+    while (i >= nbp && nbp > 0) {
+        i -= nbp;
+        ++j;
+    }
+    for (int t = sub; t < total; t += SPLIT) {
+        const PrimPair k = kprim(j);
+        const PrimPair b = ppb[bra.pp0 + i];
+        if (!(b.EIJ * k.EIJ < 1.0e-14)) prim_quartet<LA, LB, LC, LD>(b, k, Q, c0, ftT, acc);   // int2e.f90:257
+        i += SPLIT;
+        while (i >= nbp) {
+            i -= nbp;
+            ++j;
         }
     }
 }

@@ -26,6 +26,7 @@
 
 #include "context.h"
 #include "digest_plan.h"
+#include "wave.h"
 
 namespace myqc {
 
@@ -66,41 +67,6 @@ using cptr = const __attribute__((address_space(4))) T *;
 template <typename T>
 __device__ __forceinline__ cptr<T> as_const(const T *p) {
     return (cptr<T>)p;
-}
-
-// Sum over the 64 lanes of a wavefront (all active), returned wave-uniform:
-// within each row of 16 lanes by four DPP exchanges (lanes xor 1, xor 2,
-// the mirrored half-row, the mirrored row), then the four row sums read from
-// lanes 0, 16, 32, 48.  No LDS traffic.
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_add(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
-    return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int CTRL>
-__device__ __forceinline__ float dpp_add(float v) {
-    return v + __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
-}
-__device__ __forceinline__ double lane_value(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ float lane_value(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-template <typename T>
-__device__ __forceinline__ T wave_total(T v) {
-    v = dpp_add<0xB1>(v);    // quad_perm [1,0,3,2]
-    v = dpp_add<0x4E>(v);    // quad_perm [2,3,0,1]
-    v = dpp_add<0x141>(v);   // row_half_mirror
-    v = dpp_add<0x140>(v);   // row_mirror
-    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 // Bra row a = ia of one unit (A, B) over the lanes' kets of one column chunk.
@@ -336,31 +302,37 @@ __global__ __launch_bounds__(256) void k_gather_jk(const double *__restrict__ jk
     }
 }
 
-// A(i, j) += the partial rows of one group holding AO i (A zeroed before the
-// pass), and for the AO's first group the ket-side J of the pair (shell(i),
-// shell(j)) when it is stored in that orientation
-__global__ __launch_bounds__(128) void k_gather_rows(const double *__restrict__ partial, const int4 *__restrict__ group,
-                                                     const int *__restrict__ list, const double *__restrict__ jk,
-                                                     const int *__restrict__ jkoff, const int *__restrict__ ao_shell,
-                                                     const int *__restrict__ shell, int ns, int n,
-                                                     double *__restrict__ Graw) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int4 gr = group[blockIdx.y];
-    const int i = gr.x, k0 = gr.y, k1 = gr.z;
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-    int k = k0;
-    for (; k + 4 <= k1; k += 4)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s4[q] += partial[(size_t)list[k + q] * n + j];
-    for (; k < k1; ++k) s4[0] += partial[(size_t)list[k] * n + j];
-    double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-    if (gr.w) {
-        const int X = ao_shell[i], Y = ao_shell[j];
-        const int o = jkoff[X * ns + Y];
-        if (o >= 0) s += jk[o + (i - shell[4 * X]) * (shell[4 * Y + 1] ? 3 : 1) + (j - shell[4 * Y])];
+// A(i, j) = the sum of the partial rows holding AO i, plus the ket-side J of
+// the pair (shell(i), shell(j)) when it is stored in that orientation.  One
+// workgroup per (AO, 64 columns); its four thread rows take every fourth
+// partial row (two independent sums each) and meet in LDS in a fixed order.
+__global__ __launch_bounds__(256) void k_gather_rows(const double *__restrict__ partial,
+                                                     const int *__restrict__ list_off, const int *__restrict__ list,
+                                                     const double *__restrict__ jk, const int *__restrict__ jkoff,
+                                                     const int *__restrict__ ao_shell, const int *__restrict__ shell,
+                                                     int ns, int n, double *__restrict__ Graw) {
+    __shared__ double s_sum[4][64];
+    const int j = blockIdx.x * 64 + threadIdx.x, i = blockIdx.y, q = threadIdx.y;
+    double s = 0.0;
+    if (j < n) {
+        const int k0 = list_off[i], k1 = list_off[i + 1];
+        double s2[2] = {0.0, 0.0};
+        int k = k0 + q;
+        for (; k + 4 < k1; k += 8) {
+            s2[0] += partial[(size_t)list[k] * n + j];
+            s2[1] += partial[(size_t)list[k + 4] * n + j];
+        }
+        if (k < k1) s2[0] += partial[(size_t)list[k] * n + j];
+        s = s2[0] + s2[1];
     }
-    atomicAdd(&Graw[i + (size_t)n * j], s);
+    s_sum[q][threadIdx.x] = s;
+    __syncthreads();
+    if (q != 0 || j >= n) return;
+    s = (s_sum[0][threadIdx.x] + s_sum[1][threadIdx.x]) + (s_sum[2][threadIdx.x] + s_sum[3][threadIdx.x]);
+    const int X = ao_shell[i], Y = ao_shell[j];
+    const int o = jkoff[X * ns + Y];
+    if (o >= 0) s += jk[o + (i - shell[4 * X]) * (shell[4 * Y + 1] ? 3 : 1) + (j - shell[4 * Y])];
+    Graw[i + (size_t)n * j] = s;
 }
 
 __global__ void k_zero(double *__restrict__ p, size_t cnt) {
@@ -450,8 +422,7 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
         if (nps[c] > 0)
             k_pack_kets<TV><<<(nps[c] + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_sorted[c], nps[c],
                                                                             c ? 3 : 1, c == 2 ? 3 : 1, D, n, dks[c]);
-    // the gathers add into A and the ket-side J
-    k_zero<<<128, 256, 0, ctx->stream>>>(Graw, (size_t)n * n);
+    // the ket-side J gather adds into jk
     k_zero<<<64, 256, 0, ctx->stream>>>(ctx->d_jk, (size_t)std::max(pl.jk_base[3], 1));
     if (ctx->n_wg > 0) {
         DigArgs<TV> a;
@@ -492,9 +463,9 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
                 make_int4((int)ctx->pair_ids[0].size(), (int)ctx->pair_ids[1].size(), (int)ctx->pair_ids[2].size(), 0),
                 ctx->d_jk);
     }
-    k_gather_rows<<<dim3((n + 127) / 128, (unsigned)(pl.row_group.size() / 4)), 128, 0, ctx->stream>>>(
-        partial, reinterpret_cast<const int4 *>(ctx->d_row_group), ctx->d_row_list, ctx->d_jk, ctx->d_jkoff,
-        ctx->d_ao_shell, ctx->d_shell, ctx->ns, n, Graw);
+    k_gather_rows<<<dim3((n + 63) / 64, n), dim3(64, 4), 0, ctx->stream>>>(
+        partial, ctx->d_row_list_off, ctx->d_row_list, ctx->d_jk, ctx->d_jkoff, ctx->d_ao_shell, ctx->d_shell,
+        ctx->ns, n, Graw);
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
 }

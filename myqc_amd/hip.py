@@ -33,6 +33,8 @@ EXPORTS = {
                          ctypes.c_int], ctypes.c_int),
     "myqc_eri_full": ([_vp, ctypes.c_double, _dp], ctypes.c_int),
     "myqc_eri_build": ([_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "myqc_eri_tune": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
+    "myqc_eri_set_shapes": ([_vp, _ip], ctypes.c_int),
     "myqc_fock_G": ([_vp, _dp, _dp, ctypes.c_int], ctypes.c_int),
     "myqc_fock_G_uhf": ([_vp, _dp, _dp, _dp, _dp], ctypes.c_int),
     "myqc_density_rhf": ([_vp, _dp, ctypes.c_int, _dp], ctypes.c_int),
@@ -179,6 +181,22 @@ class Context:
         ns = ctypes.c_longlong()
         self._chk(self.L.myqc_eri_build(self.h, schwarz_tol, ctypes.byref(ns)))
         return ns.value
+
+    def eri_tune(self, reps: int = 2):
+        """Times the ERI launch shapes on the current system and keeps the
+        fastest per class for the process; returns ({class: (lanes,
+        threads)}, {class: ms})."""
+        sh = np.zeros(12, dtype=np.int32)
+        ms = np.zeros(6)
+        self._chk(self.L.myqc_eri_tune(self.h, reps, _p(sh, _ip), _p(ms)))
+        return {c: (int(sh[2 * c]), int(sh[2 * c + 1])) for c in range(6)}, {c: float(ms[c]) for c in range(6)}
+
+    def eri_set_shapes(self, shapes):
+        """shapes: {class: (lanes, threads)} (classes left out keep theirs)."""
+        sh = np.zeros(12, dtype=np.int32)
+        for c, (sp, tpb) in shapes.items():
+            sh[2 * c], sh[2 * c + 1] = sp, tpb
+        self._chk(self.L.myqc_eri_set_shapes(self.h, _p(sh, _ip)))
 
     def fock_G(self, D, precision: int = FP64) -> np.ndarray:
         """RHFI2G: G from the stored integrals."""

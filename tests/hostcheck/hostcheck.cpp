@@ -466,29 +466,16 @@ extern "C" int hc_fock_G_plan(int nnuc, const double *xyz, int nset, const doubl
     std::vector<int> ao_shell(n);
     for (int X = 0; X < ns; ++X)
         for (int k = 0; k < (sL[X] ? 3 : 1); ++k) ao_shell[sao[X] + k] = X;
-    std::vector<double> A((size_t)n * n, 0.0);
-    std::vector<int> row_cover(pl.row_list.size(), 0), firsts(n, 0);
-    for (size_t g = 0; g < pl.row_group.size() / 4; ++g) {
-        const int i = pl.row_group[4 * g], k0 = pl.row_group[4 * g + 1], k1 = pl.row_group[4 * g + 2];
-        const int first = pl.row_group[4 * g + 3];
-        if (k1 - k0 > kRowGroup || k0 < pl.row_list_off[i] || k1 > pl.row_list_off[i + 1]) return -4;
-        for (int k = k0; k < k1; ++k) ++row_cover[k];
-        firsts[i] += first;
+    std::vector<double> A((size_t)n * n);
+    for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             double sum = 0.0;
-            for (int k = k0; k < k1; ++k) sum += partial[(size_t)pl.row_list[k] * n + j];
-            if (first) {
-                const int X = ao_shell[i], Y = ao_shell[j];
-                const int o = pl.jkoff[(size_t)X * ns + Y];
-                if (o >= 0) sum += jk[o + (i - sao[X]) * (sL[Y] ? 3 : 1) + (j - sao[Y])];
-            }
-            A[i + (size_t)n * j] += sum;
+            for (int k = pl.row_list_off[i]; k < pl.row_list_off[i + 1]; ++k) sum += partial[(size_t)pl.row_list[k] * n + j];
+            const int X = ao_shell[i], Y = ao_shell[j];
+            const int o = pl.jkoff[(size_t)X * ns + Y];
+            if (o >= 0) sum += jk[o + (i - sao[X]) * (sL[Y] ? 3 : 1) + (j - sao[Y])];
+            A[i + (size_t)n * j] = sum;
         }
-    }
-    for (int v : row_cover)
-        if (v != 1) return -4;
-    for (int v : firsts)
-        if (v != 1) return -4;
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) G[i + (size_t)n * j] = A[i + (size_t)n * j] + A[j + (size_t)n * i];
     if (stats) {

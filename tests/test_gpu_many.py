@@ -113,3 +113,31 @@ def test_many_refuses_mixed_sizes_and_repeated_contexts(ctx):
     finally:
         a.close()
         b.close()
+
+
+def test_pipelined_solver_matches_single_solves(ctx):
+    """myqc_amd.many.ManySolver: three batches (of 3, 3 and 2 molecules) on
+    two alternating context sets, each batch set up while the previous one
+    solves; every energy equals the single-system solve's."""
+    from myqc_amd.many import ManySolver
+    seeds = [[11, 12, 13], [14, 15, 16], [17, 18]]
+    batches = []
+    for bs in seeds:
+        b = []
+        for s in bs:
+            Z, x = M.water_cluster(6, seed=s)
+            b.append((Z, M.to_bohr(Z, x)))
+        batches.append(b)
+    solver = ManySolver(3, precision=FP32, workers=3)
+    try:
+        results = list(solver.solve(batches))
+    finally:
+        solver.close()
+    assert [len(r) for r in results] == [3, 3, 2]
+    for b, res in zip(batches, results):
+        for (Z, xyz), r in zip(b, res):
+            ctx.set_system(Z, xyz)
+            S, H = ctx.int1e()
+            ctx.eri_build(1e-12)
+            r1 = ctx.rhf(H, S, sum(Z) // 2, enuc(Z, xyz), precision=FP64)
+            assert abs(r["E"] - r1["E"]) <= 1e-8
