@@ -137,15 +137,14 @@ def concurrent_leg(waters, k, steps, warmup):
 def lockstep_leg(waters, k, steps, warmup, forms):
     """Batches of k seeded clusters of the same size (seeds waters ..
     waters+k-1, the headline cluster first) solved together by
-    myqc_amd.many.ManySolver: one lockstep SCF of the k clusters
-    (myqc_rhf_scf_many) on one stream, where every step but the digestion
-    passes is one launch for all clusters, while host threads set up the next
-    batch (one-electron integrals, the screened two-electron integrals and,
-    in the learned form, the network's density) on a second set of contexts
-    and streams.  One step is one batch of k full solves from the molecules;
-    the timed steps start with the first batch's set-up and end with the last
-    batch's SCF.  Every energy is checked against the same cluster's plain
-    FP64 single-system solve, run beforehand outside the timed region."""
+    myqc_amd.many.ManySolver: host threads set the batch up (one-electron
+    integrals, the screened two-electron integrals and, in the learned form,
+    the network's density; one context and stream per cluster), then one
+    lockstep SCF of the k clusters (myqc_rhf_scf_many) runs on one stream,
+    where every step but the digestion passes is one launch for all
+    clusters.  One step is one batch of k full solves from the molecules.
+    Every energy is checked against the same cluster's plain FP64
+    single-system solve, run beforehand outside the timed region."""
     from myqc_amd import guess as G
     from myqc_amd import molecules as M
     from myqc_amd.basis import build_basis, read_ftab
@@ -175,8 +174,8 @@ def lockstep_leg(waters, k, steps, warmup, forms):
         dt = time.perf_counter() - t0
         tim = np.array(solver.timings)
         solver.close()
-        log(f"[bench] lockstep {name}: per batch set-up {1e3 * tim[:, 0].mean():.1f} ms (beside the previous "
-            f"batch's SCF), lockstep SCF {1e3 * tim[:, 1].mean():.1f} ms")
+        log(f"[bench] lockstep {name}: per batch set-up {1e3 * tim[:, 0].mean():.1f} ms, "
+            f"lockstep SCF {1e3 * tim[:, 1].mean():.1f} ms")
         dmax = max(abs(r[i]["E"] - e64[i]) for r in rs for i in range(k))
         its = float(np.mean([r[i]["iterations"] for r in rs for i in range(k)]))
         log(f"[bench] lockstep {name}: {k} clusters x {steps} batches in {dt * 1e3:.1f} ms "
@@ -185,39 +184,59 @@ def lockstep_leg(waters, k, steps, warmup, forms):
                          mean_iterations=its, E_first=rs[-1][0]["E"], E_first_fp64=e64[0],
                          setup_ms=1e3 * float(tim[:, 0].mean()), scf_ms=1e3 * float(tim[:, 1].mean()))
     return dict(workload=f"{k} RHF/STO-3G {waters}-water clusters (seeds {waters}-{waters + k - 1}) per batch, "
-                         f"one lockstep SCF per batch, the next batch set up meanwhile", clusters=k, forms=out)
+                         f"set up side by side, one lockstep SCF per batch", clusters=k, forms=out)
 
 
-def cpu_baseline(Z, xyz, basis, ftab, n_iter, budget_s=20.0):
-    """The oracle's restatement of the reference path (port), single core:
-    int2e's per-bra-set-pair work timed on a random sample of bra set pairs
-    and scaled to all nset^2, plus the RHFI2G n^4 G pass timed on a 12-water
-    slice and scaled by (n/n_slice)^4, times the solve's iteration count."""
+def cpu_baseline(Z, xyz, basis, ftab, n_iter, budget_s=12.0, batch_mols=None, cores=16):
+    """CPU numbers beside the GPU rate (oracle/baseline.py), on this host:
+
+    * the headline solve by the oracle's port, single core: int2e (proc2e)
+      from a stratified sample of bra set pairs (ratio estimator on a
+      screen-based cost model, bootstrap standard error), plus the RHFI2G n^4
+      pass timed at n = 56 and 84 and scaled to n, times the solve's
+      iteration count;
+    * the reference's own programs (built from their sources into
+      oracle/_ref) against the port: int2e on CO2, H2O and (H2O)2, one
+      RHFI2G pass at n = 56; the headline solve scaled by those ratios;
+    * the batch: the reference's solve (int1e, int2e, the SCF loop spawning
+      dens and RHFI2G) over molecules of the seeded batch, one process per
+      core, for about 20 s."""
     from myqc_amd import molecules as M
-    from myqc_amd.basis import build_basis
+    from oracle import baseline as BL
     from oracle import oracle as O
     sysm = O.System(Z, xyz, basis, ftab)
-    # calibrate the sample size to the budget
-    t, cnt, scale = O.eri_sample_time(sysm, 4, seed=1)
-    per_pair = max(t / 4, 1e-6)
-    npairs = int(max(4, min(sysm.nset ** 2, 0.7 * budget_s / per_pair)))
-    t, cnt, scale = O.eri_sample_time(sysm, npairs, seed=2)
-    t_int2e = t * scale
-    Zs, xs = M.water_cluster(12, seed=12)
-    xs = M.to_bohr(Zs, xs)
-    ssl = O.System(Zs, xs, build_basis(Zs), ftab)
-    ns = ssl.n
-    rng = np.random.default_rng(0)
-    XX = rng.standard_normal(ns ** 4)
-    D = rng.standard_normal((ns, ns))
-    t0 = time.perf_counter()
-    O.fock_G(XX.reshape(ns, ns, ns, ns, order="F"), D)
-    t_g = (time.perf_counter() - t0) * (sysm.n / ns) ** 4
-    total = t_int2e + n_iter * t_g
-    return dict(value=1.0 / total, unit="RHF solves/s", cores=1, kind="port",
-                sample=f"oracle int2e on {npairs} of {sysm.nset ** 2} bra set pairs (x{scale:.0f}) + "
-                       f"RHFI2G n^4 pass timed at n={ns} scaled (n/{ns})^4, x{n_iter} iterations; "
-                       f"extrapolated {total:.0f} s per solve")
+    est = BL.stratified_int2e(sysm, basis, xyz, budget_s=budget_s, seed=2)
+    g = BL.rhfi2g_time(56, 84, basis.norb)
+    t_port = est["seconds"] + n_iter * g["scaled"]
+    out = dict(value=1.0 / t_port, unit="RHF solves/s", cores=1, kind="port",
+               sample=(f"oracle port: int2e on {est['pairs_sampled']} of {est['pairs_total']} bra set pairs in "
+                       f"{est['strata']} cost strata ({est['sample_seconds']:.1f} s timed), "
+                       f"{est['seconds']:.0f} +- {est['stderr']:.0f} s (bootstrap); RHFI2G n^4 pass timed at "
+                       f"n=56, 84 (exponent {g['exponent']:.2f}) and scaled to n={basis.norb}: "
+                       f"{g['scaled']:.1f} s x {n_iter} iterations; {t_port:.0f} s per solve"),
+               int2e_s=est["seconds"], int2e_stderr_s=est["stderr"], rhfi2g_pass_s=g["scaled"])
+    mols = []
+    for k in (1, 2):
+        Zw, xw = M.water_cluster(k, seed=k)
+        mols.append((f"(H2O){k}", Zw, M.to_bohr(Zw, xw)))
+    mols.append(("CO2", [6, 8, 8], M.to_bohr([6, 8, 8], np.array([[0, 0, 0], [0, 0, 1.16], [0, 0, -1.16]]))))
+    cal = BL.calibrate(mols)
+    if cal:
+        t_ref = est["seconds"] * cal["int2e_ref_over_port"] + n_iter * g["scaled"] * cal["rhfi2g_ref_over_port"]
+        out["reference_scaled"] = {"value": 1.0 / t_ref, "unit": "RHF solves/s", "cores": 1,
+                                   "kind": "reference (scaled)",
+                                   "int2e_ref_over_port": cal["int2e_ref_over_port"],
+                                   "rhfi2g_ref_over_port": cal["rhfi2g_ref_over_port"],
+                                   "calibration": cal["rows"], "seconds_per_solve": t_ref}
+    if batch_mols:
+        rate = BL.batch_reference_rate(batch_mols, cores, budget_s=20.0)
+        out["batch_reference"] = {"value": rate["value"], "unit": "converged solves/s", "cores": cores,
+                                  "kind": "reference",
+                                  "sample": f"{rate['molecules']} molecules of the seeded batch finished in "
+                                            f"{rate['seconds']:.1f} s on {cores} processes, "
+                                            f"{rate['converged']} converged, {rate['mean_seconds']:.1f} s each "
+                                            f"(the quicker ones: the rate leans in the CPU's favour)"}
+    return out
 
 
 def mp2_leg(ctx, waters, reps=2):
@@ -487,8 +506,17 @@ def main():
     if batch:
         out["batch"] = batch
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle port, sampled)")
-        out["cpu_baseline"] = cpu_baseline(Z, xyz, basis, ftab, p["iters"])
+        log("[bench] CPU baseline (oracle port, sampled; reference programs for calibration and the batch)")
+        bm = None
+        if args.batch:
+            brng = np.random.default_rng(args.batch)
+            bm = []
+            for _ in range(args.batch):
+                Zb, xb = M.random_molecule(brng)
+                bm.append((Zb, M.to_bohr(Zb, xb)))
+            bm = [bm[i] for i in np.random.default_rng(5).permutation(len(bm))]
+        out["cpu_baseline"] = cpu_baseline(Z, xyz, basis, ftab, p["iters"], batch_mols=bm,
+                                           cores=min(16, os.cpu_count() or 1))
     print(json.dumps(out))
 
 

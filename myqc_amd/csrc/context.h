@@ -123,6 +123,7 @@ struct myqc_ctx {
 
     // ao2mo output kept on the device for mp2
     double *d_iajb = nullptr;             // [i][a][j][b] with i fastest (o*v*o*v)
+    double *d_q12 = nullptr, *d_q3 = nullptr;   // the RHF transform's quarter-step intermediates (kept)
     uint16_t *d_planes = nullptr;         // split-bf16 GEMM: hi/lo planes of the small operand
     size_t iajb_elems = 0;
     int iajb_nocc = 0, iajb_nv = 0;

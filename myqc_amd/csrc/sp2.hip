@@ -229,6 +229,117 @@ __global__ __launch_bounds__(64 * KS) void k_sp2_step_many(Sp2Many a, int n, int
                   blockIdx.x);
 }
 
+// The batched step in 64 x 64 output tiles: four wavefronts per workgroup,
+// each a 32 x 32 quarter (2 x 2 MFMA blocks, so every A and B fragment read
+// from LDS feeds two MFMAs); the k range goes through LDS in chunks of 16 (A
+// rows and B columns of the tile, both read as columns of the symmetric P).
+// Step 0 reads the nin0 trace partials of the start kernel, later steps the
+// tiles' own; the choice and the convergence record are sp2_step_tile's.
+constexpr int SP2T = 64, SP2K = 16;
+
+__global__ __launch_bounds__(256) void k_sp2_step_many64(Sp2Many a, int n, int stride, int nin0, int step,
+                                                         int first) {
+    __shared__ double sA[SP2K][SP2T + 1], sB[SP2K][SP2T + 1];
+    __shared__ double s_t[4], s_f[4];
+    const int k = blockIdx.y;
+    const double *P = (step & 1) ? a.W[k] : a.P[k];
+    double *Pn = (step & 1) ? a.P[k] : a.W[k];
+    const double *tin = a.tp[k] + 2 * (size_t)stride * ((step + 1) & 1);
+    double *tout = a.tp[k] + 2 * (size_t)stride * (step & 1);
+    int *ctrl = reinterpret_cast<int *>(a.slots[k] + 8);
+    const int tiles = (n + SP2T - 1) / SP2T, tile = blockIdx.x;
+    const int ti = tile % tiles, tj = tile / tiles;
+    const int nin = step == 0 ? nin0 : tiles * tiles;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
+    const int wr = w & 1, wc = w >> 1;
+    // the traces of P from the previous step's partials
+    double t_in = 0.0, f_in = 0.0;
+    for (int i = threadIdx.x; i < nin; i += blockDim.x) {
+        t_in += tin[2 * i];
+        f_in += tin[2 * i + 1];
+    }
+    t_in = wsum(t_in);
+    f_in = wsum(f_in);
+    if (lane == 0) {
+        s_t[w] = t_in;
+        s_f[w] = f_in;
+    }
+    const int seen = __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const double tr = (s_t[0] + s_t[1]) + (s_t[2] + s_t[3]), fro = (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
+    const double nocc = a.nocc[k];
+    const bool crit = step >= first && fabs(tr - fro) < 1e-8 && fabs(tr - nocc) < 1e-6;
+    const int cs = (seen >= 0 && seen < step) ? seen : (crit ? step : -1);
+    if (cs >= 0 && step >= cs + 2) return;   // (uniform over the workgroup)
+    if (tile == 0 && threadIdx.x == 0 && seen < 0 && crit) ctrl[0] = step;
+    // P^2 on the tile
+    const int row0 = ti * SP2T, col0 = tj * SP2T;
+    d4_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += SP2K) {
+        // 2 x 16 x 64 values, eight per thread: the tile's 64 rows (A side)
+        // and 64 columns (B side, rows of the symmetric P) at columns k of P,
+        // each a contiguous run of 64
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = threadIdx.x + 256 * q;          // 0 .. 2047
+            const int side = e >> 10, kk = (e >> 6) & 15, i = e & 63;
+            const int kg = k0 + kk;
+            const int rg = (side ? col0 : row0) + i;
+            const double v = (kg < n && rg < n) ? P[rg + (size_t)n * kg] : 0.0;
+            if (side) sB[kk][i] = v;
+            else sA[kk][i] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < SP2K; ks += 4) {
+            double fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) fa[x] = sA[ks + kq][wr * 32 + x * 16 + r];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) fb[y] = sB[ks + kq][wc * 32 + y * 16 + r];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // the choice, the new tile and its trace partials
+    const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
+    double t = 0.0, f = 0.0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = row0 + wr * 32 + x * 16 + kq + 4 * v, col = col0 + wc * 32 + y * 16 + r;
+                if (row < n && col < n) {
+                    const double p2 = acc[x][y][v];
+                    const double nw = square ? p2 : 2.0 * P[row + (size_t)n * col] - p2;
+                    Pn[row + (size_t)n * col] = nw;
+                    f += nw * nw;
+                    if (row == col) t += nw;
+                }
+            }
+    t = wsum(t);
+    f = wsum(f);
+    __syncthreads();
+    if (lane == 0) {
+        s_t[w] = t;
+        s_f[w] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        tout[2 * tile] = (s_t[0] + s_t[1]) + (s_t[2] + s_t[3]);
+        tout[2 * tile + 1] = (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
+    }
+}
+
 // After the launched steps: status = {converged, final step}; the final
 // projector (the input of step ctrl[0] + 2) is moved into P when it sits in W.
 __global__ void k_sp2_finish(double *__restrict__ P, const double *__restrict__ W, int n, int *__restrict__ ctrl) {
@@ -350,6 +461,7 @@ int gershgorin_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n) {
 int sp2_purify_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n, int *status, std::vector<int> &ok, int *steps) {
     if (K < 1 || K > kMany) return set_error(ctx, MYQC_E_ARG, "sp2_purify_many: %d systems", K);
     const int tiles = (n + 15) / 16, tt = tiles * tiles;
+    const int t64 = (n + SP2T - 1) / SP2T, tt64 = t64 * t64;
     k_gersh_rows_many<<<dim3(n, K), 64, 0, ctx->stream>>>(a, n);
     k_gersh_reduce_many<<<K, 1024, 0, ctx->stream>>>(a, n);
     k_sp2_start_many<<<dim3(tt, K), 256, 0, ctx->stream>>>(a, n, tt);
@@ -359,7 +471,7 @@ int sp2_purify_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n, int *status, 
     ok.assign(K, 0);
     while (true) {
         for (; launched < target; ++launched)
-            k_sp2_step_many<<<dim3(tt, K), 64 * KS, 0, ctx->stream>>>(a, n, tt, launched, first);
+            k_sp2_step_many64<<<dim3(tt64, K), 256, 0, ctx->stream>>>(a, n, tt, tt, launched, first);
         k_sp2_finish_many<<<dim3(64, K), 256, 0, ctx->stream>>>(a, n, status);
         MYQC_HIP(ctx, hipGetLastError());
         MYQC_HIP(ctx, d2h(ctx, st.data(), status, sizeof(int) * 2 * K));

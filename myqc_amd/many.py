@@ -1,9 +1,11 @@
 """Many molecules of one basis size, solved in batches of up to 32 by the
-lockstep RHF (myqc_rhf_scf_many), with the set-up of the next batch
-(one-electron integrals, the screened two-electron integrals and, with the
-learned guess, the network's density) running on host threads and HIP
-streams of its own while the current batch's SCF runs: two sets of contexts
-take turns.
+lockstep RHF (myqc_rhf_scf_many).  Each batch is set up (one-electron
+integrals, the screened two-electron integrals and, with the learned guess,
+the network's density) on host threads, one context and HIP stream per
+molecule, then solved in one lockstep SCF.  With pipeline=True the next
+batch's set-up runs beside the current batch's SCF, on a second set of
+contexts (the two sets take turns); with the GPU's few hardware queues
+shared by both, that pays only when the set-up is mostly host work.
 
     solver = ManySolver(k=16, learned=True, precision=FP32)
     for results in solver.solve(batches):   # batches: lists of (Z, xyz_bohr)
@@ -23,17 +25,18 @@ from .hip import FP64, Context, enuc, rhf_many
 
 
 class ManySolver:
-    """Two sets of k contexts on one device: while the lockstep SCF of one
-    batch runs on one set, `workers` host threads set up the next batch on
-    the other.  learned: start from the trained network's density (each
-    context loads it once); precision: FP64 or FP32 (mixed)."""
+    """Two sets of k contexts on one device, used in turn by consecutive
+    batches; `workers` host threads set a batch up.  learned: start from the
+    trained network's density (each context loads it once); precision: FP64
+    or FP32 (mixed); pipeline: set up the next batch while the current one
+    solves."""
 
     def __init__(self, k: int, device: int = 0, learned: bool = False, precision: int = FP64,
                  workers: int = 8, schwarz_tol: float = 1e-12, e_tol: float = 1e-10, d_tol: float = 1e-8,
-                 max_iter: int = 200):
+                 max_iter: int = 200, pipeline: bool = False):
         if not 1 <= k <= 32:
             raise ValueError(f"ManySolver: k={k} (1..32)")
-        self.k, self.learned, self.precision = k, learned, precision
+        self.k, self.learned, self.precision, self.pipeline = k, learned, precision, pipeline
         self.tol, self.e_tol, self.d_tol, self.max_iter = schwarz_tol, e_tol, d_tol, max_iter
         self.ftab = read_ftab()
         self.sets = [[Context(device) for _ in range(k)] for _ in range(2)]
@@ -74,7 +77,7 @@ class ManySolver:
         fut = self.lead.submit(self._setup, 0, batches[0])
         for b in range(len(batches)):
             hs, t_setup = fut.result()
-            if b + 1 < len(batches):
+            if b + 1 < len(batches) and self.pipeline:
                 fut = self.lead.submit(self._setup, (b + 1) % 2, batches[b + 1])
             ctxs = self.sets[b % 2][:len(hs)]
             t0 = time.perf_counter()
@@ -83,6 +86,8 @@ class ManySolver:
                            d_tol=self.d_tol, max_iter=self.max_iter, precision=self.precision,
                            want_orbitals=want_orbitals)
             self.timings.append((t_setup, time.perf_counter() - t0))
+            if b + 1 < len(batches) and not self.pipeline:
+                fut = self.lead.submit(self._setup, (b + 1) % 2, batches[b + 1])
             yield res
 
     def close(self):

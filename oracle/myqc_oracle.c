@@ -20,7 +20,9 @@
  * Index layouts are the reference's: bas(set*OpS+k), basinfo, setinfo,
  * xyz(nnuc,3) column-major in bohr, XX(i,j,g,h) column-major with i fastest.
  */
+#define _POSIX_C_SOURCE 199309L   /* clock_gettime under -std=c99 */
 #include <math.h>
+#include <time.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -290,9 +292,27 @@ int oracle_eri_full(int nnuc, const double *xyz, int nset, const double *alpha, 
  * set indices) against every ket set pair, accumulated into a scratch that is
  * indexed like XX's canonical elements for those bra orbitals.  Returns the
  * number of primitive set quartets evaluated. */
+/* the same with times[t] = seconds spent on bra set pair t (times may be NULL) */
+long long oracle_eri_sample_timed(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                                  const double *bas, int norb, const int *basinfo, const double *Ft,
+                                  int nab, const int *abs_, double *checksum, double *times);
+
 long long oracle_eri_sample(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
                             const double *bas, int norb, const int *basinfo, const double *Ft,
                             int nab, const int *abs_, double *checksum) {
+    return oracle_eri_sample_timed(nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo, Ft, nab, abs_, checksum,
+                                   NULL);
+}
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+long long oracle_eri_sample_timed(int nnuc, const double *xyz, int nset, const double *alpha, const int *setinfo,
+                                  const double *bas, int norb, const int *basinfo, const double *Ft,
+                                  int nab, const int *abs_, double *checksum, double *times) {
     Sys s;
     init_sys(&s, nnuc, xyz, nset, alpha, setinfo, bas, norb, basinfo);
     int n = norb;
@@ -308,6 +328,7 @@ long long oracle_eri_sample(int nnuc, const double *xyz, int nset, const double 
     double pi = pi_sp();
     static Rtab R;
     for (int t = 0; t < nab; ++t) {
+        const double t0 = times ? now_s() : 0.0;
         int a = abs_[2 * t], b = abs_[2 * t + 1];
         const Pair *ab = &pairs[a * nset + b];
         for (int c = 0; c < nset; ++c)
@@ -339,6 +360,7 @@ long long oracle_eri_sample(int nnuc, const double *xyz, int nset, const double 
                 }
                 ++cnt;
             }
+        if (times) times[t] = now_s() - t0;
     }
     double cs = 0.0;
     for (size_t i = 0; i < 16 * n2; ++i) cs += acc[i];

@@ -128,7 +128,7 @@ def test_pipelined_solver_matches_single_solves(ctx):
             Z, x = M.water_cluster(6, seed=s)
             b.append((Z, M.to_bohr(Z, x)))
         batches.append(b)
-    solver = ManySolver(3, precision=FP32, workers=3)
+    solver = ManySolver(3, precision=FP32, workers=3, pipeline=True)
     try:
         results = list(solver.solve(batches))
     finally:
