@@ -140,9 +140,11 @@ def lockstep_leg(waters, k, steps, warmup, forms):
     myqc_amd.many.ManySolver: host threads set the batch up (one-electron
     integrals, the screened two-electron integrals and, in the learned form,
     the network's density; one context and stream per cluster), then one
-    lockstep SCF of the k clusters (myqc_rhf_scf_many) runs on one stream,
-    where every step but the digestion passes is one launch for all
-    clusters.  One step is one batch of k full solves from the molecules.
+    lockstep SCF of the k clusters (myqc_rhf_scf_many), where every step but
+    the digestion passes is one launch for all clusters and the digestion
+    passes fan out over four streams.  The next batch is set up on a second
+    set of contexts while the current one solves.  One step is one batch of
+    k full solves from the molecules.
     Every energy is checked against the same cluster's plain FP64
     single-system solve, run beforehand outside the timed region."""
     from myqc_amd import guess as G
@@ -164,7 +166,7 @@ def lockstep_leg(waters, k, steps, warmup, forms):
     for name, prec, learned in (("fp64", FP64, False), ("learned_mixed", FP32, True)):
         if name not in forms or (learned and not have_net):
             continue
-        solver = ManySolver(k, learned=learned, precision=prec, workers=8)
+        solver = ManySolver(k, learned=learned, precision=prec, workers=8, pipeline=True)
         # at least two warm-up batches, so that both context sets hold their
         # buffers before the timed batches
         for _ in solver.solve([mols] * max(warmup, 2)):
@@ -241,8 +243,9 @@ def cpu_baseline(Z, xyz, basis, ftab, n_iter, budget_s=12.0, batch_mols=None, co
 
 def mp2_leg(ctx, waters, reps=2):
     """RHF-MP2 of a seeded water cluster: the ao2mo quarter transforms in FP64
-    (rocBLAS DGEMM) and split-bf16 (MFMA, FP32 accumulation), each timed over
-    its four GEMM launches with HIP events; E(MBPT2) of both forms."""
+    (hand-written FP64 MFMA kernels) and split-bf16 (bf16 MFMA, FP32
+    accumulation), each timed over its quarter-step launches with HIP events;
+    E(MBPT2) of both forms."""
     from myqc_amd import molecules as M
     from myqc_amd.basis import build_basis, read_ftab
     from myqc_amd.hip import BF16X3, FP64, T_AO2MO, enuc

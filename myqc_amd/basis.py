@@ -60,15 +60,51 @@ def _records(path):
                 yield toks
 
 
-def build_basis(Z, basis_name: str = "STO-3G", path: str | None = None) -> Basis:
-    path = path or os.path.join(DATA_DIR, "mybasis")
+_PARSED: dict = {}
+
+
+def _parsed(path: str, basis_name: str):
+    """The basis section of ``mybasis`` parsed once per (file, name, mtime):
+    the header numbers and, per element, (sec, orb, nset, sections) with
+    sections = [(func, coef, pri, ang, ori, [(values, setn), ...])], read as
+    basis.f90:60-150 reads them."""
+    key = (path, basis_name, os.stat(path).st_mtime_ns)
+    hit = _PARSED.get(key)
+    if hit is not None:
+        return hit
     recs = list(_records(path))
     try:
         start = next(i for i, r in enumerate(recs) if r[0] == basis_name)
     except StopIteration:
         raise ValueError(f"basis {basis_name!r} not in {path}")
-    Smax, Cmax, Omax, almax, OpS = (int(x) for x in recs[start + 1][:5])
+    head = tuple(int(x) for x in recs[start + 1][:5])
     maxN, maxL = (int(x) for x in recs[start + 2][:2])
+    elements = {}
+    for name in ATOM_NAMES:
+        pos = next((k for k in range(start + 3, len(recs)) if recs[k][0] == name), None)
+        if pos is None:
+            continue
+        sec, orb, nset = (int(x) for x in recs[pos + 1][:3])
+        p = pos + 2
+        sections = []
+        for _ in range(sec):
+            func, coef, pri, ang, ori = (int(x) for x in recs[p][:5])
+            p += 1
+            rows = []
+            for _k in range(func):
+                r = recs[p]
+                p += 1
+                rows.append(([float(x) for x in r[:coef]], int(round(float(r[coef])))))
+            sections.append((func, coef, pri, ang, ori, rows))
+        elements[name] = (sec, orb, nset, sections)
+    out = (head, maxN, maxL, elements)
+    _PARSED[key] = out
+    return out
+
+
+def build_basis(Z, basis_name: str = "STO-3G", path: str | None = None) -> Basis:
+    path = path or os.path.join(DATA_DIR, "mybasis")
+    (Smax, Cmax, Omax, almax, OpS), maxN, maxL, elements = _parsed(path, basis_name)
     anum = len(Z)
     bas = np.zeros(anum * almax * OpS)
     sets = np.zeros(anum * almax)
@@ -79,24 +115,15 @@ def build_basis(Z, basis_name: str = "STO-3G", path: str | None = None) -> Basis
     setl = 3 + OpS
     for i, z in enumerate(Z):
         name = ATOM_NAMES[z - 1]
-        try:
-            pos = next(k for k in range(start + 3, len(recs)) if recs[k][0] == name)
-        except StopIteration:
+        if name not in elements:
             raise ValueError(f"atom {name} not in basis {basis_name}")
-        sec, orb, nset = (int(x) for x in recs[pos + 1][:3])
+        sec, orb, nset, sections = elements[name]
         basinfo[0] = OpS
         basinfo[1] += orb
         setinfo[0] += nset
         setinfo[1] = setl
-        p = pos + 2
-        for _ in range(sec):
-            func, coef, pri, ang, ori = (int(x) for x in recs[p][:5])
-            p += 1
-            for _k in range(func):
-                r = recs[p]
-                p += 1
-                val = [float(x) for x in r[:coef]]
-                setn = int(round(float(r[coef])))
+        for func, coef, pri, ang, ori, rows in sections:
+            for val, setn in rows:
                 s = setnum + setn
                 sets[s] = val[coef - 1]
                 setorbs = int(setinfo[2 + s * setl])

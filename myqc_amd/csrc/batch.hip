@@ -162,8 +162,13 @@ struct BScfArgs {
     double *Dout;
     double *energy;
     int *iters, *conv;
+    int *fallback;             // NULL, or 1 where a learned start restarted from the core guess
     double e_tol, d_tol;
     int max_iter;
+    int learned_iter;          // iterations a start from `guess` gets before the restart
+    int restart;               // 1: restart an unconverged start from `guess` from the core guess
+    int shift_after;           // iterations after which the virtual space is level-shifted
+    double shift;              // the shift (Eh)
 };
 
 // Workgroup reductions: a wavefront butterfly, then the per-wave results
@@ -564,6 +569,7 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
                 a.energy[m] = 0.0;
                 a.iters[m] = 0;
                 a.conv[m] = -1;   // overlap numerically singular
+                if (a.fallback) a.fallback[m] = 0;
             }
             __syncthreads();
             continue;
@@ -572,7 +578,15 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
         __syncthreads();
         mm(X, L3, L2, n, true, 1.0);
         __syncthreads();
-        if (a.guess) {
+        // a start from the given density gets learned_iter iterations; if it
+        // does not converge, the molecule restarts from the core guess in the
+        // same workgroup with the full budget
+        int it_total = 0, conv = 0, it = 0;
+        double E = 0.0;
+        for (int attempt = a.guess ? 0 : 1; attempt < 2; ++attempt) {
+        const bool from_guess = attempt == 0;
+        const int maxit = from_guess ? a.learned_iter : a.max_iter;
+        if (from_guess) {
             // Dp = S^{1/2} D S^{1/2} / 2
             for (int e = threadIdx.x; e < (int)nn; e += blockDim.x) L3[e] = L2[e] * sqrt(L0[(e / n) * (n + 1)]);
             __syncthreads();
@@ -598,9 +612,10 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
             mm(L1, X, L3, n, false, 2.0);
             __syncthreads();
         }
-        int hist = 0, head = 0, it = 0, conv = 0;
-        double E = 0.0, E_old = 0.0, dE_prev = 1e300, err_prev = 1e300;
-        for (it = 1; it <= a.max_iter + 1; ++it) {
+        int hist = 0, head = 0;
+        double E_old = 0.0, dE_prev = 1e300, err_prev = 1e300;
+        E = 0.0;
+        for (it = 1; it <= maxit + 1; ++it) {
             // F = H + G(D) into W; E = (D.F + D.H)/2 + E_nuc
             fock(eri, Hg, L1, L0, L2, W, n, s_pi, s_pj);
             double ed = 0.0;
@@ -616,7 +631,7 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
                     break;
                 }
             }
-            if (it > a.max_iter) break;
+            if (it > maxit) break;
             // F' = X F X into the history; error F' Dp - Dp F'
             double *fs = hF + head * nn, *es = hE + head * nn;
             mm(L3, W, X, n, false, 1.0);
@@ -671,6 +686,16 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
             E_old = E;
             head = (head + 1) % MD;
             __syncthreads();
+            // a run that has not converged within shift_after iterations has
+            // its virtual space raised: F' + shift (I - Dp), whose occupied
+            // projector is the same at self-consistency
+            if (it > a.shift_after) {
+                for (int e = threadIdx.x; e < (int)nn; e += blockDim.x) {
+                    const int i = e % n, j = e / n;
+                    L0[e] += a.shift * ((i == j ? 1.0 : 0.0) - Dp[e]);
+                }
+                __syncthreads();
+            }
             // new orthonormal density: the projector onto the nocc lowest
             // levels of F' by purification, or by diagonalisation when the
             // frontier levels are too close for it
@@ -689,10 +714,16 @@ __global__ __launch_bounds__(512, 4) void k_batch_scf(BScfArgs a, int nmax) {
             mm(L1, X, W, n, false, 2.0);
             __syncthreads();
         }
+        it_total += min(it - 1, maxit);
+        if (conv || !from_guess || !a.restart) {
+            if (threadIdx.x == 0 && a.fallback) a.fallback[m] = from_guess ? 0 : (a.guess ? 1 : 0);
+            break;
+        }
+        }   // attempts
         for (int e = threadIdx.x; e < (int)nn; e += blockDim.x) a.Dout[mol.mat0 + e] = L1[e];
         if (threadIdx.x == 0) {
             a.energy[m] = E;
-            a.iters[m] = min(it - 1, a.max_iter);
+            a.iters[m] = it_total;
             a.conv[m] = conv;
         }
         __syncthreads();
@@ -802,8 +833,25 @@ static int scf_grid(myqc_ctx *ctx, const BatchState &b) {
     return std::max(1, std::min(b.nmol, 4 * cus));
 }
 
+// the level shift of runs that stall (MYQC_BATCH_SHIFT, MYQC_BATCH_SHIFT_AFTER; read once)
+static void batch_shift(double &shift, int &after) {
+    static const double s = [] {
+        const char *e = getenv("MYQC_BATCH_SHIFT");
+        return e ? atof(e) : 0.5;
+    }();
+    static const int a = [] {
+        const char *e = getenv("MYQC_BATCH_SHIFT_AFTER");
+        return e ? atoi(e) : 1 << 30;
+    }();
+    shift = s;
+    after = a;
+}
+
+// restart: a start from the guess that does not converge within
+// learned_iter iterations restarts from the core guess (same launch)
 static void launch_batch_scf(myqc_ctx *ctx, BatchState &b, bool guess, double e_tol, double d_tol, int max_iter,
-                             int threads, const int *order = nullptr, int count = -1) {
+                             int threads, const int *order = nullptr, int count = -1, bool restart = false,
+                             int learned_iter = 0) {
     BScfArgs a;
     a.mols = b.d_mols;
     a.order = order ? order : b.d_order;
@@ -822,6 +870,10 @@ static void launch_batch_scf(myqc_ctx *ctx, BatchState &b, bool guess, double e_
     a.e_tol = e_tol;
     a.d_tol = d_tol;
     a.max_iter = max_iter;
+    a.restart = (guess && restart) ? 1 : 0;
+    a.learned_iter = a.restart ? std::min(learned_iter, max_iter) : max_iter;
+    a.fallback = b.d_fail;
+    batch_shift(a.shift, a.shift_after);
     k_zero_int<<<1, 64, 0, ctx->stream>>>(b.d_counter, 1);
     const int grid = std::max(1, std::min(scf_grid(ctx, b), a.nmol));
     k_batch_scf<<<grid, threads, scf_lds_bytes(b.nmax, threads), ctx->stream>>>(a, b.nmax);
@@ -1153,9 +1205,10 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
     MYQC_CHECK(guess_reserve(ctx, b.npair));
     if (!b.tuned) MYQC_CHECK(batch_tune(ctx, b, false, e_tol, d_tol, max_iter));
     // the learned start gets a shorter budget (typical solves take about 25
-    // iterations); what it leaves unconverged restarts from the core guess
+    // iterations); a molecule it leaves unconverged restarts from the core
+    // guess inside the same persistent launch
     const int learned_iter = std::min(max_iter, 48);
-    if (!b.graph_learned || b.gl_etol != e_tol || b.gl_dtol != d_tol || b.gl_maxit != learned_iter ||
+    if (!b.graph_learned || b.gl_etol != e_tol || b.gl_dtol != d_tol || b.gl_maxit != max_iter ||
         b.gl_gen != ctx->mlp_gen) {
         if (b.graph_learned) (void)hipGraphExecDestroy(b.graph_learned);
         b.graph_learned = nullptr;
@@ -1166,7 +1219,7 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
         for (int c = 0; c < 6; ++c)
             if (b.nq[c] > 0) launch_batch_eri(ctx, b, c, b.eri_threads[c]);
         guess_launch(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.d_guess);
-        launch_batch_scf(ctx, b, true, e_tol, d_tol, learned_iter, b.scf_threads);
+        launch_batch_scf(ctx, b, true, e_tol, d_tol, max_iter, b.scf_threads, nullptr, -1, true, learned_iter);
         MYQC_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
         hipError_t e = hipGraphInstantiate(&b.graph_learned, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -1177,35 +1230,14 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
         }
         b.gl_etol = e_tol;
         b.gl_dtol = d_tol;
-        b.gl_maxit = learned_iter;
+        b.gl_maxit = max_iter;
         b.gl_gen = ctx->mlp_gen;
     }
     MYQC_HIP(ctx, hipGraphLaunch(b.graph_learned, ctx->stream));
     ++b.replays;
-    std::vector<int> conv(b.nmol), it1(b.nmol);
-    MYQC_HIP(ctx, d2h(ctx, conv.data(), b.d_conv, sizeof(int) * b.nmol));
-    MYQC_HIP(ctx, d2h(ctx, it1.data(), b.d_iters, sizeof(int) * b.nmol));
-    // fallback: molecules the learned start did not bring to convergence
-    // restart from the core-Hamiltonian guess
-    std::vector<int> fail;
-    for (int m = 0; m < b.nmol; ++m)
-        if (conv[m] == 0) fail.push_back(m);
-    if (!fail.empty()) {
-        MYQC_HIP(ctx, hipMemcpyAsync(b.d_fail, fail.data(), sizeof(int) * fail.size(), hipMemcpyHostToDevice,
-                                     ctx->stream));
-        launch_batch_scf(ctx, b, false, e_tol, d_tol, max_iter, b.scf_threads, b.d_fail, (int)fail.size());
-        MYQC_HIP(ctx, hipGetLastError());
-        MYQC_HIP(ctx, d2h(ctx, conv.data(), b.d_conv, sizeof(int) * b.nmol));
-    }
-    std::vector<int> it2(b.nmol);
-    MYQC_HIP(ctx, d2h(ctx, it2.data(), b.d_iters, sizeof(int) * b.nmol));
     MYQC_HIP(ctx, d2h(ctx, energy, b.d_energy, sizeof(double) * b.nmol));
-    std::vector<int> fb(b.nmol, 0);
-    for (int m : fail) fb[m] = 1;
-    for (int m = 0; m < b.nmol; ++m) {
-        if (iterations) iterations[m] = fb[m] ? it1[m] + it2[m] : it1[m];
-        if (converged) converged[m] = conv[m];
-        if (fallback) fallback[m] = fb[m];
-    }
+    if (iterations) MYQC_HIP(ctx, d2h(ctx, iterations, b.d_iters, sizeof(int) * b.nmol));
+    if (converged) MYQC_HIP(ctx, d2h(ctx, converged, b.d_conv, sizeof(int) * b.nmol));
+    if (fallback) MYQC_HIP(ctx, d2h(ctx, fallback, b.d_fail, sizeof(int) * b.nmol));
     return 0;
 }

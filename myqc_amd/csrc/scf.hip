@@ -761,6 +761,11 @@ struct ManyBuffers {
     int *info = nullptr, *status = nullptr;
     RedJob *h_jobs = nullptr;   // pinned
     double *h_red = nullptr;    // pinned
+    // the Fock builds of one iteration fan out over kFan streams (the
+    // contexts' own): one event to start them after the lead stream's work,
+    // one per stream to join them back
+    static constexpr int kFan = 4;
+    hipEvent_t fork = nullptr, join[kFan] = {};
 };
 
 static int many_buffers(myqc_ctx *ctx, int K, int n, int njobs, ManyBuffers *&mb) {
@@ -788,6 +793,11 @@ static int many_buffers(myqc_ctx *ctx, int K, int n, int njobs, ManyBuffers *&mb
         mb->cap_k = K;
         mb->n = n;
     }
+    if (!mb->fork) {
+        MYQC_HIP(ctx, hipEventCreateWithFlags(&mb->fork, hipEventDisableTiming));
+        for (int s = 0; s < ManyBuffers::kFan; ++s)
+            MYQC_HIP(ctx, hipEventCreateWithFlags(&mb->join[s], hipEventDisableTiming));
+    }
     return 0;
 }
 
@@ -799,6 +809,9 @@ void free_many(myqc_ctx *ctx) {
     for (auto p : ps) dfree(p);
     if (mb->h_jobs) hipHostFree(mb->h_jobs);
     if (mb->h_red) hipHostFree(mb->h_red);
+    if (mb->fork) hipEventDestroy(mb->fork);
+    for (hipEvent_t e : mb->join)
+        if (e) hipEventDestroy(e);
     delete mb;
     ctx->many = nullptr;
 }
@@ -870,7 +883,7 @@ static int rhf_scf_many(myqc_ctx *const *ctxs, int K, const double *const *H, co
         myqc_ctx *const *c;
         std::vector<hipStream_t> &s;
         ~Restore() {
-            (void)hipStreamSynchronize(c[0]->stream);
+            for (size_t k = 0; k < s.size(); ++k) (void)hipStreamSynchronize(s[k]);
             for (size_t k = 0; k < s.size(); ++k) {
                 c[k]->stream = s[k];
                 rocblas_set_stream(c[k]->blas, s[k]);
@@ -1052,22 +1065,38 @@ static int rhf_scf_many(myqc_ctx *const *ctxs, int K, const double *const *H, co
         MYQC_CHECK(gemm_many(lead, core, n, Xs, cT1s, DsW, 2.0));
     }
     std::vector<int> active = all;
+    // the fan-out streams of the Fock builds: the contexts' own (the first is
+    // the lead stream)
+    const int nfan = std::min(K, (int)ManyBuffers::kFan);
     while (!active.empty()) {
-        // Fock builds, one system after another (each pass fills the GPU)
-        for (int k : active) {
-            ManyState &q = st[k];
+        // Fock builds: system i of the active list on fan-out stream i mod
+        // nfan, so that one pass's tail overlaps the next pass's start
+        MYQC_HIP(lead, hipEventRecord(mb.fork, lead->stream));
+        for (int f = 1; f < nfan; ++f) MYQC_HIP(lead, hipStreamWaitEvent(own[f], mb.fork, 0));
+        for (size_t i = 0; i < active.size(); ++i) {
+            ManyState &q = st[active[i]];
             ScfBuffers &b = *q.b;
+            hipStream_t fs = own[i % nfan];
+            q.ctx->stream = fs;
             q.full = !incr || q.final_fp64 || q.since_full == 0 || q.since_full >= 8;
+            int rc;
             if (q.full) {
-                if (digest_G(q.ctx, b.D, b.G, MYQC_FP64)) return set_error(lead, MYQC_E_HIP, "%s", q.ctx->err.c_str());
+                rc = digest_G(q.ctx, b.D, b.G, MYQC_FP64);
                 q.since_full = 1;
             } else {
-                k_axpbyc<<<g2, 128, 0, lead->stream>>>(b.D, b.Dref, b.T1, 1.0, -1.0, 0.0, n);
-                if (digest_G(q.ctx, b.T1, b.G, MYQC_FP32)) return set_error(lead, MYQC_E_HIP, "%s", q.ctx->err.c_str());
+                k_axpbyc<<<g2, 128, 0, fs>>>(b.D, b.Dref, b.T1, 1.0, -1.0, 0.0, n);
+                rc = digest_G(q.ctx, b.T1, b.G, MYQC_FP32);
                 ++q.since_full;
             }
-            if (incr) MYQC_HIP(lead, hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, lead->stream));
+            if (rc == 0 && incr && hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, fs) != hipSuccess)
+                rc = set_error(q.ctx, MYQC_E_HIP, "copy of the reference density failed");
+            q.ctx->stream = lead->stream;
+            if (rc) return set_error(lead, MYQC_E_HIP, "%s", q.ctx->err.c_str());
             q.fslot = b.hF + nn * q.head;
+        }
+        for (int f = 1; f < nfan; ++f) {
+            MYQC_HIP(lead, hipEventRecord(mb.join[f], own[f]));
+            MYQC_HIP(lead, hipStreamWaitEvent(lead->stream, mb.join[f], 0));
         }
         // F = H + G; Fo = X F X into the DIIS slot; error Fo Do - Do Fo
         elt(active, [&](EltMany &e, int z, ManyState &q) { e.X[z] = q.b->H; e.Y[z] = q.b->G; e.Z[z] = q.b->F; e.a[z] = 1.0; e.b[z] = 1.0; }, false);

@@ -217,39 +217,43 @@ __global__ __launch_bounds__(64 * KS) void k_sp2_step_mfma(const double *__restr
     sp2_step_tile(P, Pn, n, nocc, tin, nin, tout, step, first, ctrl, blockIdx.x);
 }
 
-// The same step for up to kMany systems of one size at once (blockIdx.y:
-// the system); each system has its own buffers, traces and control word.
-__global__ __launch_bounds__(64 * KS) void k_sp2_step_many(Sp2Many a, int n, int tt, int step, int first) {
-    const int k = blockIdx.y;
-    const double *cur = (step & 1) ? a.W[k] : a.P[k];
-    double *nxt = (step & 1) ? a.P[k] : a.W[k];
-    const double *tin = a.tp[k] + 2 * (size_t)tt * ((step + 1) & 1);
-    double *tout = a.tp[k] + 2 * (size_t)tt * (step & 1);
-    sp2_step_tile(cur, nxt, n, a.nocc[k], tin, tt, tout, step, first, reinterpret_cast<int *>(a.slots[k] + 8),
-                  blockIdx.x);
-}
-
-// The batched step in 64 x 64 output tiles: four wavefronts per workgroup,
-// each a 32 x 32 quarter (2 x 2 MFMA blocks, so every A and B fragment read
-// from LDS feeds two MFMAs); the k range goes through LDS in chunks of 16 (A
-// rows and B columns of the tile, both read as columns of the symmetric P).
+// The batched step for up to kMany systems of one size.  P is symmetric, so
+// so is P^2: a workgroup computes one 64 x 64 tile (ti, tj) with ti <= tj and
+// writes it and, off the diagonal, its mirror.  Four wavefronts, each a
+// 32 x 32 quarter (2 x 2 MFMA blocks, every A and B fragment read from LDS
+// feeding two MFMAs).  The k range goes through LDS in slices of 32 (A rows
+// and B columns of the tile, both read as columns of P; a diagonal tile
+// reads one side only), double-buffered: the next slice's global loads are
+// in flight while the current slice's MFMAs run, with one barrier per slice.
+// Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD
+// b mod 8), and the grid is laid out so that every tile of system k runs on
+// XCD k mod 8, where that system's P stays in one L2.
 // Step 0 reads the nin0 trace partials of the start kernel, later steps the
-// tiles' own; the choice and the convergence record are sp2_step_tile's.
-constexpr int SP2T = 64, SP2K = 16;
+// nup tiles' own (an off-diagonal tile's Frobenius part counts twice); the
+// choice and the convergence record are as in sp2_step_tile.
+constexpr int SYT = 64, SYK = 32;
 
-__global__ __launch_bounds__(256) void k_sp2_step_many64(Sp2Many a, int n, int stride, int nin0, int step,
-                                                         int first) {
-    __shared__ double sA[SP2K][SP2T + 1], sB[SP2K][SP2T + 1];
+__global__ __launch_bounds__(256) void k_sp2_step_sym(Sp2Many a, int K, int n, int stride, int nin0, int step,
+                                                     int first) {
+    __shared__ double sA[2][SYK][SYT + 1], sB[2][SYK][SYT + 1];
     __shared__ double s_t[4], s_f[4];
-    const int k = blockIdx.y;
+    const int tiles = (n + SYT - 1) / SYT, nup = tiles * (tiles + 1) / 2;
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int k = xcd + 8 * (j / nup), tile = j % nup;
+    if (k >= K) return;   // (uniform over the workgroup)
+    int ti = 0, rem = tile;
+    while (rem >= tiles - ti) {
+        rem -= tiles - ti;
+        ++ti;
+    }
+    const int tj = ti + rem;
+    const bool diag = ti == tj;
     const double *P = (step & 1) ? a.W[k] : a.P[k];
     double *Pn = (step & 1) ? a.P[k] : a.W[k];
     const double *tin = a.tp[k] + 2 * (size_t)stride * ((step + 1) & 1);
     double *tout = a.tp[k] + 2 * (size_t)stride * (step & 1);
     int *ctrl = reinterpret_cast<int *>(a.slots[k] + 8);
-    const int tiles = (n + SP2T - 1) / SP2T, tile = blockIdx.x;
-    const int ti = tile % tiles, tj = tile / tiles;
-    const int nin = step == 0 ? nin0 : tiles * tiles;
+    const int nin = step == 0 ? nin0 : nup;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
     const int wr = w & 1, wc = w >> 1;
     // the traces of P from the previous step's partials
@@ -272,43 +276,52 @@ __global__ __launch_bounds__(256) void k_sp2_step_many64(Sp2Many a, int n, int s
     const int cs = (seen >= 0 && seen < step) ? seen : (crit ? step : -1);
     if (cs >= 0 && step >= cs + 2) return;   // (uniform over the workgroup)
     if (tile == 0 && threadIdx.x == 0 && seen < 0 && crit) ctrl[0] = step;
-    // P^2 on the tile
-    const int row0 = ti * SP2T, col0 = tj * SP2T;
+    const int row0 = ti * SYT, col0 = tj * SYT;
+    // element q of a thread's share of one slice: side (q >= 8: B), column
+    // kk of the slice, row i of the tile; 2 x 32 x 64 values, 16 per thread
+    double pre[16];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            const int side = e >> 11, kk = (e >> 6) & (SYK - 1), i = e & 63;
+            const int kg = k0 + kk, rg = (side ? col0 : row0) + i;
+            pre[q] = (kg < n && rg < n && !(side && diag)) ? P[rg + (size_t)n * kg] : 0.0;
+        }
+    };
     d4_t acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = d4_t{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += SP2K) {
-        // 2 x 16 x 64 values, eight per thread: the tile's 64 rows (A side)
-        // and 64 columns (B side, rows of the symmetric P) at columns k of P,
-        // each a contiguous run of 64
+    fetch(0);
+    int buf = 0;
+    for (int k0 = 0; k0 < n; k0 += SYK, buf ^= 1) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = threadIdx.x + 256 * q;          // 0 .. 2047
-            const int side = e >> 10, kk = (e >> 6) & 15, i = e & 63;
-            const int kg = k0 + kk;
-            const int rg = (side ? col0 : row0) + i;
-            const double v = (kg < n && rg < n) ? P[rg + (size_t)n * kg] : 0.0;
-            if (side) sB[kk][i] = v;
-            else sA[kk][i] = v;
+        for (int q = 0; q < 16; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            const int side = e >> 11, kk = (e >> 6) & (SYK - 1), i = e & 63;
+            if (!side) sA[buf][kk][i] = pre[q];
+            else if (!diag) sB[buf][kk][i] = pre[q];
         }
         __syncthreads();
+        if (k0 + SYK < n) fetch(k0 + SYK);
+        const double(*As)[SYT + 1] = sA[buf];
+        const double(*Bs)[SYT + 1] = diag ? sA[buf] : sB[buf];
 #pragma unroll
-        for (int ks = 0; ks < SP2K; ks += 4) {
+        for (int ks = 0; ks < SYK; ks += 4) {
             double fa[2], fb[2];
 #pragma unroll
-            for (int x = 0; x < 2; ++x) fa[x] = sA[ks + kq][wr * 32 + x * 16 + r];
+            for (int x = 0; x < 2; ++x) fa[x] = As[ks + kq][wr * 32 + x * 16 + r];
 #pragma unroll
-            for (int y = 0; y < 2; ++y) fb[y] = sB[ks + kq][wc * 32 + y * 16 + r];
+            for (int y = 0; y < 2; ++y) fb[y] = Bs[ks + kq][wc * 32 + y * 16 + r];
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
                 for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
         }
-        __syncthreads();
     }
-    // the choice, the new tile and its trace partials
+    // the choice, the new tile (and its mirror) and its trace partials
     const bool square = fabs(fro - nocc) < fabs(2.0 * tr - fro - nocc);
     double t = 0.0, f = 0.0;
 #pragma unroll
@@ -322,10 +335,12 @@ __global__ __launch_bounds__(256) void k_sp2_step_many64(Sp2Many a, int n, int s
                     const double p2 = acc[x][y][v];
                     const double nw = square ? p2 : 2.0 * P[row + (size_t)n * col] - p2;
                     Pn[row + (size_t)n * col] = nw;
+                    if (!diag) Pn[col + (size_t)n * row] = nw;
                     f += nw * nw;
                     if (row == col) t += nw;
                 }
             }
+    if (!diag) f *= 2.0;
     t = wsum(t);
     f = wsum(f);
     __syncthreads();
@@ -461,7 +476,9 @@ int gershgorin_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n) {
 int sp2_purify_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n, int *status, std::vector<int> &ok, int *steps) {
     if (K < 1 || K > kMany) return set_error(ctx, MYQC_E_ARG, "sp2_purify_many: %d systems", K);
     const int tiles = (n + 15) / 16, tt = tiles * tiles;
-    const int t64 = (n + SP2T - 1) / SP2T, tt64 = t64 * t64;
+    const int t64 = (n + SYT - 1) / SYT, nup = t64 * (t64 + 1) / 2;
+    if (nup > tt) return set_error(ctx, MYQC_E_ARG, "sp2_purify_many: n=%d", n);
+    const unsigned grid = 8u * (unsigned)((K + 7) / 8) * (unsigned)nup;
     k_gersh_rows_many<<<dim3(n, K), 64, 0, ctx->stream>>>(a, n);
     k_gersh_reduce_many<<<K, 1024, 0, ctx->stream>>>(a, n);
     k_sp2_start_many<<<dim3(tt, K), 256, 0, ctx->stream>>>(a, n, tt);
@@ -471,7 +488,7 @@ int sp2_purify_many(myqc_ctx *ctx, const Sp2Many &a, int K, int n, int *status, 
     ok.assign(K, 0);
     while (true) {
         for (; launched < target; ++launched)
-            k_sp2_step_many64<<<dim3(tt64, K), 256, 0, ctx->stream>>>(a, n, tt, tt, launched, first);
+            k_sp2_step_sym<<<grid, 256, 0, ctx->stream>>>(a, K, n, tt, tt, launched, first);
         k_sp2_finish_many<<<dim3(64, K), 256, 0, ctx->stream>>>(a, n, status);
         MYQC_HIP(ctx, hipGetLastError());
         MYQC_HIP(ctx, d2h(ctx, st.data(), status, sizeof(int) * 2 * K));

@@ -552,12 +552,11 @@ class Context:
 
 def enuc(Z, xyz) -> float:
     """Nuclear repulsion (src/scf/scf.f90:1122-1152)."""
-    xyz = np.asarray(xyz).reshape(-1, 3)
-    e = 0.0
-    for a in range(len(Z)):
-        for b in range(a):
-            e += Z[a] * Z[b] / np.sqrt(((xyz[a] - xyz[b]) ** 2).sum())
-    return e
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    z = np.asarray(Z, dtype=np.float64)
+    a, b = np.tril_indices(len(z), -1)
+    r = np.sqrt(((xyz[a] - xyz[b]) ** 2).sum(axis=1))
+    return float(np.sum(z[a] * z[b] / r))
 
 
 def rhf_many(ctxs, Hs, Ss, noccs, e_nucs, D_guesses=None, e_tol=1e-10, d_tol=1e-8, max_iter=200,
