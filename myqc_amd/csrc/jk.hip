@@ -241,9 +241,12 @@ __device__ __forceinline__ void digest_column_la(const DigArgs<TV> &a, double *s
 }
 
 // One workgroup: stage its D rows, run its column chunks (wavefront w takes
-// chunks w, w + 4, ...), write its rows of A to its partial rows.
+// chunks w, w + 4, ...), write its rows of A to its partial rows.  Three
+// workgroups per CU for the FP64 form (LDS: 16 B per row element); the FP32
+// form's rows take 12 B, so it runs four (at 128 VGPRs, a few spilled: 5-7 %
+// faster per pass than three).
 template <typename TV>
-__global__ __launch_bounds__(256, 3) void k_digest(DigArgs<TV> a) {
+__global__ __launch_bounds__(256, sizeof(TV) == 4 ? 4 : 3) void k_digest(DigArgs<TV> a) {
     extern __shared__ unsigned char smem[];
     const int n = a.n;
     const cptr<int> wr = as_const(a.wg) + (size_t)kWgInts * blockIdx.x;

@@ -220,62 +220,6 @@ __global__ __launch_bounds__(GF_THREADS) void k_gemm_f64(int m, int nb, int k, c
             }
 }
 
-// The same product for up to kMany problems with their own operands
-// (blockIdx.z: the problem): C_z = alpha A_z B_z, all m x nb x k, lda = m,
-// ldb = k, ldc = m.  For the SCF of many systems of one size.
-
-__global__ __launch_bounds__(GF_THREADS) void k_gemm_f64_many(int m, int nb, int k, GemmMany g, double alpha,
-                                                              int tiles_m) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, gg = lane >> 4, c = lane & 15;
-    const int z = blockIdx.z;
-    const double *A = g.A[z], *B = g.B[z];
-    double *C = g.C[z];
-    const int r0 = (int)blockIdx.x * (64 * GF_MT) + 16 * GF_MT * w;
-    const int j0 = blockIdx.y * (16 * GF_NT);
-    d4_t acc[GF_MT][GF_NT];
-#pragma unroll
-    for (int i = 0; i < GF_MT; ++i)
-#pragma unroll
-        for (int j = 0; j < GF_NT; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
-    double fa[GF_MT], fb[GF_NT];
-    auto load = [&](int k0) {
-        const int kk = k0 + gg;
-#pragma unroll
-        for (int i = 0; i < GF_MT; ++i) {
-            const int r = r0 + 16 * i + c;
-            fa[i] = (r < m && kk < k) ? A[r + (size_t)m * kk] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < GF_NT; ++j) {
-            const int col = j0 + 16 * j + c;
-            fb[j] = (col < nb && kk < k) ? B[kk + (size_t)k * col] : 0.0;
-        }
-    };
-    load(0);
-    for (int k0 = 0; k0 < k; k0 += 4) {
-        double a[GF_MT], b[GF_NT];
-#pragma unroll
-        for (int i = 0; i < GF_MT; ++i) a[i] = fa[i];
-#pragma unroll
-        for (int j = 0; j < GF_NT; ++j) b[j] = fb[j];
-        if (k0 + 4 < k) load(k0 + 4);
-#pragma unroll
-        for (int i = 0; i < GF_MT; ++i)
-#pragma unroll
-            for (int j = 0; j < GF_NT; ++j) acc[i][j] = mfma16x16x4(b[j], a[i], acc[i][j]);
-    }
-#pragma unroll
-    for (int i = 0; i < GF_MT; ++i)
-#pragma unroll
-        for (int j = 0; j < GF_NT; ++j)
-#pragma unroll
-            for (int vv = 0; vv < 4; ++vv) {
-                const int r = r0 + 16 * i + c, col = j0 + 16 * j + gg + 4 * vv;
-                if (r < m && col < nb) C[r + (size_t)m * col] = alpha * acc[i][j][vv];
-            }
-    (void)tiles_m;
-}
-
 template <int NKS>
 static void launch12(dim3 grid, hipStream_t s, int np, int nq, int n, const double *Cp, const double *Cq,
                      const double *XX, double *M, long long nld) {
@@ -324,14 +268,90 @@ int gemm_f64(myqc_ctx *ctx, int m, int nb, int k, const double *A, long long lda
     return 0;
 }
 
+// The same product for many problems in 64 x 64 output tiles: four
+// wavefronts, each a 32 x 32 quarter (2 x 2 MFMA blocks); the k range goes
+// through LDS in slices of 32, double-buffered (the next slice's global loads
+// in flight during the current slice's MFMAs, one barrier per slice).  A is
+// read down its columns (64 rows of one column contiguous), B down its
+// columns too (32 k of one column contiguous), both stored [k][row or column]
+// in LDS.  Workgroup b runs on XCD b mod 8; every tile of problem z runs on
+// XCD z mod 8, so that its operands stay in one L2.
+constexpr int GMT = 64, GMK = 32;
+
+__global__ __launch_bounds__(256) void k_gemm_f64_many64(int m, int nb, int k, GemmMany g, double alpha, int nz,
+                                                         int tm, int tn) {
+    __shared__ double sA[2][GMK][GMT + 1], sB[2][GMK][GMT + 1];
+    const int tiles = tm * tn;
+    const int xcd = blockIdx.x & 7, jb = blockIdx.x >> 3;
+    const int z = xcd + 8 * (jb / tiles), tile = jb % tiles;
+    if (z >= nz) return;   // (uniform over the workgroup)
+    const int ti = tile % tm, tj = tile / tm;
+    const double *A = g.A[z], *B = g.B[z];
+    double *C = g.C[z];
+    const int row0 = ti * GMT, col0 = tj * GMT;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, r = lane & 15, kq = lane >> 4;
+    const int wr = w & 1, wc = w >> 1;
+    double pa[8], pb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = t + 256 * q;
+            const int ka = k0 + (e >> 6), ia = row0 + (e & 63);      // A: 64 rows of column ka
+            pa[q] = (ia < m && ka < k) ? A[ia + (size_t)m * ka] : 0.0;
+            const int kb = k0 + (e & 31), jc = col0 + (e >> 5);      // B: 32 k of column jc
+            pb[q] = (jc < nb && kb < k) ? B[kb + (size_t)k * jc] : 0.0;
+        }
+    };
+    d4_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    int buf = 0;
+    for (int k0 = 0; k0 < k; k0 += GMK, buf ^= 1) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = t + 256 * q;
+            sA[buf][e >> 6][e & 63] = pa[q];
+            sB[buf][e & 31][e >> 5] = pb[q];
+        }
+        __syncthreads();
+        if (k0 + GMK < k) fetch(k0 + GMK);
+#pragma unroll
+        for (int ks = 0; ks < GMK; ks += 4) {
+            double fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) fa[x] = sA[buf][ks + kq][wr * 32 + x * 16 + r];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) fb[y] = sB[buf][ks + kq][wc * 32 + y * 16 + r];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) acc[x][y] = mfma16x16x4(fa[x], fb[y], acc[x][y]);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = row0 + wr * 32 + x * 16 + kq + 4 * v, col = col0 + wc * 32 + y * 16 + r;
+                if (row < m && col < nb) C[row + (size_t)m * col] = alpha * acc[x][y][v];
+            }
+}
+
 // C_z = alpha A_z B_z for nz problems (nz <= kMany), square or not:
 // A_z m x k, B_z k x nb, C_z m x nb, all with their row counts as leading
 // dimensions.
 int gemm_f64_many(myqc_ctx *ctx, int m, int nb, int k, const GemmMany &g, int nz, double alpha) {
     if (m <= 0 || nb <= 0 || k <= 0 || nz <= 0 || nz > kMany)
         return set_error(ctx, MYQC_E_ARG, "gemm_f64_many: m=%d nb=%d k=%d problems %d", m, nb, k, nz);
-    const int tiles_m = (m + 64 * GF_MT - 1) / (64 * GF_MT), gy = (nb + 16 * GF_NT - 1) / (16 * GF_NT);
-    k_gemm_f64_many<<<dim3(tiles_m, gy, nz), GF_THREADS, 0, ctx->stream>>>(m, nb, k, g, alpha, tiles_m);
+    const int tm = (m + GMT - 1) / GMT, tn = (nb + GMT - 1) / GMT;
+    const long long grid = 8LL * ((nz + 7) / 8) * tm * tn;
+    if (grid > 0x7fffffffLL) return set_error(ctx, MYQC_E_ARG, "gemm_f64_many: grid too large");
+    k_gemm_f64_many64<<<(unsigned)grid, 256, 0, ctx->stream>>>(m, nb, k, g, alpha, nz, tm, tn);
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
 }

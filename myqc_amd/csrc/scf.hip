@@ -423,11 +423,17 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
     const bool incr = (precision == MYQC_FP32);
     int since_full = 0;
     bool final_fp64 = false, conv = false;
+    // redo: this iteration rebuilds in FP64 the density the incremental
+    // passes converged on (E32: their energy for it)
+    bool redo = false, redo_next = false;
+    double E32 = 0.0;
     double *fslot = nullptr;
     int it = 1;
     // per-iteration convergence data to stderr when MYQC_SCF_TRACE is set
     const bool trace = getenv("MYQC_SCF_TRACE") != nullptr;
     while (true) {
+        redo = redo_next;
+        redo_next = false;
         const bool full = !incr || final_fp64 || since_full == 0 || since_full >= 8;
         if (full) {
             MYQC_CHECK(digest_G(ctx, b.D, b.G, MYQC_FP64));
@@ -463,10 +469,22 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
                 conv = true;
                 break;
             }
+            // the FP64 rebuild of the density the incremental passes
+            // converged on is accepted at once when the exact Fock matrix
+            // confirms it: its own commutator error below the bound, and its
+            // energy within e_tol of the incremental one
+            if (redo && sqrt(fabs(r[jb0 + head])) < 100 * d_tol && fabs(E - E32) < e_tol) {
+                conv = true;
+                break;
+            }
             if (!final_fp64) {
                 // confirm with FP64 builds, starting with this iteration's
                 final_fp64 = true;
-                if (!full) continue;
+                if (!full) {
+                    E32 = E;
+                    redo_next = true;
+                    continue;
+                }
             }
         }
         if (it > max_iter) break;
@@ -852,6 +870,8 @@ struct ManyState {
     int hist = 0, head = 0;
     double E = 0.0, E_old = 0.0, dE_prev = 1e300, err_prev = 1e300;
     bool prev64 = false, final_fp64 = false, conv = false, done = false;
+    bool redo = false;   // this iteration rebuilds the incrementally converged density in FP64
+    double E32 = 0.0;    // the incremental energy of that density
     int since_full = 0, it = 1;
     double *fslot = nullptr;
     const double *fuse = nullptr;
@@ -1133,8 +1153,15 @@ static int rhf_scf_many(myqc_ctx *const *ctxs, int K, const double *const *H, co
             const double *r = mb.h_red;
             q.E = 0.5 * (r[q.j0] + r[q.j0 + 1]) + q.e_nuc;
             bool stop = false;
+            const bool redo = q.redo;
+            q.redo = false;
             if (q.jdd >= 0 && q.it > 2 && q.dE_prev < e_tol && r[q.jdd] < d_tol && q.err_prev < 100 * d_tol) {
                 if (!incr || (q.prev64 && q.full)) {
+                    q.conv = true;
+                    stop = true;
+                } else if (redo && sqrt(fabs(r[q.jb0 + q.head])) < 100 * d_tol && fabs(q.E - q.E32) < e_tol) {
+                    // the FP64 rebuild confirms the incrementally converged
+                    // density (as in the single-system loop)
                     q.conv = true;
                     stop = true;
                 } else if (!q.final_fp64) {
@@ -1142,6 +1169,8 @@ static int rhf_scf_many(myqc_ctx *const *ctxs, int K, const double *const *H, co
                     if (!q.full) {
                         // confirm with an FP64 build of this same density
                         q.since_full = 0;
+                        q.redo = true;
+                        q.E32 = q.E;
                         go.push_back(-1 - k);
                         continue;
                     }

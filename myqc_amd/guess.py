@@ -5,10 +5,10 @@ ABI (myqc_mlp_*, myqc_batch_pair_data, myqc_batch_rhf_learned; kernels in
 csrc/mlp.hip).  This module holds what surrounds them:
 
   * training data: seeded config-4 molecules (2-8 atoms from H..Ne on random
-    trees) and perturbations of them (every atom displaced uniformly within
-    0.1 A, i.e. bond changes up to +-0.2 A and angle changes of a few
-    degrees), plus seeded clusters of 2-5 waters at the density of the
-    40-water workload, solved by the batched GPU SCF, with the atom-pair
+    trees) and perturbations of them (every bond +-0.1 A, every bond angle
+    +-5 degrees), plus seeded clusters of 2-5 waters at the density of the
+    40-water workload (every atom displaced within 0.1 A), solved by the
+    batched GPU SCF, with the atom-pair
     features, bond-frame density invariants and masks the library extracts;
   * training and saving/loading of the parameters (myqc_amd/data/guess_mlp.npz);
   * a numpy statement of the pair features, invariants and density
@@ -40,14 +40,37 @@ def perturb(rng, xyz_ang, amp=0.1):
     return xyz_ang + d * r
 
 
+def perturb_tree(rng, xyz_ang, parent, d_bond=0.1, d_angle=5.0):
+    """Perturb a molecule built on a tree (parent[k] < k) in its internal
+    coordinates: every bond to a parent changes length by U(-d_bond, d_bond)
+    angstrom and tilts by up to d_angle / 2 degrees about a random axis normal
+    to it, so that every bond angle moves by at most d_angle.  Atoms are
+    rebuilt in order, each from its parent's new position."""
+    xyz = np.asarray(xyz_ang, dtype=np.float64)
+    new = xyz.copy()
+    for k in range(1, len(xyz)):
+        j = parent[k]
+        b = xyz[k] - xyz[j]
+        L = np.linalg.norm(b)
+        u = b / L
+        a = rng.normal(size=3)
+        a -= a.dot(u) * u
+        a /= np.linalg.norm(a)
+        th = np.radians(rng.uniform(0.0, 0.5 * d_angle))
+        u2 = u * np.cos(th) + np.cross(a, u) * np.sin(th)
+        new[k] = new[j] + u2 * (L + rng.uniform(-d_bond, d_bond))
+    return new
+
+
 def molecules(n_base, n_perturb, seed):
-    """n_base config-4 molecules, each with n_perturb perturbed copies (bohr)."""
+    """n_base config-4 molecules, each with n_perturb copies perturbed by
+    +-0.1 A in every bond and +-5 degrees in every bond angle (bohr)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n_base):
-        Z, x = M.random_molecule(rng)
+        Z, x, parent = M.random_molecule_tree(rng)
         for _ in range(n_perturb):
-            out.append((Z, M.to_bohr(Z, perturb(rng, x))))
+            out.append((Z, M.to_bohr(Z, perturb_tree(rng, x, parent))))
     return out
 
 

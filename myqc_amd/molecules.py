@@ -97,10 +97,18 @@ def water_cluster(nw: int, seed: int, box: float = 12.0, rmin: float = 2.6):
 
 def random_molecule(rng: np.random.Generator, max_atoms: int = 8):
     """2..max_atoms atoms from H..Ne on a random tree, made closed-shell."""
+    Z, xyz, _ = random_molecule_tree(rng, max_atoms)
+    return Z, xyz
+
+
+def random_molecule_tree(rng: np.random.Generator, max_atoms: int = 8):
+    """random_molecule with its tree: parent[k] is the atom atom k was bonded
+    to when placed (-1 for the first); the same draws from rng."""
     while True:
         na = int(rng.integers(2, max_atoms + 1))
         Z = [int(rng.integers(1, 11)) for _ in range(na)]
         xyz = [np.zeros(3)]
+        parent = [-1]
         ok = True
         for k in range(1, na):
             placed = False
@@ -111,6 +119,7 @@ def random_molecule(rng: np.random.Generator, max_atoms: int = 8):
                 p = xyz[j] + d * rng.uniform(0.9, 1.6)
                 if all(np.linalg.norm(p - q) >= 0.9 for q in xyz):
                     xyz.append(p)
+                    parent.append(j)
                     placed = True
                     break
             if not placed:
@@ -128,6 +137,7 @@ def random_molecule(rng: np.random.Generator, max_atoms: int = 8):
                     p = xyz[j] + d * rng.uniform(0.9, 1.6)
                     if all(np.linalg.norm(p - q) >= 0.9 for q in xyz):
                         xyz.append(p)
+                        parent.append(j)
                         Z.append(1)
                         placed = True
                         break
@@ -137,7 +147,7 @@ def random_molecule(rng: np.random.Generator, max_atoms: int = 8):
                 Z[-1] = 1 if Z[-1] != 1 else 2
         if sum(Z) % 2 == 1:
             continue
-        return Z, np.array(xyz)
+        return Z, np.array(xyz), parent
 
 
 def nbasis(Z) -> int:

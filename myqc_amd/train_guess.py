@@ -3,7 +3,8 @@
     python -m myqc_amd.train_guess [--base 1000] [--perturb 20] [--epochs 60] [--out PATH]
 
 Data: `base` seeded config-4 molecules (seed 20000) with `perturb` perturbed
-copies each (every atom displaced within 0.1 A): 20k geometries by default,
+copies each (every bond +-0.1 A, every bond angle +-5 degrees): 20k
+geometries by default,
 plus `waters` seeded clusters of 2-5 waters (seed 20002), solved by the
 batched GPU SCF from the core-Hamiltonian guess.  Training:
 Adam on the library's own forward/backward kernels, half the epochs at
