@@ -736,7 +736,7 @@ __global__ void k_zero_int(int *p, int cnt) {
 // host side
 // ---------------------------------------------------------------------------
 struct BatchState {
-    int nmol = 0, nmax = 0, npairs = 0;
+    int nmol = 0, nmax = 0, npairs = 0, natom = 0;
     long long total_eri = 0, total_mat = 0, nq[6] = {0, 0, 0, 0, 0, 0};
     std::vector<BMol> mols;
     std::vector<int> order;
@@ -1085,6 +1085,7 @@ extern "C" int myqc_batch_set(myqc_ctx *ctx, int nmol, const int *mol_nnuc, cons
         MYQC_HIP(ctx, hipGetLastError());
     }
     MYQC_CHECK(upload(ctx, &b.d_atoms, atoms));
+    b.natom = (int)(atoms.size() / 4);
     MYQC_CHECK(upload(ctx, &b.d_ftT, ftT));
     const size_t M = (size_t)std::max<long long>(b.total_mat, 1);
     MYQC_CHECK(dalloc(ctx, (void **)&b.d_eri, sizeof(double) * std::max<long long>(b.total_eri, 1)));
@@ -1173,7 +1174,7 @@ extern "C" int myqc_batch_pair_data(myqc_ctx *ctx, int *npair, float *features, 
         return set_error(ctx, MYQC_E_ARG, "myqc_batch_pair_data: the learned guess covers H..Ne with STO-3G only");
     *npair = b.npair;
     if (!features && !targets && !mask) return 0;
-    MYQC_CHECK(guess_export(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.d_D, features, targets, mask));
+    MYQC_CHECK(guess_export(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.nmol, b.natom, b.d_D, features, targets, mask));
     if (mask) {
         // pairs of molecules that did not converge carry no target
         std::vector<int> conv(b.nmol);
@@ -1193,7 +1194,7 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
     BatchState &b = *ctx->batch;
     if (!b.guess_ok)
         return set_error(ctx, MYQC_E_ARG, "myqc_batch_rhf_learned: the learned guess covers H..Ne with STO-3G only");
-    MYQC_CHECK(guess_reserve(ctx, b.npair));
+    MYQC_CHECK(guess_reserve(ctx, b.npair, b.natom));
     if (!b.tuned) MYQC_CHECK(batch_tune(ctx, b, false, e_tol, d_tol, max_iter));
     // the learned start gets a shorter budget (typical solves take about 25
     // iterations); a molecule it leaves unconverged restarts from the core
@@ -1209,7 +1210,7 @@ extern "C" int myqc_batch_rhf_learned(myqc_ctx *ctx, double e_tol, double d_tol,
                                                                        b.d_pair_mol, b.d_mols, b.npairs, b.d_S, b.d_H);
         for (int c = 0; c < 6; ++c)
             if (b.nq[c] > 0) launch_batch_eri(ctx, b, c, b.eri_threads[c]);
-        guess_launch(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.d_guess);
+        guess_launch(ctx, b.d_pairrec, b.npair, b.d_atoms, b.d_mol_atoms, b.nmol, b.natom, b.d_guess);
         launch_batch_scf(ctx, b, true, e_tol, d_tol, max_iter, b.scf_threads, nullptr, -1, true, learned_iter);
         MYQC_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
         hipError_t e = hipGraphInstantiate(&b.graph_learned, g, nullptr, nullptr, 0);

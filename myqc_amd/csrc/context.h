@@ -279,7 +279,9 @@ struct HostClock {
 int eri_schwarz(myqc_ctx *ctx);
 int eri_store(myqc_ctx *ctx, double tol, bool want32);
 int eri_fill_dense(myqc_ctx *ctx, double tol, double *d_XX);
-int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision);
+// G(D) into d_G (precision FP64), or d_G += G32(D - Dprev) (precision FP32;
+// Dprev may be NULL for D itself)
+int digest_G(myqc_ctx *ctx, const double *d_D, double *d_G, int precision, const double *d_Dprev = nullptr);
 int digest_G_uhf(myqc_ctx *ctx, const double *d_Da, const double *d_Db, double *d_Ga, double *d_Gb);
 int ensure_work(myqc_ctx *ctx);
 void free_eri(myqc_ctx *ctx, bool keep_memory = false);

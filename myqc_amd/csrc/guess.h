@@ -18,13 +18,14 @@ struct PairRec {
 
 // network rows for npair pairs (allocates the activation buffers; call
 // before graph capture)
-int guess_reserve(myqc_ctx *ctx, int npair);
+int guess_reserve(myqc_ctx *ctx, int npair, int natoms);
 // features of the pairs into the network input, forward pass, and the
-// predicted density blocks into D (batch layout); capturable
+// predicted density blocks into D (batch layout); capturable.  d_mol_atoms:
+// first atom and atom count of each of the nmol molecules (natoms in all)
 int guess_launch(myqc_ctx *ctx, const PairRec *d_pairs, int npair, const double *d_atoms, const int *d_mol_atoms,
-                 double *d_D);
+                 int nmol, int natoms, double *d_D);
 // features, targets and masks of the pairs for the densities D, to the host
 int guess_export(myqc_ctx *ctx, const PairRec *d_pairs, int npair, const double *d_atoms, const int *d_mol_atoms,
-                 const double *d_D, float *X, float *T, float *M);
+                 int nmol, int natoms, const double *d_D, float *X, float *T, float *M);
 
 }  // namespace myqc

@@ -343,36 +343,54 @@ __global__ void k_zero(double *__restrict__ p, size_t cnt) {
         p[i] = 0.0;
 }
 
-// the kets' density blocks in Schwarz-rank order: dk[r*(nc*nd) + c*nd + d] =
-// D(ao_a + c, ao_b + d) of pair sorted[r]
+// The kets' density blocks of the three classes in Schwarz-rank order,
+// dk_c[r (nc nd) + c nd + d] = D(ao_a + c, ao_b + d) of pair sorted_c[r], with
+// dk_c at dk + {0, np0, np0 + 3 np1}; and the zeroing of jk (njk values), in
+// one launch
 template <typename TV>
-__global__ void k_pack_kets(const ShellPair *__restrict__ pairs, const int *__restrict__ sorted, int np, int nc, int nd,
-                            const TV *__restrict__ D, int n, TV *__restrict__ dk) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= np) return;
+__global__ void k_pack_kets_all(const ShellPair *__restrict__ pairs, const int *__restrict__ s0,
+                                const int *__restrict__ s1, const int *__restrict__ s2, int np0, int np1, int np2,
+                                const TV *__restrict__ D, int n, TV *__restrict__ dk, double *__restrict__ jk,
+                                size_t njk) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < njk) jk[id] = 0.0;
+    if (id >= (size_t)np0 + np1 + np2) return;
+    int r = (int)id, nc = 1, nd = 1;
+    const int *sorted = s0;
+    TV *out = dk;
+    if (r >= np0 + np1) {
+        r -= np0 + np1;
+        sorted = s2;
+        nc = nd = 3;
+        out = dk + np0 + 3 * (size_t)np1;
+    } else if (r >= np0) {
+        r -= np0;
+        sorted = s1;
+        nc = 3;
+        out = dk + np0;
+    }
     const ShellPair sp = pairs[sorted[r]];
     for (int c = 0; c < nc; ++c)
-        for (int d = 0; d < nd; ++d) dk[(size_t)r * nc * nd + c * nd + d] = D[(sp.ao_a + c) + (size_t)n * (sp.ao_b + d)];
+        for (int d = 0; d < nd; ++d) out[(size_t)r * nc * nd + c * nd + d] = D[(sp.ao_a + c) + (size_t)n * (sp.ao_b + d)];
 }
 
 // --- small elementwise kernels ------------------------------------------------
 
-__global__ void k_add(const double *__restrict__ X, const double *__restrict__ Y, double *__restrict__ Z, size_t cnt) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
-        Z[i] = X[i] + Y[i];
-}
 
-// Ds = scale (D + D^T); with amax, also max |Ds32| into *amax as the bit
-// pattern of a non-negative float (ordered like the value; *amax starts at 0)
+// Ds = scale (E + E^T), E = D - Dm (Dm may be NULL: E = D); with amax, also
+// max |Ds32| into *amax as the bit pattern of a non-negative float (ordered
+// like the value; *amax starts at 0)
 __global__ __launch_bounds__(128) void k_symmetrize(const double *__restrict__ D, double *__restrict__ Ds, float *__restrict__ Ds32, int n,
-                             double scale, unsigned *__restrict__ amax = nullptr) {
+                             double scale, unsigned *__restrict__ amax = nullptr,
+                             const double *__restrict__ Dm = nullptr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     float m = 0.0f;
     if (i < n) {
-        const double v = scale * (D[i + (size_t)n * j] + D[j + (size_t)n * i]);
-        if (Ds) Ds[i + (size_t)n * j] = v;
+        const size_t ij = i + (size_t)n * j, ji = j + (size_t)n * i;
+        const double v = Dm ? scale * ((D[ij] - Dm[ij]) + (D[ji] - Dm[ji])) : scale * (D[ij] + D[ji]);
+        if (Ds) Ds[ij] = v;
         if (Ds32) {
-            Ds32[i + (size_t)n * j] = (float)v;
+            Ds32[ij] = (float)v;
             m = fabsf((float)v);
         }
     }
@@ -396,6 +414,15 @@ __global__ __launch_bounds__(128) void k_symmetrize(const double *__restrict__ D
 
 
 
+// G += A + A^T (the incremental passes); the pass's max |dD| word is
+// cleared for the next pass (the digestion has read it)
+__global__ __launch_bounds__(128) void k_sym_add(const double *__restrict__ A, double *__restrict__ G, int n,
+                                                 unsigned *__restrict__ amax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < n) G[i + (size_t)n * j] += A[i + (size_t)n * j] + A[j + (size_t)n * i];
+    if (i == 0 && j == 0) *amax = 0u;
+}
+
 static dim3 grid2(int n) { return dim3((n + 127) / 128, n); }
 
 int ensure_work(myqc_ctx *ctx) {
@@ -407,6 +434,9 @@ int ensure_work(myqc_ctx *ctx) {
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_D32, sizeof(float) * nn));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_A32, sizeof(float) * nn));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_dmax, sizeof(double)));
+    // the incremental passes' max |dD| word starts at 0 (k_sym_add clears it
+    // after each pass)
+    MYQC_HIP(ctx, hipMemsetAsync(ctx->d_dmax, 0, sizeof(double), ctx->stream));
     ctx->work_n = ctx->n;
     return 0;
 }
@@ -419,14 +449,11 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
     const int np0 = (int)ctx->pair_ids[0].size(), np1 = (int)ctx->pair_ids[1].size(),
               np2 = (int)ctx->pair_ids[2].size();
     TV *dk0 = dk, *dk1 = dk + np0, *dk2 = dk + np0 + 3 * (size_t)np1;
-    const int nps[3] = {np0, np1, np2};
-    TV *dks[3] = {dk0, dk1, dk2};
-    for (int c = 0; c < 3; ++c)
-        if (nps[c] > 0)
-            k_pack_kets<TV><<<(nps[c] + 255) / 256, 256, 0, ctx->stream>>>(ctx->d_pairs, ctx->d_sorted[c], nps[c],
-                                                                            c ? 3 : 1, c == 2 ? 3 : 1, D, n, dks[c]);
-    // the ket-side J gather adds into jk
-    k_zero<<<64, 256, 0, ctx->stream>>>(ctx->d_jk, (size_t)std::max(pl.jk_base[3], 1));
+    // the kets' density blocks, and jk cleared (the ket-side J gather adds into it)
+    const size_t njk = (size_t)std::max(pl.jk_base[3], 1);
+    const size_t npack = std::max(njk, (size_t)np0 + np1 + np2);
+    k_pack_kets_all<TV><<<(unsigned)((npack + 255) / 256), 256, 0, ctx->stream>>>(
+        ctx->d_pairs, ctx->d_sorted[0], ctx->d_sorted[1], ctx->d_sorted[2], np0, np1, np2, D, n, dk, ctx->d_jk, njk);
     if (ctx->n_wg > 0) {
         DigArgs<TV> a;
         a.meta0 = ctx->d_meta[0];
@@ -482,7 +509,7 @@ constexpr double kIncrementalScreen = 1e-12;
 // G(D) into d_Gout (precision FP64), or d_Gout += G32(D) (precision FP32:
 // D is then the density change; partners below the incremental screen drop
 // out)
-int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) {
+int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision, const double *d_Dprev) {
     if (!ctx->have_eri) return set_error(ctx, MYQC_E_STATE, "no stored integrals (call myqc_eri_build)");
     MYQC_CHECK(ensure_work(ctx));
     const int n = ctx->n;
@@ -493,14 +520,15 @@ int digest_G(myqc_ctx *ctx, const double *d_Din, double *d_Gout, int precision) 
         k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, nullptr, n, 1.0);
     } else {
         if (!ctx->have_eri32) return set_error(ctx, MYQC_E_STATE, "no FP32 integrals stored");
-        // max |dD| of the pass for the screen, folded into the symmetrisation
-        MYQC_HIP(ctx, hipMemsetAsync(ctx->d_dmax, 0, sizeof(unsigned), ctx->stream));
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n, 0.5,
-                                                        reinterpret_cast<unsigned *>(ctx->d_dmax));
+        // the density change (Din - Dprev when Dprev is given) symmetrised
+        // into FP32, with the pass's max |dD| for the screen folded in; the
+        // word is 0 here (k_sym_add clears it at the end of each pass)
+        unsigned *amax = reinterpret_cast<unsigned *>(ctx->d_dmax);
+        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(d_Din, nullptr, ctx->d_D32, n, 0.5, amax, d_Dprev);
         MYQC_CHECK(launch_digest<float>(ctx, ctx->d_D32, ctx->d_x32, ctx->d_Dpair32, ctx->d_partial, ctx->d_A, 1.0,
                                         kIncrementalScreen));
-        k_symmetrize<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, ctx->d_G, nullptr, n, 1.0);
-        k_add<<<256, 256, 0, ctx->stream>>>(d_Gout, ctx->d_G, d_Gout, nn);
+        k_sym_add<<<grid2(n), 128, 0, ctx->stream>>>(ctx->d_A, d_Gout, n, amax);
+        (void)nn;
     }
     MYQC_HIP(ctx, hipGetLastError());
     return 0;

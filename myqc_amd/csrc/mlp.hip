@@ -42,11 +42,15 @@ struct MlpState {
     float *d_X = nullptr, *d_T = nullptr, *d_M = nullptr, *d_Z[3] = {nullptr, nullptr, nullptr},
           *d_H[3] = {nullptr, nullptr, nullptr}, *d_Y = nullptr, *d_dA = nullptr, *d_dB = nullptr,
           *d_loss = nullptr;
+    // per-atom environment features (kElem x kEnv each) of up to env_cap atoms
+    float *d_env = nullptr;
+    int env_cap = 0;
     void release() {
         float **ps[] = {&d_par, &d_grad, &d_m, &d_v, &d_X, &d_T, &d_M, &d_Z[0], &d_Z[1], &d_Z[2],
-                        &d_H[0], &d_H[1], &d_H[2], &d_Y, &d_dA, &d_dB, &d_loss};
+                        &d_H[0], &d_H[1], &d_H[2], &d_Y, &d_dA, &d_dB, &d_loss, &d_env};
         for (auto p : ps) dfree((void **)p);
         cap = 0;
+        env_cap = 0;
     }
 };
 
@@ -251,35 +255,64 @@ __device__ __forceinline__ float gauss(float r, float mu, float w) {
 }
 
 // atoms: [Z, x, y, z] per atom (batch table); mol_atoms: [atom0, nnuc] per molecule
-__global__ void k_pair_features(const PairRec *__restrict__ pr, int npair, const double *__restrict__ atoms,
-                                const int *__restrict__ mol_atoms, float *__restrict__ F) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npair) return;
-    const PairRec q = pr[p];
-    float *f = F + (size_t)p * kFeat;
-    for (int c = 0; c < kFeat; ++c) f[c] = 0.f;
-    const double *A = atoms + 4 * q.a, *B = atoms + 4 * q.b;
-    const int za = (int)A[0], zb = (int)B[0];
-    f[za - 1] = 1.f;
-    f[kElem + zb - 1] = 1.f;
-    const double dx = B[1] - A[1], dy = B[2] - A[2], dz = B[3] - A[3];
-    const float r = (float)sqrt(dx * dx + dy * dy + dz * dz);
-    for (int k = 0; k < kRbf; ++k) f[2 * kElem + k] = gauss(r, 0.5f * k, 0.5f);
-    f[2 * kElem + kRbf] = q.a == q.b ? 1.f : 0.f;
-    const int a0 = mol_atoms[2 * q.mol], na = mol_atoms[2 * q.mol + 1];
-    for (int side = 0; side < 2; ++side) {
-        const double *X = side ? B : A;
-        const int self = side ? q.b : q.a;
-        float *env = f + 2 * kElem + kRbf + 1 + side * kElem * kEnv;
-        for (int c = a0; c < a0 + na; ++c) {
-            if (c == self) continue;
-            const double *C = atoms + 4 * c;
-            const double ex = C[1] - X[1], ey = C[2] - X[2], ez = C[3] - X[3];
-            const float rc = (float)sqrt(ex * ex + ey * ey + ez * ez);
-            const int zc = (int)C[0];
-            for (int k = 0; k < kEnv; ++k) env[(zc - 1) * kEnv + k] += gauss(rc, 1.0f + k, 0.7f);
-        }
+// Environment features of every atom: env[a][z][k] = sum over the other
+// atoms c of its molecule with Z_c = z + 1 of gauss(|R_c - R_a|, 1 + k, 0.7),
+// c in input order.  A thread per (atom, z, k); mol_atoms (first atom and
+// count of each of the nmol molecules, in atom order) gives the atom's
+// molecule by bisection.
+__global__ void k_atom_env(const double *__restrict__ atoms, const int *__restrict__ mol_atoms, int nmol, int natoms,
+                           float *__restrict__ env) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)natoms * kElem * kEnv) return;
+    const int a = (int)(id / (kElem * kEnv)), slot = (int)(id - (long long)a * kElem * kEnv);
+    const int z = slot / kEnv, k = slot - z * kEnv;
+    int lo = 0, hi = nmol - 1;
+    while (lo < hi) {   // the last molecule whose first atom is <= a
+        const int mid = (lo + hi + 1) >> 1;
+        if (mol_atoms[2 * mid] <= a) lo = mid;
+        else hi = mid - 1;
     }
+    const int a0 = mol_atoms[2 * lo], na = mol_atoms[2 * lo + 1];
+    const double *X = atoms + 4 * a;
+    float e = 0.f;
+    for (int c = a0; c < a0 + na; ++c) {
+        const double *C = atoms + 4 * c;
+        if (c == a || (int)C[0] != z + 1) continue;
+        const double ex = C[1] - X[1], ey = C[2] - X[2], ez = C[3] - X[3];
+        e += gauss((float)sqrt(ex * ex + ey * ey + ez * ez), 1.0f + k, 0.7f);
+    }
+    env[id] = e;
+}
+
+// The network input of each pair (A, B): one-hot Z_A and Z_B, 16 Gaussians of
+// R_AB, the on-site flag, the environments of A and B (k_atom_env), zeros up
+// to kFeat.  A thread per (pair, feature), so that the rows are written
+// coalesced.
+__global__ void k_pair_features(const PairRec *__restrict__ pr, int npair, const double *__restrict__ atoms,
+                                const float *__restrict__ env, float *__restrict__ F) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)npair * kFeat) return;
+    const int p = (int)(id / kFeat), c = (int)(id - (long long)p * kFeat);
+    const PairRec q = pr[p];
+    const double *A = atoms + 4 * q.a, *B = atoms + 4 * q.b;
+    float v;
+    if (c < kElem) {
+        v = c == (int)A[0] - 1 ? 1.f : 0.f;
+    } else if (c < 2 * kElem) {
+        v = c - kElem == (int)B[0] - 1 ? 1.f : 0.f;
+    } else if (c < 2 * kElem + kRbf) {
+        const double dx = B[1] - A[1], dy = B[2] - A[2], dz = B[3] - A[3];
+        const float r = (float)sqrt(dx * dx + dy * dy + dz * dz);
+        v = gauss(r, 0.5f * (c - 2 * kElem), 0.5f);
+    } else if (c == 2 * kElem + kRbf) {
+        v = q.a == q.b ? 1.f : 0.f;
+    } else if (c < 2 * kElem + kRbf + 1 + 2 * kElem * kEnv) {
+        const int e = c - (2 * kElem + kRbf + 1), side = e / (kElem * kEnv);
+        v = env[(size_t)(side ? q.b : q.a) * kElem * kEnv + (e - side * kElem * kEnv)];
+    } else {
+        v = 0.f;   // the input row's padding up to kFeat
+    }
+    F[id] = v;
 }
 
 __device__ __forceinline__ void bond_axis(const double *A, const double *B, double u[3]) {
@@ -382,33 +415,48 @@ __global__ void k_pair_reconstruct(const PairRec *__restrict__ pr, int npair, co
             for (int z = 0; z < 3; ++z) put(2 + x, 2 + z, (x == z ? y[9] : 0.0) + (y[8] - y[9]) * u[x] * u[z]);
 }
 
-int guess_reserve(myqc_ctx *ctx, int npair) {
+int guess_reserve(myqc_ctx *ctx, int npair, int natoms) {
     if (!ctx->mlp) return set_error(ctx, MYQC_E_STATE, "learned guess: no network (myqc_mlp_create)");
     MlpState &s = *ctx->mlp;
     if (s.n_in != kFeat || s.n_out != kOut)
         return set_error(ctx, MYQC_E_ARG, "learned guess: the network must map %d features to %d outputs", kFeat, kOut);
+    if (natoms > s.env_cap) {
+        ++ctx->mlp_gen;   // captured graphs that use the buffer are stale now
+        MYQC_CHECK(dalloc(ctx, (void **)&s.d_env, sizeof(float) * kElem * kEnv * (size_t)natoms));
+        s.env_cap = natoms;
+    }
     return mlp_ensure(ctx, s, npair);
 }
 
+// the environments of all atoms, then the pairs' features
+static void pair_features(myqc_ctx *ctx, const PairRec *d_pairs, int npair, const double *d_atoms,
+                          const int *d_mol_atoms, int nmol, int natoms, float *env, float *X) {
+    const long long ne = (long long)natoms * kElem * kEnv;
+    k_atom_env<<<(unsigned)((ne + 255) / 256), 256, 0, ctx->stream>>>(d_atoms, d_mol_atoms, nmol, natoms, env);
+    const long long cnt = (long long)npair * kFeat;
+    k_pair_features<<<(unsigned)((cnt + 255) / 256), 256, 0, ctx->stream>>>(d_pairs, npair, d_atoms, env, X);
+}
+
 int guess_launch(myqc_ctx *ctx, const PairRec *d_pairs, int npair, const double *d_atoms, const int *d_mol_atoms,
-                 double *d_D) {
+                 int nmol, int natoms, double *d_D) {
     MlpState &s = *ctx->mlp;
     const int g = (npair + 127) / 128;
-    k_pair_features<<<g, 128, 0, ctx->stream>>>(d_pairs, npair, d_atoms, d_mol_atoms, s.d_X);
+    pair_features(ctx, d_pairs, npair, d_atoms, d_mol_atoms, nmol, natoms, s.d_env, s.d_X);
     mlp_forward(ctx, s, npair);
     k_pair_reconstruct<<<g, 128, 0, ctx->stream>>>(d_pairs, npair, d_atoms, s.d_Y, d_D);
     return 0;
 }
 
 int guess_export(myqc_ctx *ctx, const PairRec *d_pairs, int npair, const double *d_atoms, const int *d_mol_atoms,
-                 const double *d_D, float *X, float *T, float *M) {
-    float *dX = nullptr, *dT = nullptr, *dM = nullptr;
+                 int nmol, int natoms, const double *d_D, float *X, float *T, float *M) {
+    float *dX = nullptr, *dT = nullptr, *dM = nullptr, *dE = nullptr;
     int rc = dalloc(ctx, (void **)&dX, sizeof(float) * (size_t)npair * kFeat);
+    if (!rc) rc = dalloc(ctx, (void **)&dE, sizeof(float) * kElem * kEnv * (size_t)std::max(natoms, 1));
     if (!rc) rc = dalloc(ctx, (void **)&dT, sizeof(float) * (size_t)npair * kOut);
     if (!rc) rc = dalloc(ctx, (void **)&dM, sizeof(float) * (size_t)npair * kOut);
     if (!rc) {
         const int g = (npair + 127) / 128;
-        k_pair_features<<<g, 128, 0, ctx->stream>>>(d_pairs, npair, d_atoms, d_mol_atoms, dX);
+        pair_features(ctx, d_pairs, npair, d_atoms, d_mol_atoms, nmol, natoms, dE, dX);
         k_pair_targets<<<g, 128, 0, ctx->stream>>>(d_pairs, npair, d_atoms, d_D, dT, dM);
         if (hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "pair data: launch failed");
     }
@@ -419,6 +467,7 @@ int guess_export(myqc_ctx *ctx, const PairRec *d_pairs, int npair, const double 
     dfree((void **)&dX);
     dfree((void **)&dT);
     dfree((void **)&dM);
+    dfree((void **)&dE);
     return rc;
 }
 
@@ -628,7 +677,7 @@ extern "C" int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const d
         for (int d = 0; d < 3; ++d) atoms[4 * a + 1 + d] = xyz_bohr[a + (size_t)nnuc * d];
     }
     const int npair = (int)prs.size(), mol_atoms[2] = {0, nnuc};
-    MYQC_CHECK(guess_reserve(ctx, npair));
+    MYQC_CHECK(guess_reserve(ctx, npair, nnuc));
     PairRec *dP = nullptr;
     double *dA = nullptr, *dD = nullptr;
     int *dM = nullptr;
@@ -640,7 +689,7 @@ extern "C" int myqc_guess_density(myqc_ctx *ctx, int nnuc, const int *Z, const d
                 hipMemcpyAsync(dA, atoms.data(), sizeof(double) * atoms.size(), hipMemcpyHostToDevice, ctx->stream) ||
                 hipMemcpyAsync(dM, mol_atoms, sizeof(int) * 2, hipMemcpyHostToDevice, ctx->stream)))
         rc = set_error(ctx, MYQC_E_HIP, "myqc_guess_density: upload failed");
-    if (!rc) rc = guess_launch(ctx, dP, npair, dA, dM, dD);
+    if (!rc) rc = guess_launch(ctx, dP, npair, dA, dM, 1, nnuc, dD);
     if (!rc && hipGetLastError() != hipSuccess) rc = set_error(ctx, MYQC_E_HIP, "myqc_guess_density: launch failed");
     if (!rc && d2h(ctx, D, dD, sizeof(double) * (size_t)n * n) != hipSuccess)
         rc = set_error(ctx, MYQC_E_HIP, "myqc_guess_density: download failed");

@@ -129,14 +129,14 @@ __global__ __launch_bounds__(64) void k_reduce_final(RedJobs J, const double *__
 struct ScfBuffers {
     int n = -1;
     double *H = nullptr, *S = nullptr, *X = nullptr, *Xi = nullptr, *F = nullptr, *Fx = nullptr, *D = nullptr,
-           *Dn = nullptr, *Do = nullptr, *Don = nullptr, *Dref = nullptr, *G = nullptr, *C = nullptr, *T1 = nullptr,
+           *Dn = nullptr, *Do = nullptr, *Don = nullptr, *G = nullptr, *C = nullptr, *T1 = nullptr,
            *T2 = nullptr, *P2 = nullptr, *eig = nullptr, *work = nullptr, *dsc = nullptr, *lohi = nullptr,
            *part = nullptr, *red = nullptr, *sp2t = nullptr;
     double *hF = nullptr, *hE = nullptr;   // DIIS history, kDiis x n^2 each
     int *info = nullptr;
     static constexpr int kDiis = 8;
     void release() {
-        double **ps[] = {&H,  &S,  &X,   &Xi,  &F,   &Fx,   &D,    &Dn,   &Do,   &Don, &Dref, &G, &C,
+        double **ps[] = {&H,  &S,  &X,   &Xi,  &F,   &Fx,   &D,    &Dn,   &Do,   &Don, &G, &C,
                          &T1, &T2, &P2, &eig, &work, &dsc, &lohi, &part, &red, &hF,  &hE, &sp2t};
         for (auto p : ps) dfree((void **)p);
         dfree((void **)&info);
@@ -146,7 +146,7 @@ struct ScfBuffers {
         if (n == nb) return 0;
         release();
         const size_t nn = (size_t)nb * nb;
-        double **mats[] = {&H, &S, &X, &Xi, &F, &Fx, &D, &Dn, &Do, &Don, &Dref, &G, &C, &T1, &T2, &P2};
+        double **mats[] = {&H, &S, &X, &Xi, &F, &Fx, &D, &Dn, &Do, &Don, &G, &C, &T1, &T2, &P2};
         for (auto m : mats) MYQC_CHECK(dalloc(ctx, (void **)m, sizeof(double) * nn));
         MYQC_CHECK(dalloc(ctx, (void **)&eig, sizeof(double) * nb));
         MYQC_CHECK(dalloc(ctx, (void **)&work, sizeof(double) * nb));
@@ -439,11 +439,10 @@ static int rhf_scf(myqc_ctx *ctx, const double *H, const double *S, int nocc, do
             MYQC_CHECK(digest_G(ctx, b.D, b.G, MYQC_FP64));
             since_full = 1;
         } else {
-            k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(b.D, b.Dref, b.T1, 1.0, -1.0, 0.0, n);
-            MYQC_CHECK(digest_G(ctx, b.T1, b.G, MYQC_FP32));   // G += G32(dD)
+            // G += G32(D - Dn): Dn holds the density the previous pass digested
+            MYQC_CHECK(digest_G(ctx, b.D, b.G, MYQC_FP32, b.Dn));
             ++since_full;
         }
-        if (incr) MYQC_HIP(ctx, hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
         k_axpbyc<<<grid2(n), 128, 0, ctx->stream>>>(b.H, b.G, b.F, 1.0, 1.0, 0.0, n);
         // orthonormal basis: Fo = X^T F X, DIIS error Fo Do - Do Fo
         fslot = b.hF + nn * head;
@@ -1104,12 +1103,9 @@ static int rhf_scf_many(myqc_ctx *const *ctxs, int K, const double *const *H, co
                 rc = digest_G(q.ctx, b.D, b.G, MYQC_FP64);
                 q.since_full = 1;
             } else {
-                k_axpbyc<<<g2, 128, 0, fs>>>(b.D, b.Dref, b.T1, 1.0, -1.0, 0.0, n);
-                rc = digest_G(q.ctx, b.T1, b.G, MYQC_FP32);
+                rc = digest_G(q.ctx, b.D, b.G, MYQC_FP32, b.Dn);   // Dn: the density the previous pass digested
                 ++q.since_full;
             }
-            if (rc == 0 && incr && hipMemcpyAsync(b.Dref, b.D, sizeof(double) * nn, hipMemcpyDeviceToDevice, fs) != hipSuccess)
-                rc = set_error(q.ctx, MYQC_E_HIP, "copy of the reference density failed");
             q.ctx->stream = lead->stream;
             if (rc) return set_error(lead, MYQC_E_HIP, "%s", q.ctx->err.c_str());
             q.fslot = b.hF + nn * q.head;
