@@ -58,8 +58,8 @@ __global__ __launch_bounds__(128) void k_batch_one_e(const ShellPair *__restrict
                                                      const double *__restrict__ atoms, const int *__restrict__ pair_mol,
                                                      const BMol *__restrict__ mols, int npairs, double *__restrict__ S,
                                                      double *__restrict__ H) {
-    __shared__ double s_ft[FT_NT * FT_NM];
-    for (int i = threadIdx.x; i < FT_NT * FT_NM; i += blockDim.x) s_ft[i] = ftT[i];
+    __shared__ double s_ft[FT_TOTAL];
+    for (int i = threadIdx.x; i < FT_TOTAL; i += blockDim.x) s_ft[i] = ftT[i];
     __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npairs) return;
@@ -99,8 +99,8 @@ struct BEriArgs {
 
 template <int CB, int CK>
 __global__ __launch_bounds__(256) void k_batch_eri(BEriArgs a) {
-    __shared__ double s_ft[FT_NT * FT_NM];
-    for (int i = threadIdx.x; i < FT_NT * FT_NM; i += blockDim.x) s_ft[i] = a.ftT[i];
+    __shared__ double s_ft[FT_TOTAL];
+    for (int i = threadIdx.x; i < FT_TOTAL; i += blockDim.x) s_ft[i] = a.ftT[i];
     __syncthreads();
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nq) return;
@@ -1063,9 +1063,7 @@ extern "C" int myqc_batch_set(myqc_ctx *ctx, int nmol, const int *mol_nnuc, cons
         b.nq[c] = qc[nmol];
         MYQC_CHECK(upload(ctx, &b.d_qcum[c], qc));
     }
-    std::vector<double> ftT(FT_NT * FT_NM);
-    for (int tk = 0; tk < FT_NT; ++tk)
-        for (int mm = 0; mm < FT_NM; ++mm) ftT[tk * FT_NM + mm] = Ftab[tk + FT_NT * mm];
+    const std::vector<double> ftT = build_ftT(Ftab);
     MYQC_CHECK(upload(ctx, &b.d_mols, b.mols));
     MYQC_CHECK(upload(ctx, &b.d_order, b.order));
     MYQC_CHECK(upload(ctx, &b.d_pair_mol, pair_mol));

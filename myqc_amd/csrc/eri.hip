@@ -162,7 +162,7 @@ __device__ __forceinline__ void find_quartet(const long long *__restrict__ cum, 
 }
 
 __device__ __forceinline__ void load_ft(const double *__restrict__ ftT, double *s_ft) {
-    for (int i = threadIdx.x; i < FT_NT * FT_NM; i += blockDim.x) s_ft[i] = ftT[i];
+    for (int i = threadIdx.x; i < FT_TOTAL; i += blockDim.x) s_ft[i] = ftT[i];
     __syncthreads();
 }
 
@@ -171,7 +171,7 @@ __device__ __forceinline__ void load_ft(const double *__restrict__ ftT, double *
 // sums (group_sum), and each stores every SPLIT-th element of the block.
 template <int CB, int CK, int MODE, int SPLIT>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
-    __shared__ double s_ft[FT_NT * FT_NM];
+    __shared__ double s_ft[FT_TOTAL];
     load_ft(a.ftT, s_ft);
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long t = tid / SPLIT;
@@ -269,7 +269,7 @@ template <int C>
 __global__ __launch_bounds__(256) void k_schwarz(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
                                                  const double *__restrict__ ftT, const int *__restrict__ ids, int np,
                                                  double *__restrict__ q) {
-    __shared__ double s_ft[FT_NT * FT_NM];
+    __shared__ double s_ft[FT_TOTAL];
     load_ft(ftT, s_ft);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= np) return;
@@ -787,9 +787,7 @@ extern "C" int myqc_set_system(myqc_ctx *ctx, int nnuc, const int *Z, const doub
     if (!why.empty()) return set_error(ctx, MYQC_E_ARG, "myqc_set_system: %s", why.c_str());
     hc.mark("set_system: shells and pairs");
     // Boys table transposed to [Tk][m] so the 7 Taylor rows are contiguous
-    ctx->ftT.assign(FT_NT * FT_NM, 0.0);
-    for (int tk = 0; tk < FT_NT; ++tk)
-        for (int m = 0; m < FT_NM; ++m) ctx->ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
+    ctx->ftT = build_ftT(Ftab);
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_pairs, sizeof(ShellPair) * ctx->pairs.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_prims, sizeof(PrimPair) * ctx->prims.size()));
     MYQC_CHECK(dalloc(ctx, (void **)&ctx->d_ftT, sizeof(double) * ctx->ftT.size()));

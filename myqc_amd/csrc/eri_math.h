@@ -22,6 +22,9 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <cmath>
+#include <vector>
+
 #define MYQC_HD __host__ __device__ __forceinline__
 
 namespace myqc {
@@ -31,6 +34,12 @@ constexpr double PI_SP = (double)3.1415926535897931f;
 
 constexpr int FT_NT = 121;   // Tk = 0..120  (T = 0 .. 12 step 0.1)
 constexpr int FT_NM = 23;    // m  = 0..22
+// The staged table also holds e^{-k/10}, k = 0..600, after the Boys values
+// (FT_E0 on): e^{-T} for T < 60 (the largest T at which it is used, 2Q + 36
+// with Q <= 12) from one entry and a short series
+constexpr int FT_NE = 601;
+constexpr int FT_E0 = FT_NT * FT_NM;
+constexpr int FT_TOTAL = FT_E0 + FT_NE;
 
 // Cartesian components up to l = 2: 0 s | 1 x 2 y 3 z | 4 xx 5 xy 6 xz 7 yy 8 yz 9 zz
 MYQC_HD constexpr int cexp(int idx, int d) {
@@ -83,21 +92,47 @@ MYQC_HD double recip_hd(double x) {
 #endif
 }
 
+// The staged table ftT of (Boys table transposed, ftT[Tk*23 + m]; then
+// e^{-k/10} at FT_E0 + k), built from the Ftab record (Ft(Tk, m) at
+// Tk + 121 m).  Host only.
+inline std::vector<double> build_ftT(const double *Ftab) {
+    std::vector<double> ftT(FT_TOTAL);
+    for (int tk = 0; tk < FT_NT; ++tk)
+        for (int m = 0; m < FT_NM; ++m) ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
+    for (int k = 0; k < FT_NE; ++k) ftT[FT_E0 + k] = std::exp(-0.1 * k);
+    return ftT;
+}
+
 // Boys F_0..F_M(T) as the reference computes them at order Q (M <= Q <= 16).
-// ft is the table transposed to ftT[Tk*23 + m].
+// ftT: the staged table (build_ftT).
 template <int M>
 MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&F)[M + 1]) {
+    // Tk = NINT(10 T) (T >= 0; T clamped to 60, beyond every table use) and
+    // x = Tk / 10 - T, |x| <= 0.05.  Tk / 10 without a division: the product
+    // and one fma correction equal the correctly rounded quotient for every
+    // integer 0..1000 (checked exhaustively)
+    const double Tc = T < 60.0 ? T : 60.0;
+    const int Tk = (int)floor(Tc * 10.0 + 0.5);
+    const double tkd = (double)Tk, q10 = tkd * 0.1;
+    const double x = fma(fma(-q10, 10.0, tkd), 0.1, q10) - Tc;
     // e^{-T} feeds the downward recursion below T = 12 (when Q > 0) and the
     // asymptotic form up to 2Q + 36; one evaluation serves both branches (the
-    // lanes of a wavefront usually fall on both sides of T = 12)
-    const double e = (Q > 0 || T >= 12.0) && T < 2.0 * Q + 36.0 ? exp(-T) : 0.0;
+    // lanes of a wavefront usually fall on both sides of T = 12):
+    // e^{-T} = e^{-Tk/10} e^x with the Taylor series of e^x to x^7 (the next
+    // term is below 1e-15 relative)
+    double e = 0.0;
+    if ((Q > 0 || T >= 12.0) && T < 2.0 * Q + 36.0) {
+        double s = 1.0 / 5040.0;
+        s = fma(s, x, 1.0 / 720.0);
+        s = fma(s, x, 1.0 / 120.0);
+        s = fma(s, x, 1.0 / 24.0);
+        s = fma(s, x, 1.0 / 6.0);
+        s = fma(s, x, 0.5);
+        s = fma(s, x, 1.0);
+        s = fma(s, x, 1.0);
+        e = ftT[FT_E0 + Tk] * s;
+    }
     if (T < 12.0) {
-        int Tk = (int)floor(T * 10.0 + 0.5);          // NINT, T >= 0
-        // Tk / 10 without a division: the product and one fma correction
-        // equal the correctly rounded quotient for every integer 0..1000
-        // (checked exhaustively; Tk <= 120 here)
-        const double tkd = (double)Tk, q10 = tkd * 0.1;
-        double x = fma(fma(-q10, 10.0, tkd), 0.1, q10) - T;
         const double *r = ftT + Tk * FT_NM + Q;
         // sum_{k=0..6} Ft(Tk,Q+k) x^k / k!   (auxilary.f90:152-155)
         double xk = 1.0, fq = 0.0;

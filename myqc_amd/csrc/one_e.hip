@@ -13,8 +13,8 @@ namespace myqc {
 __global__ __launch_bounds__(256) void k_one_e(const ShellPair *__restrict__ pairs, const PrimPair *__restrict__ prims,
                                                const double *__restrict__ ftT, const double *__restrict__ atoms,
                                                int nnuc, int npairs, int n, double *__restrict__ S, double *__restrict__ H) {
-    __shared__ double s_ft[FT_NT * FT_NM];
-    for (int i = threadIdx.x; i < FT_NT * FT_NM; i += blockDim.x) s_ft[i] = ftT[i];
+    __shared__ double s_ft[FT_TOTAL];
+    for (int i = threadIdx.x; i < FT_TOTAL; i += blockDim.x) s_ft[i] = ftT[i];
     __syncthreads();
     // one wavefront per shell pair: lane l takes the nuclei l, l + 64, ... of
     // the attraction integrals and the primitive pairs l, l + 64, ... of S and

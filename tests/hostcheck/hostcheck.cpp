@@ -28,9 +28,7 @@ std::string make(Sys &s, int nnuc, const double *xyzF, int nset, const double *a
     for (int a = 0; a < nnuc; ++a)
         for (int c = 0; c < 3; ++c) s.xyz[3 * a + c] = xyzF[a + nnuc * c];
     s.n = norb;
-    s.ftT.assign(FT_NT * FT_NM, 0.0);
-    for (int tk = 0; tk < FT_NT; ++tk)
-        for (int m = 0; m < FT_NM; ++m) s.ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
+    s.ftT = build_ftT(Ftab);
     return build_system(nnuc, s.xyz.data(), nset, alpha, setinfo, bas, norb, basinfo, s.shells, s.pairs, s.prims, s.ids);
 }
 
@@ -143,9 +141,7 @@ extern "C" int hc_int1e(int nnuc, const int *Z, const double *xyz, int nset, con
 }
 
 extern "C" int hc_boys(int M, int Q, double T, const double *Ftab, double *F) {
-    std::vector<double> ftT(FT_NT * FT_NM);
-    for (int tk = 0; tk < FT_NT; ++tk)
-        for (int m = 0; m < FT_NM; ++m) ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
+    const std::vector<double> ftT = build_ftT(Ftab);
     switch (M) {
         case 0: { double f[1]; boys_ref<0>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
         case 1: { double f[2]; boys_ref<1>(Q, T, ftT.data(), f); memcpy(F, f, sizeof f); break; }
