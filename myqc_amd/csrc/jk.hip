@@ -69,78 +69,183 @@ __device__ __forceinline__ cptr<T> as_const(const T *p) {
     return (cptr<T>)p;
 }
 
-// Bra row a = ia of one unit (A, B) over the lanes' kets of one column chunk.
-// s_acc / s_d: the workgroup's LDS rows of G and D (A's rows first, B's from
-// row sb); act[j]: the lane's ket j is in the unit's range (and passes the
-// screen), ri[j]: its rank (an inactive lane reads the unit's first ket, in
-// range, and drops it); dA[j]: D(a, c), D(a, d) at the ket's columns; jk /
-// ka: the lane's ket-side J sums and its row-a K sums, carried over the
-// chunk's units (every unit of a workgroup shares row shell A); xu + r + e
-// len: element e of the unit's block of ket rank r.
-template <int LB, int CK, typename TV, int KPL = dig_kpl(CK), int NCD = dig_ncd(CK),
+// One row of the walk over a column chunk for bra AO a = ia: row ib of
+// unit u's block (B's function ib) over the lanes' kets.  off: value offset
+// of its element 0 for rank 0 (element e of ket rank r at off + e len + r);
+// bit j of act: the lane's ket j is in the unit's range and passes the
+// screen, ri[j]: its rank (an inactive lane reads the unit's first ket, in range,
+// and drops it).
+template <int KPL>
+struct DigRow {
+    long long off;
+    int u, len, nb, ib, aoB, sb;
+    int ri[KPL];
+    unsigned act;
+};
+
+// The first unit at or after u (of u1) whose rank range meets the chunk
+// [r0, r0 + nk) and that the density screen keeps, as its row 0 for bra AO
+// ia; false when there is none.  Every value but act / ri is wave-uniform;
+// qv[j]: the Schwarz factor of the lane's ket j.  The screen's wave-uniform
+// reads are scalar loads, so that no wait for them also waits for the
+// stored values in flight.
+template <int CK, typename TV, int KPL>
+__device__ __forceinline__ bool dig_find(const DigArgs<TV> &a, int u, int u1, int r0, int nk, int ia, double dmax,
+                                         bool screen, const float *qk, const int (&rk)[KPL], const float (&qv)[KPL],
+                                         const bool (&valid)[KPL], DigRow<KPL> &row) {
+    constexpr int NCD = dig_ncd(CK);
+#pragma unroll 1
+    for (; u < u1; ++u) {
+        const cptr<int> ur = as_const(a.unit) + (size_t)kUnitInts * u;
+        const int lo = ur[3 + 2 * CK], hi = ur[4 + 2 * CK];
+        if (hi <= r0 || lo >= r0 + nk) continue;
+        double thr = 0.0;
+        if (screen) {
+            // |x| <= q_unit q_ket, kets in descending order: the unit drops
+            // out whole when its first ket in the chunk is below the screen,
+            // runs whole when its last one is above, else tests ket by ket
+            const double qu = (double)__int_as_float(ur[9]) * dmax;
+            const int rf = max(lo, r0), rl = min(hi, r0 + nk) - 1;
+            if ((double)as_const(qk)[rf] * qu < a.qtau) continue;
+            if ((double)as_const(qk)[rl] * qu < a.qtau) thr = a.qtau / qu;
+        }
+        row.act = 0u;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            const bool on = valid[j] && rk[j] >= lo && rk[j] < hi && (thr <= 0.0 || (double)qv[j] >= thr);
+            row.act |= on ? 1u << j : 0u;
+            row.ri[j] = on ? rk[j] : lo;
+        }
+        row.u = u;
+        row.len = hi - lo;
+        row.nb = (ur[1] >> 15) ? 3 : 1;
+        row.ib = 0;
+        row.aoB = ur[1] & 0x7fff;
+        row.sb = ur[2];
+        row.off = as_const(a.unit_off)[3 * (size_t)u + CK] + (long long)ia * row.nb * NCD * row.len;
+        return true;
+    }
+    return false;
+}
+
+// The row after `cur`: the unit's next function, else the next unit's row 0.
+template <int CK, typename TV, int KPL>
+__device__ __forceinline__ bool dig_next(const DigArgs<TV> &a, const DigRow<KPL> &cur, int u1, int r0, int nk, int ia,
+                                         double dmax, bool screen, const float *qk, const int (&rk)[KPL],
+                                         const float (&qv)[KPL], const bool (&valid)[KPL], DigRow<KPL> &nxt) {
+    if (cur.ib + 1 < cur.nb) {
+        nxt = cur;
+        nxt.ib = cur.ib + 1;
+        nxt.off = cur.off + (long long)dig_ncd(CK) * cur.len;
+        return true;
+    }
+    return dig_find<CK>(a, cur.u + 1, u1, r0, nk, ia, dmax, screen, qk, rk, qv, valid, nxt);
+}
+
+template <int KPL, int NCD, typename TV>
+__device__ __forceinline__ void dig_load(const TV *x, const DigRow<KPL> &row, TV (&v)[KPL][NCD]) {
+#pragma unroll
+    for (int e = 0; e < NCD; ++e)
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) v[j][e] = x[row.off + (long long)e * row.len + row.ri[j]];
+}
+
+// The same loads from row `nx` when `more`, else again from row `cu`.
+// Issued on every path, so that the wait counts the compiler places before
+// the current row's arithmetic are the same on all paths and never cover
+// these loads.
+template <int KPL, int NCD, typename TV>
+__device__ __forceinline__ void dig_load_sel(const TV *x, bool more, const DigRow<KPL> &nx, const DigRow<KPL> &cu,
+                                             TV (&v)[KPL][NCD]) {
+    const long long off = more ? nx.off : cu.off;
+    const int len = more ? nx.len : cu.len;
+#pragma unroll
+    for (int e = 0; e < NCD; ++e)
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) v[j][e] = x[off + (long long)e * len + (more ? nx.ri[j] : cu.ri[j])];
+}
+
+// The arithmetic of one row (bra AO a = ia, B's function ib) on its loaded
+// values v.  s_acc / s_d: the workgroup's LDS rows of G and D (A's rows
+// first, B's from row sb); dA[j]: D(a, c), D(a, d) at the lane's ket j's
+// columns; jk / ka: the lane's ket-side J sums and its row-a K sums,
+// carried over the chunk's units (every unit of a workgroup shares row
+// shell A).
+template <int CK, typename TV, int KPL = dig_kpl(CK), int NCD = dig_ncd(CK),
           int NK = ncart(PL<CK>::a) + ncart(PL<CK>::b)>
 __device__ __forceinline__ void digest_row(const DigArgs<TV> &a, double *s_acc, const TV *s_d, int ia,
-                                           const int (&ri)[KPL], const bool (&act)[KPL], const int (&aoC)[KPL],
+                                           const DigRow<KPL> &row, const TV (&v)[KPL][NCD], const int (&aoC)[KPL],
                                            const int (&aoD)[KPL], const TV (&dq)[KPL][NCD], const TV (&dA)[KPL][NK],
-                                           double (&jk)[KPL][NCD], TV (&ka)[KPL][NK], const TV *xu, int len, int aoB,
-                                           int sb) {
-    constexpr int nb = ncart(LB), nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
+                                           double (&jk)[KPL][NCD], TV (&ka)[KPL][NK]) {
+    constexpr int nc = ncart(PL<CK>::a), nd = ncart(PL<CK>::b);
     static_assert(nc * nd == NCD && nc + nd == NK, "ket block");
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-#pragma unroll 1
-    for (int ib = 0; ib < nb; ++ib) {
-        TV x[KPL][NCD];
+    const int colB = row.aoB + row.ib, rowB = row.sb + row.ib;
+    TV x[KPL][NCD];
 #pragma unroll
-        for (int e = 0; e < NCD; ++e) {
-            const TV *xe = xu + (long long)((ia * nb + ib) * NCD + e) * len;
+    for (int j = 0; j < KPL; ++j)
 #pragma unroll
-            for (int j = 0; j < KPL; ++j) {
-                const TV val = xe[ri[j]];
-                x[j][e] = act[j] ? val : (TV)0;
+        for (int e = 0; e < NCD; ++e) x[j][e] = (row.act >> j) & 1u ? v[j][e] : (TV)0;
+    // J, bra side: A(a, b) += 2 sum_cd D(c, d) x
+    TV t = 0;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j)
+#pragma unroll
+        for (int e = 0; e < NCD; ++e) t += dq[j][e] * x[j][e];
+    t = wave_total(t);
+    if (lane == 0 && a.jscale != 0.0) atomicAdd(&s_acc[ia * n + colB], 2.0 * a.jscale * (double)t);
+    const TV dab = s_d[ia * n + colB];   // D(a, b), wave-uniform
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        // J, ket side: A(c, d) += 2 D(a, b) x
+#pragma unroll
+        for (int e = 0; e < NCD; ++e) jk[j][e] += (double)(dab * x[j][e]);
+        // K: A(a, c) -= D(b, d) x / 2, A(a, d) -= D(b, c) x / 2 (into ka),
+        //    A(b, c) -= D(a, d) x / 2, A(b, d) -= D(a, c) x / 2 (LDS)
+        TV dB[NK], kb[NK];
+#pragma unroll
+        for (int c = 0; c < nc; ++c) dB[c] = s_d[rowB * n + aoC[j] + c];
+#pragma unroll
+        for (int d = 0; d < nd; ++d) dB[nc + d] = s_d[rowB * n + aoD[j] + d];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) kb[k] = 0;
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+            for (int d = 0; d < nd; ++d) {
+                const TV xv = x[j][c * nd + d];
+                ka[j][c] += dB[nc + d] * xv;
+                ka[j][nc + d] += dB[c] * xv;
+                kb[c] += dA[j][nc + d] * xv;
+                kb[nc + d] += dA[j][c] * xv;
             }
-        }
-        // J, bra side: A(a, b) += 2 sum_cd D(c, d) x
-        TV t = 0;
+        if ((row.act >> j) & 1u) {
 #pragma unroll
-        for (int j = 0; j < KPL; ++j)
+            for (int c = 0; c < nc; ++c) atomicAdd(&s_acc[rowB * n + aoC[j] + c], -0.5 * (double)kb[c]);
 #pragma unroll
-            for (int e = 0; e < NCD; ++e) t += dq[j][e] * x[j][e];
-        t = wave_total(t);
-        if (lane == 0 && a.jscale != 0.0) atomicAdd(&s_acc[ia * n + aoB + ib], 2.0 * a.jscale * (double)t);
-        const TV dab = s_d[ia * n + aoB + ib];   // D(a, b), wave-uniform
-#pragma unroll
-        for (int j = 0; j < KPL; ++j) {
-            // J, ket side: A(c, d) += 2 D(a, b) x
-#pragma unroll
-            for (int e = 0; e < NCD; ++e) jk[j][e] += (double)(dab * x[j][e]);
-            // K: A(a, c) -= D(b, d) x / 2, A(a, d) -= D(b, c) x / 2 (into ka),
-            //    A(b, c) -= D(a, d) x / 2, A(b, d) -= D(a, c) x / 2 (LDS)
-            TV dB[NK], kb[NK];
-#pragma unroll
-            for (int c = 0; c < nc; ++c) dB[c] = s_d[(sb + ib) * n + aoC[j] + c];
-#pragma unroll
-            for (int d = 0; d < nd; ++d) dB[nc + d] = s_d[(sb + ib) * n + aoD[j] + d];
-#pragma unroll
-            for (int k = 0; k < NK; ++k) kb[k] = 0;
-#pragma unroll
-            for (int c = 0; c < nc; ++c)
-#pragma unroll
-                for (int d = 0; d < nd; ++d) {
-                    const TV xv = x[j][c * nd + d];
-                    ka[j][c] += dB[nc + d] * xv;
-                    ka[j][nc + d] += dB[c] * xv;
-                    kb[c] += dA[j][nc + d] * xv;
-                    kb[nc + d] += dA[j][c] * xv;
-                }
-            if (act[j]) {
-#pragma unroll
-                for (int c = 0; c < nc; ++c) atomicAdd(&s_acc[(sb + ib) * n + aoC[j] + c], -0.5 * (double)kb[c]);
-#pragma unroll
-                for (int d = 0; d < nd; ++d) atomicAdd(&s_acc[(sb + ib) * n + aoD[j] + d], -0.5 * (double)kb[nc + d]);
-            }
+            for (int d = 0; d < nd; ++d) atomicAdd(&s_acc[rowB * n + aoD[j] + d], -0.5 * (double)kb[nc + d]);
         }
     }
+}
+
+// Half a step of the walk: find the row after `cur` into `nxt` and issue
+// its loads into `vn`, then do `cur`'s arithmetic on `vc`.  The caller
+// alternates two row records and two register buffers, so the loads of one
+// row are in flight during the arithmetic of the one before.
+template <int CK, typename TV, int KPL = dig_kpl(CK), int NCD = dig_ncd(CK),
+          int NK = ncart(PL<CK>::a) + ncart(PL<CK>::b)>
+__device__ __forceinline__ bool dig_half(const DigArgs<TV> &a, double *s_acc, const TV *s_d, int ia,
+                                         const DigRow<KPL> &cur, DigRow<KPL> &nxt, const TV (&vc)[KPL][NCD],
+                                         TV (&vn)[KPL][NCD], int u1, int r0, int nk, double dmax, bool screen,
+                                         const float *qk, const int (&rk)[KPL], const float (&qv)[KPL],
+                                         const bool (&valid)[KPL], const int (&aoC)[KPL], const int (&aoD)[KPL],
+                                         const TV (&dq)[KPL][NCD], const TV (&dA)[KPL][NK], double (&jk)[KPL][NCD],
+                                         TV (&ka)[KPL][NK]) {
+    const bool more = dig_next<CK>(a, cur, u1, r0, nk, ia, dmax, screen, qk, rk, qv, valid, nxt);
+    dig_load_sel(a.x, more, nxt, cur, vn);
+    digest_row<CK>(a, s_acc, s_d, ia, cur, vc, aoC, aoD, dq, dA, jk, ka);
+    return more;
 }
 
 // One column chunk: the lanes' kets; then for each bra row a of the row
@@ -159,6 +264,7 @@ __device__ __forceinline__ void digest_column(const DigArgs<TV> &a, double *s_ac
     const float *qk = CK == 0 ? a.q0 : CK == 1 ? a.q1 : a.q2;
     int rk[KPL], aoC[KPL], aoD[KPL];
     bool valid[KPL];
+    float qv[KPL];
     TV dq[KPL][NCD];
     double jk[KPL][NCD];
 #pragma unroll
@@ -169,6 +275,7 @@ __device__ __forceinline__ void digest_column(const DigArgs<TV> &a, double *s_ac
         const int m = meta[rk[j]];
         aoC[j] = m & 0x7fff;
         aoD[j] = (m >> 15) & 0x7fff;
+        qv[j] = screen ? qk[rk[j]] : 0.0f;
 #pragma unroll
         for (int e = 0; e < NCD; ++e) {
             dq[j][e] = valid[j] ? dkc[(size_t)rk[j] * NCD + e] : (TV)0;
@@ -187,33 +294,19 @@ __device__ __forceinline__ void digest_column(const DigArgs<TV> &a, double *s_ac
 #pragma unroll
             for (int k = 0; k < NK; ++k) ka[j][k] = 0;
         }
+        DigRow<KPL> r0w, r1w;
+        if (dig_find<CK>(a, u0, u1, r0, nk, ia, dmax, screen, qk, rk, qv, valid, r0w)) {
+            TV v0[KPL][NCD], v1[KPL][NCD];
+            dig_load(a.x, r0w, v0);
 #pragma unroll 1
-        for (int u = u0; u < u1; ++u) {
-            const cptr<int> ur = as_const(a.unit) + (size_t)kUnitInts * u;
-            const int lo = ur[3 + 2 * CK], hi = ur[4 + 2 * CK];
-            if (hi <= r0 || lo >= r0 + nk) continue;   // (wave-uniform)
-            double thr = 0.0;
-            if (screen) {
-                // |x| <= q_unit q_ket, kets in descending order: the unit drops
-                // out whole when its first ket in the chunk is below the
-                // screen, runs whole when its last one is above, else tests
-                // ket by ket
-                const double qu = (double)__int_as_float(ur[9]) * dmax;
-                const int rf = max(lo, r0), rl = min(hi, r0 + nk) - 1;
-                if ((double)qk[rf] * qu < a.qtau) continue;
-                if ((double)qk[rl] * qu < a.qtau) thr = a.qtau / qu;
+            for (;;) {
+                if (!dig_half<CK>(a, s_acc, s_d, ia, r0w, r1w, v0, v1, u1, r0, nk, dmax, screen, qk, rk, qv, valid,
+                                  aoC, aoD, dq, dA, jk, ka))
+                    break;
+                if (!dig_half<CK>(a, s_acc, s_d, ia, r1w, r0w, v1, v0, u1, r0, nk, dmax, screen, qk, rk, qv, valid,
+                                  aoC, aoD, dq, dA, jk, ka))
+                    break;
             }
-            bool act[KPL];
-            int ri[KPL];
-#pragma unroll
-            for (int j = 0; j < KPL; ++j) {
-                act[j] = valid[j] && rk[j] >= lo && rk[j] < hi && (thr <= 0.0 || (double)qk[rk[j]] >= thr);
-                ri[j] = act[j] ? rk[j] : lo;
-            }
-            const int aoB = ur[1] & 0x7fff, LB = ur[1] >> 15, sb = ur[2];
-            const TV *xu = a.x + as_const(a.unit_off)[3 * (size_t)u + CK];
-            if (LB == 0) digest_row<0, CK>(a, s_acc, s_d, ia, ri, act, aoC, aoD, dq, dA, jk, ka, xu, hi - lo, aoB, sb);
-            else digest_row<1, CK>(a, s_acc, s_d, ia, ri, act, aoC, aoD, dq, dA, jk, ka, xu, hi - lo, aoB, sb);
         }
 #pragma unroll
         for (int j = 0; j < KPL; ++j) {
