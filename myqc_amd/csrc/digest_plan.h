@@ -31,6 +31,13 @@
 // workgroups that hold it, and per ket, the partial slots of the chunks that
 // cover it.  No global atomics; the summation order is fixed up to the LDS
 // atomics inside a workgroup.
+//
+// Balance: a wavefront's time goes mostly to its rows (one load round trip
+// and one set of LDS updates each, whatever the number of active lanes), so
+// a chunk costs about the loads per lane of its rows.  A workgroup's chunks
+// are listed by descending cost and handed to its wavefronts as they become
+// free (longest first); the workgroups are listed by descending cost, so
+// that the long ones start first and the short ones fill the tail.
 #pragma once
 
 #include <algorithm>
@@ -185,7 +192,8 @@ inline std::string build_digest_schedule(DigestPlan &pl, const std::vector<int> 
     std::vector<std::vector<int>> jk_slots(pl.jk_block0[3]);
     // work vectors, reused from run to run
     std::vector<int> rowaos, chunks;   // chunks: (ck, r0, nk) triples
-    std::vector<long long> cost, bc, diff;
+    std::vector<long long> cost, bc, diff, walk, wg_walk;
+    std::vector<size_t> order;
     // workgroups
     for (int A = 0; A < ns; ++A) {
         const auto &units = of_a[A];
@@ -278,15 +286,32 @@ inline std::string build_digest_schedule(DigestPlan &pl, const std::vector<int> 
                     cost.push_back(c);
                 }
             }
+            // each chunk's walk cost: loads per lane over the rows of every
+            // unit that meets it
+            walk.assign(cost.size(), 0);
+            for (size_t k = 0; k < cost.size(); ++k) {
+                const int ck = chunks[3 * k], r0 = chunks[3 * k + 1], nk = chunks[3 * k + 2];
+                for (int q = ubase; q < uend; ++q) {
+                    const int *ur = &pl.unit[(size_t)q * kUnitInts];
+                    if (ur[4 + 2 * ck] <= r0 || ur[3 + 2 * ck] >= r0 + nk) continue;
+                    walk[k] += (long long)nA * ((ur[1] >> 15) ? 3 : 1) * dig_ncd(ck) * dig_kpl(ck);
+                }
+            }
             // split the chunks into workgroups of about `target` values
             size_t c0 = 0;
             const size_t nch = cost.size();
             while (c0 < nch) {
                 size_t c1 = c0;
-                long long acc = 0;
+                long long acc = 0, wsum = 0;
                 while (c1 < nch && (c1 == c0 || acc + cost[c1] <= target)) acc += cost[c1++];
-                const int chunk0 = (int)(pl.chunk.size() / kChunkInts);
+                order.clear();
                 for (size_t k = c0; k < c1; ++k) {
+                    order.push_back(k);
+                    wsum += walk[k];
+                }
+                std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return walk[x] > walk[y]; });
+                const int chunk0 = (int)(pl.chunk.size() / kChunkInts);
+                for (const size_t k : order) {
                     const int ck = chunks[3 * k], r0 = chunks[3 * k + 1], nk = chunks[3 * k + 2];
                     const int slot = pl.n_jk_slots++;
                     pl.chunk.push_back(ck);
@@ -298,12 +323,26 @@ inline std::string build_digest_schedule(DigestPlan &pl, const std::vector<int> 
                 const int rec[kWgInts] = {shells_ao[A] | (shells_L[A] << 15), ubase, uend, chunk0,
                                           (int)(pl.chunk.size() / kChunkInts), (int)rowaos.size(), pl.n_rows};
                 pl.wg.insert(pl.wg.end(), rec, rec + kWgInts);
+                wg_walk.push_back(wsum);
                 pl.row_ao.insert(pl.row_ao.end(), rowaos.begin(), rowaos.end());
                 pl.n_rows += (int)rowaos.size();
                 c0 = c1;
             }
             u = u1;
         }
+    }
+    // workgroups by descending walk cost (each record carries its own
+    // units, chunks and partial rows)
+    {
+        const size_t nwg = wg_walk.size();
+        std::vector<size_t> perm(nwg);
+        for (size_t w = 0; w < nwg; ++w) perm[w] = w;
+        std::stable_sort(perm.begin(), perm.end(), [&](size_t x, size_t y) { return wg_walk[x] > wg_walk[y]; });
+        std::vector<int> wg(pl.wg.size());
+        for (size_t w = 0; w < nwg; ++w)
+            std::copy(pl.wg.begin() + perm[w] * kWgInts, pl.wg.begin() + (perm[w] + 1) * kWgInts,
+                      wg.begin() + w * kWgInts);
+        pl.wg.swap(wg);
     }
     // per AO, the partial rows that hold it
     std::vector<std::vector<int>> rows_of(n);

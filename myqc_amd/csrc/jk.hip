@@ -116,12 +116,14 @@ __device__ __forceinline__ bool dig_find(const DigArgs<TV> &a, int u, int u1, in
             row.act |= on ? 1u << j : 0u;
             row.ri[j] = on ? rk[j] : lo;
         }
-        row.u = u;
-        row.len = hi - lo;
-        row.nb = (ur[1] >> 15) ? 3 : 1;
+        // (wave-uniform values, made visibly so: the walk's branches on
+        // them then stay scalar)
+        row.u = __builtin_amdgcn_readfirstlane(u);
+        row.len = __builtin_amdgcn_readfirstlane(hi - lo);
+        row.nb = __builtin_amdgcn_readfirstlane((ur[1] >> 15) ? 3 : 1);
         row.ib = 0;
-        row.aoB = ur[1] & 0x7fff;
-        row.sb = ur[2];
+        row.aoB = __builtin_amdgcn_readfirstlane(ur[1] & 0x7fff);
+        row.sb = __builtin_amdgcn_readfirstlane(ur[2]);
         row.off = as_const(a.unit_off)[3 * (size_t)u + CK] + (long long)ia * row.nb * NCD * row.len;
         return true;
     }
@@ -133,9 +135,10 @@ template <int CK, typename TV, int KPL>
 __device__ __forceinline__ bool dig_next(const DigArgs<TV> &a, const DigRow<KPL> &cur, int u1, int r0, int nk, int ia,
                                          double dmax, bool screen, const float *qk, const int (&rk)[KPL],
                                          const float (&qv)[KPL], const bool (&valid)[KPL], DigRow<KPL> &nxt) {
-    if (cur.ib + 1 < cur.nb) {
+    const int ib = __builtin_amdgcn_readfirstlane(cur.ib), nb = __builtin_amdgcn_readfirstlane(cur.nb);
+    if (ib + 1 < nb) {
         nxt = cur;
-        nxt.ib = cur.ib + 1;
+        nxt.ib = ib + 1;
         nxt.off = cur.off + (long long)dig_ncd(CK) * cur.len;
         return true;
     }
@@ -242,7 +245,8 @@ __device__ __forceinline__ bool dig_half(const DigArgs<TV> &a, double *s_acc, co
                                          const bool (&valid)[KPL], const int (&aoC)[KPL], const int (&aoD)[KPL],
                                          const TV (&dq)[KPL][NCD], const TV (&dA)[KPL][NK], double (&jk)[KPL][NCD],
                                          TV (&ka)[KPL][NK]) {
-    const bool more = dig_next<CK>(a, cur, u1, r0, nk, ia, dmax, screen, qk, rk, qv, valid, nxt);
+    const bool more =
+        __builtin_amdgcn_readfirstlane((int)dig_next<CK>(a, cur, u1, r0, nk, ia, dmax, screen, qk, rk, qv, valid, nxt));
     dig_load_sel(a.x, more, nxt, cur, vn);
     digest_row<CK>(a, s_acc, s_d, ia, cur, vc, aoC, aoD, dq, dA, jk, ka);
     return more;
@@ -334,18 +338,21 @@ __device__ __forceinline__ void digest_column_la(const DigArgs<TV> &a, double *s
 }
 
 // One workgroup: stage its D rows, run its column chunks (wavefront w takes
-// chunks w, w + 4, ...), write its rows of A to its partial rows.  Three
+// chunk w first, then whichever chunk is next when it is free), write its
+// rows of A to its partial rows.  Three
 // workgroups per CU for the FP64 form (LDS: 16 B per row element); the FP32
 // form's rows take 12 B, so it runs four (at 128 VGPRs, a few spilled: 5-7 %
 // faster per pass than three).
 template <typename TV>
 __global__ __launch_bounds__(256, sizeof(TV) == 4 ? 4 : 3) void k_digest(DigArgs<TV> a) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];   // after s_next: doubles need 8-byte alignment
     const int n = a.n;
     const cptr<int> wr = as_const(a.wg) + (size_t)kWgInts * blockIdx.x;
     const int LA = wr[0] >> 15, u0 = wr[1], u1 = wr[2], c0 = wr[3], c1 = wr[4], nrows = wr[5], row0 = wr[6];
+    __shared__ int s_next;   // the next chunk not yet taken
     double *s_acc = reinterpret_cast<double *>(smem);
     TV *s_d = reinterpret_cast<TV *>(s_acc + (size_t)nrows * n);
+    if (threadIdx.x == 0) s_next = c0 + 4;
 #pragma unroll 1
     for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) {
         s_acc[i] = 0.0;
@@ -356,14 +363,19 @@ __global__ __launch_bounds__(256, sizeof(TV) == 4 ? 4 : 3) void k_digest(DigArgs
     const bool screen = a.qtau > 0.0;
     const double dmax = screen ? (double)__uint_as_float(*a.dmax) : 0.0;
     // the wavefront's index, made visibly uniform so that its chunk records
-    // are scalar loads
+    // are scalar loads; wavefront w starts with chunk c0 + w, then takes the
+    // next chunk not yet taken (the chunks are listed longest first)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
 #pragma unroll 1
-    for (int c = c0 + wave; c < c1; c += 4) {
+    for (int c = c0 + wave; c < c1;) {
         const cptr<int> cr = as_const(reinterpret_cast<const int *>(a.chunk)) + 4 * (size_t)c;
         const int4 ch = make_int4(cr[0], cr[1], cr[2], cr[3]);
         if (LA == 0) digest_column_la<0>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
         else digest_column_la<1>(a, s_acc, s_d, ch, u0, u1, dmax, screen);
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&s_next, 1);
+        c = __builtin_amdgcn_readfirstlane(t);
     }
     __syncthreads();
     double *dst = a.partial + (size_t)row0 * n;
