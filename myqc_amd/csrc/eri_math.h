@@ -70,12 +70,29 @@ MYQC_HD constexpr int raise(int idx, int d) {
 // the direction used to build a component by recursion: its first nonzero axis
 MYQC_HD constexpr int bdir(int idx) { return cexp(idx, 0) ? 0 : cexp(idx, 1) ? 1 : 2; }
 
-// 1/sqrt(x): the device's reciprocal square root, or its host equivalent
+// 1/sqrt(x) for x > 0 in the normal range: the device's reciprocal square
+// root estimate y refined by one second-order step, y + y e (1/2 + 3/8 e)
+// with e = 1 - x y^2 (the series of (1 - e)^(-1/2)), or the host's
+// 1/sqrt.  No handling of 0, infinity or denormals: every caller passes a
+// sum of Gaussian exponents or a Boys argument above 12.
 MYQC_HD double rsqrt_hd(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return rsqrt(x);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 #else
     return 1.0 / sqrt(x);
+#endif
+}
+
+// The predicate on some active lane of the wavefront (device; the value is
+// wave-uniform, so a branch on it jumps over code no lane needs instead of
+// running it under a mask), or the predicate itself (host).
+MYQC_HD bool any_lane(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(p) != 0;
+#else
+    return p;
 #endif
 }
 
@@ -134,14 +151,11 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
     }
     if (T < 12.0) {
         const double *r = ftT + Tk * FT_NM + Q;
-        // sum_{k=0..6} Ft(Tk,Q+k) x^k / k!   (auxilary.f90:152-155)
-        double xk = 1.0, fq = 0.0;
-        const double inv_fact[7] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0};
+        // sum_{k=0..6} Ft(Tk,Q+k) x^k / k!   (auxilary.f90:152-155), in
+        // Horner form r0 + x (r1 + x/2 (r2 + x/3 (r3 + ...)))
+        double fq = r[6];
 #pragma unroll
-        for (int k = 0; k <= 6; ++k) {
-            fq += r[k] * xk * inv_fact[k];
-            xk *= x;
-        }
+        for (int k = 5; k >= 0; --k) fq = fma(fq, x * (1.0 / (k + 1)), r[k]);
         // with Q = 0 (sets of s functions only) there is no recursion: F_0 is
         // the Taylor sum itself
         const int top = Q < M ? Q : M;
@@ -152,14 +166,18 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
         // Q = 3 (sum of four set L's) <= 12.  Q is a multiple of 3, so the
         // steps j = Q-1 .. M run in blocks of three (j = 3b+2, 3b+1, 3b for
         // Q > 3b), each behind one branch that a wavefront skips when none of
-        // its lanes needs it
+        // its lanes needs it (any_lane)
 #pragma unroll
         for (int b = 3; b >= 0; --b) {
             if (3 * b + 2 < M) continue;
-            if (Q > 3 * b) {
+            if (any_lane(Q > 3 * b)) {
+                const bool on = Q > 3 * b;
 #pragma unroll
                 for (int j = 3 * b + 2; j >= 3 * b; --j)
-                    if (j >= M) fq = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
+                    if (j >= M) {
+                        const double v = (t2 * fq + e) * (1.0 / (2.0 * j + 1.0));
+                        fq = on ? v : fq;
+                    }
             }
         }
         // fq now holds F_top, top = min(Q, M); the reference never forms F_j for
