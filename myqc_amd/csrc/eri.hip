@@ -169,8 +169,12 @@ __device__ __forceinline__ void load_ft(const double *__restrict__ ftT, double *
 // MODE 0: store the block's copies for digestion;  MODE 1: scatter into dense XX (8-fold).
 // SPLIT lanes per shell quartet: they share its primitive quartets, add their
 // sums (group_sum), and each stores every SPLIT-th element of the block.
+// Waves per SIMD asked of the register allocator per class (the staged
+// table's 23 KB of LDS allows seven 256-thread workgroups per CU): ss|ss 6,
+// ps|ss 5, ps|ps and pp|ss 4, the rest unconstrained.
+constexpr int eri_waves(int cb, int ck) { return cb + ck == 0 ? 6 : cb + ck == 1 ? 5 : cb + ck == 2 ? 4 : 1; }
 template <int CB, int CK, int MODE, int SPLIT>
-__global__ __launch_bounds__(256) void k_eri(EriArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eri_waves(CB, CK)))) void k_eri(EriArgs a) {
     __shared__ double s_ft[FT_TOTAL];
     load_ft(a.ftT, s_ft);
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;

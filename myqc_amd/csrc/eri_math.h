@@ -33,7 +33,7 @@ namespace myqc {
 constexpr double PI_SP = (double)3.1415926535897931f;
 
 constexpr int FT_NT = 121;   // Tk = 0..120  (T = 0 .. 12 step 0.1)
-constexpr int FT_NM = 23;    // m  = 0..22
+constexpr int FT_NM = 19;    // m  = 0..18 of the record's 23: the largest order read is Q + 6 = 18 (Q <= 12)
 // The staged table also holds e^{-k/10}, k = 0..600, after the Boys values
 // (FT_E0 on): e^{-T} for T < 60 (the largest T at which it is used, 2Q + 36
 // with Q <= 12) from one entry and a short series
