@@ -479,7 +479,7 @@ static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s, EriSha
 // count.  They include lanes that sit idle while a neighbour computes a
 // primitive quartet that its own pair screen drops, and both sides of a
 // divergent Boys branch, so they count issued rather than strictly useful work.
-static const double kQuartetFlops[6] = {4836.7, 4930.0, 6708.2, 6929.1, 11986.4, 33463.8};
+static const double kQuartetFlops[6] = {3837.7, 4240.0, 6188.7, 6405.1, 11501.3, 34757.2};
 
 static int build_classes(myqc_ctx *ctx, double tol) {
     std::vector<long long> cum;

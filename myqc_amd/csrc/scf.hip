@@ -55,8 +55,12 @@ __global__ void k_axpbyc(const double *__restrict__ X, const double *__restrict_
     Z[e] = v;
 }
 
+// DIIS history slots (the extrapolation's coefficient arrays and the host
+// solver are sized by it)
+constexpr int kDiisSlots = 8;
+
 struct LinComb {
-    double c[8];
+    double c[kDiisSlots];
     int m;
 };
 
@@ -134,7 +138,7 @@ struct ScfBuffers {
            *part = nullptr, *red = nullptr, *sp2t = nullptr;
     double *hF = nullptr, *hE = nullptr;   // DIIS history, kDiis x n^2 each
     int *info = nullptr;
-    static constexpr int kDiis = 8;
+    static constexpr int kDiis = kDiisSlots;
     void release() {
         double **ps[] = {&H,  &S,  &X,   &Xi,  &F,   &Fx,   &D,    &Dn,   &Do,   &Don, &G, &C,
                          &T1, &T2, &P2, &eig, &work, &dsc, &lohi, &part, &red, &hF,  &hE, &sp2t};
@@ -337,7 +341,7 @@ static int density(myqc_ctx *ctx, const double *C, int nocc, double *D, int n, d
     return gemm(ctx, NN, TT, n, n, nocc, occ, C, n, C, n, 0.0, D, n);
 }
 
-// solve the DIIS system on the host (m <= 8), Gaussian elimination with pivoting
+// solve the DIIS system on the host (m <= kDiisSlots), Gaussian elimination with pivoting
 static bool diis_coeffs(const std::vector<double> &B, int m, std::vector<double> &c) {
     const int N = m + 1;
     std::vector<double> M((size_t)N * (N + 1), 0.0);
