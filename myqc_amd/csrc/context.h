@@ -105,14 +105,14 @@ struct myqc_ctx {
     int *d_jk_list = nullptr, *d_jk_list_off = nullptr;     // per rank block, its chunk slots (CSR)
     int *d_jk_group = nullptr;                              // the ket-side gather's groups
     int *d_jkoff = nullptr;               // [X*ns + Y]: J-ket block of pair (X,Y) if stored so, else -1
-    double *d_jk_part = nullptr;          // ket-side J partial slots (kJkSlot doubles each)
+    double *d_jk_part = nullptr;          // ket-side J partial slots (kJkSlot values each, FP64 or FP32)
     double *d_jk = nullptr;               // ket-side J by class and rank
     int *d_meta[3] = {nullptr, nullptr, nullptr};   // per class, by Schwarz rank: aoC | aoD << 15
     float *d_qrank[3] = {nullptr, nullptr, nullptr}; // per class, the Schwarz factor of each rank (descending)
     double *d_dmax = nullptr;             // max |dD| of the current incremental pass (float bits in the first word)
     int *d_shell = nullptr;               // [s] = {ao, L, 0, 0}
     int *d_ao_shell = nullptr;            // shell of each AO
-    double *d_partial = nullptr;          // partial rows x n (both precisions accumulate in FP64)
+    double *d_partial = nullptr;          // partial rows x n (FP64, or FP32 in the incremental passes)
     double *d_Dpair = nullptr;            // density blocks of the kets, by class and rank (see jk.hip)
     float *d_Dpair32 = nullptr;
 

@@ -46,8 +46,8 @@ struct DigArgs {
     const int *row_ao;                        // AO of each partial row
     const TV *x;                              // stored (pre-weighted) values
     const TV *D;                              // symmetric density (or change), n x n
-    double *partial;                          // [row][n]
-    double *jk_part;                          // [slot][kJkSlot]
+    TV *partial;                              // [row][n] (FP32 for the incremental passes)
+    TV *jk_part;                              // [slot][kJkSlot]
     int n;
     double jscale;                            // 1, or 0 for the exchange part alone
     // density-weighted screen of the incremental (FP32) passes: a unit's ket
@@ -321,12 +321,12 @@ __device__ __forceinline__ void digest_column(const DigArgs<TV> &a, double *s_ac
             for (int d = 0; d < nd; ++d) atomicAdd(&s_acc[ia * n + aoD[j] + d], -0.5 * (double)ka[j][nc + d]);
         }
     }
-    double *dst = a.jk_part + (size_t)ch.w * kJkSlot;
+    TV *dst = a.jk_part + (size_t)ch.w * kJkSlot;
     const double js = 2.0 * a.jscale;
 #pragma unroll
     for (int j = 0; j < KPL; ++j)
 #pragma unroll
-        for (int e = 0; e < NCD; ++e) dst[(lane + 64 * j) * NCD + e] = js * jk[j][e];
+        for (int e = 0; e < NCD; ++e) dst[(lane + 64 * j) * NCD + e] = (TV)(js * jk[j][e]);
 }
 
 template <int LA, typename TV>
@@ -378,9 +378,9 @@ __global__ __launch_bounds__(256, sizeof(TV) == 4 ? 4 : 3) void k_digest(DigArgs
         c = __builtin_amdgcn_readfirstlane(t);
     }
     __syncthreads();
-    double *dst = a.partial + (size_t)row0 * n;
+    TV *dst = a.partial + (size_t)row0 * n;
 #pragma unroll 1
-    for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) dst[i] = s_acc[i];
+    for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) dst[i] = (TV)s_acc[i];
 }
 
 // The ket-side J of every pair: jk[base(c) + r ncd(c) + e] = sum of the
@@ -388,7 +388,8 @@ __global__ __launch_bounds__(256, sizeof(TV) == 4 ? 4 : 3) void k_digest(DigArgs
 // per group of at most kJkGroup slots of one block of W(c) ranks (positions
 // of the block across the threads, coalesced; four independent sums), added
 // into jk (zeroed before the pass) with one atomic per position.
-__global__ __launch_bounds__(256) void k_gather_jk(const double *__restrict__ jk_part, const int *__restrict__ list,
+template <typename PT>
+__global__ __launch_bounds__(256) void k_gather_jk(const PT *__restrict__ jk_part, const int *__restrict__ list,
                                                    const int *__restrict__ group, int3 base, int3 block0, int4 nps,
                                                    double *__restrict__ jk) {
     const int g = blockIdx.x;
@@ -414,7 +415,8 @@ __global__ __launch_bounds__(256) void k_gather_jk(const double *__restrict__ jk
 // the pair (shell(i), shell(j)) when it is stored in that orientation.  One
 // workgroup per (AO, 64 columns); its four thread rows take every fourth
 // partial row (two independent sums each) and meet in LDS in a fixed order.
-__global__ __launch_bounds__(256) void k_gather_rows(const double *__restrict__ partial,
+template <typename PT>
+__global__ __launch_bounds__(256) void k_gather_rows(const PT *__restrict__ partial,
                                                      const int *__restrict__ list_off, const int *__restrict__ list,
                                                      const double *__restrict__ jk, const int *__restrict__ jkoff,
                                                      const int *__restrict__ ao_shell, const int *__restrict__ shell,
@@ -574,8 +576,12 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
         a.row_ao = ctx->d_row_ao;
         a.x = x;
         a.D = D;
-        a.partial = partial;
-        a.jk_part = ctx->d_jk_part;
+        // the partial rows and ket-side slots in the pass's own precision
+        // (sums of FP32 products, accumulated in FP64, rounded once to FP32
+        // in the incremental passes), summed in FP64 by the gathers; the
+        // FP64-sized buffers hold either
+        a.partial = reinterpret_cast<TV *>(partial);
+        a.jk_part = reinterpret_cast<TV *>(ctx->d_jk_part);
         a.n = n;
         a.jscale = jscale;
         a.q0 = ctx->d_qrank[0];
@@ -592,14 +598,14 @@ static int launch_digest(myqc_ctx *ctx, const TV *D, const TV *x, TV *dk, double
         MYQC_CHECK(timer_end(ctx, T_JK, ev, (double)sizeof(TV) * (double)ctx->n_stored + 16.0 * n * n));
         const int ngroups = (int)(pl.jk_group.size() / 3);
         if (ngroups > 0)
-            k_gather_jk<<<ngroups, 256, 0, ctx->stream>>>(
-                ctx->d_jk_part, ctx->d_jk_list, ctx->d_jk_group, make_int3(pl.jk_base[0], pl.jk_base[1], pl.jk_base[2]),
+            k_gather_jk<TV><<<ngroups, 256, 0, ctx->stream>>>(
+                reinterpret_cast<const TV *>(ctx->d_jk_part), ctx->d_jk_list, ctx->d_jk_group, make_int3(pl.jk_base[0], pl.jk_base[1], pl.jk_base[2]),
                 make_int3(pl.jk_block0[0], pl.jk_block0[1], pl.jk_block0[2]),
                 make_int4((int)ctx->pair_ids[0].size(), (int)ctx->pair_ids[1].size(), (int)ctx->pair_ids[2].size(), 0),
                 ctx->d_jk);
     }
-    k_gather_rows<<<dim3((n + 63) / 64, n), dim3(64, 4), 0, ctx->stream>>>(
-        partial, ctx->d_row_list_off, ctx->d_row_list, ctx->d_jk, ctx->d_jkoff, ctx->d_ao_shell, ctx->d_shell,
+    k_gather_rows<TV><<<dim3((n + 63) / 64, n), dim3(64, 4), 0, ctx->stream>>>(
+        reinterpret_cast<const TV *>(partial), ctx->d_row_list_off, ctx->d_row_list, ctx->d_jk, ctx->d_jkoff, ctx->d_ao_shell, ctx->d_shell,
         ctx->ns, n, Graw);
     MYQC_HIP(ctx, hipGetLastError());
     return 0;
