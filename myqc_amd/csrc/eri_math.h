@@ -39,7 +39,10 @@ constexpr int FT_NM = 19;    // m  = 0..18 of the record's 23: the largest order
 // with Q <= 12) from one entry and a short series
 constexpr int FT_NE = 601;
 constexpr int FT_E0 = FT_NT * FT_NM;
-constexpr int FT_TOTAL = FT_E0 + FT_NE;
+// and, from FT_G0 on, the coefficients of BoysG's g(T) = c0 + c1/T + c2/T^2 +
+// c3/T^3 on its four ranges of T (12-15, 15-18, 18-24, 24 on)
+constexpr int FT_G0 = FT_E0 + FT_NE;
+constexpr int FT_TOTAL = FT_G0 + 16;
 
 // Cartesian components up to l = 2: 0 s | 1 x 2 y 3 z | 4 xx 5 xy 6 xz 7 yy 8 yz 9 zz
 MYQC_HD constexpr int cexp(int idx, int d) {
@@ -109,14 +112,21 @@ MYQC_HD double recip_hd(double x) {
 #endif
 }
 
-// The staged table ftT of (Boys table transposed, ftT[Tk*23 + m]; then
-// e^{-k/10} at FT_E0 + k), built from the Ftab record (Ft(Tk, m) at
-// Tk + 121 m).  Host only.
+// The staged table ftT of (Boys table transposed, ftT[Tk*FT_NM + m]; then
+// e^{-k/10} at FT_E0 + k; then g's coefficients at FT_G0 + 4 range + power),
+// built from the Ftab record (Ft(Tk, m) at Tk + 121 m).  Host only.
 inline std::vector<double> build_ftT(const double *Ftab) {
     std::vector<double> ftT(FT_TOTAL);
     for (int tk = 0; tk < FT_NT; ++tk)
         for (int m = 0; m < FT_NM; ++m) ftT[tk * FT_NM + m] = Ftab[tk + FT_NT * m];
     for (int k = 0; k < FT_NE; ++k) ftT[FT_E0 + k] = std::exp(-0.1 * k);
+    // BoysG (auxilary.f90:271-283); beyond T = 30 the reference's g is
+    // undefined and 0.49 is used (changes F_0 by at most 3e-15 |g|)
+    const double gc[16] = {0.4999489092, -0.2473631686, 0.321180909, -0.3811559346,
+                           0.4998436875, -0.24249438,   0.24642845,  0.0,
+                           0.499093162,  -0.2152832,    0.0,         0.0,
+                           0.490,        0.0,           0.0,         0.0};
+    for (int k = 0; k < 16; ++k) ftT[FT_G0 + k] = gc[k];
     return ftT;
 }
 
@@ -194,14 +204,12 @@ MYQC_HD void boys_ref(int Q, double T, const double *__restrict__ ftT, double (&
         const double iT = rt * rt;
         const double it2 = 0.5 * iT;
         if (T < 2.0 * Q + 36.0) {
-            // auxilary.f90:181-187 with BoysG (:271-283); beyond T = 30 the
-            // reference's g is undefined and 0.49 is used (changes F_0 by
-            // at most 3e-15 |g|)
-            double g;
-            if (T < 15.0) g = 0.4999489092 - 0.2473631686 * iT + 0.321180909 * iT * iT - 0.3811559346 * iT * iT * iT;
-            else if (T < 18.0) g = 0.4998436875 - 0.24249438 * iT + 0.24642845 * iT * iT;
-            else if (T < 24.0) g = 0.499093162 - 0.2152832 * iT;
-            else g = 0.490;
+            // auxilary.f90:181-187 with BoysG (:271-283): g's coefficients
+            // for T's range from the staged table, one polynomial for every
+            // lane (no branch per range)
+            const int rg = (T >= 15.0) + (T >= 18.0) + (T >= 24.0);
+            const double *gc = ftT + FT_G0 + 4 * rg;
+            const double g = fma(fma(fma(gc[3], iT, gc[2]), iT, gc[1]), iT, gc[0]);
             F[0] = half_sqrt_pi * rt - e * g * iT;
 #pragma unroll
             for (int j = 1; j <= M; ++j) F[j] = it2 * ((2.0 * (j - 1) + 1.0) * F[j - 1] - e);
