@@ -479,7 +479,7 @@ static void launch_class(myqc_ctx *ctx, int c, double *XX, hipStream_t s, EriSha
 // count.  They include lanes that sit idle while a neighbour computes a
 // primitive quartet that its own pair screen drops, and both sides of a
 // divergent Boys branch, so they count issued rather than strictly useful work.
-static const double kQuartetFlops[6] = {3837.7, 4240.0, 6188.7, 6405.1, 11501.3, 34757.2};
+static const double kQuartetFlops[6] = {3589.3, 4008.7, 5969.1, 6161.0, 11273.7, 34674.3};
 
 static int build_classes(myqc_ctx *ctx, double tol) {
     std::vector<long long> cum;
