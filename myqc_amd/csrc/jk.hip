@@ -7,19 +7,21 @@
 // pre-weighted, under its bra pair (digest_plan.h describes the layout, the
 // schedule and the six updates each stored element makes).  A workgroup
 // holds in LDS the rows of G of its row shell A and of its units' B shells,
-// and the same rows of D; each wavefront takes column chunks (one ket class,
-// W consecutive Schwarz ranks, KPL kets per lane) and, per chunk, every unit
-// that has kets there.  For a unit it streams the bra rows of its lanes'
-// blocks, loads issued before the arithmetic:
+// and the same rows of D; its wavefronts take its column chunks (one ket
+// class, W consecutive Schwarz ranks, KPL kets per lane), longest first, as
+// they become free, and per chunk walk the rows (unit, B function) of every
+// unit that has kets there, the next row's loads in flight during the
+// current row's arithmetic:
 //   J, bra side:  reduced across the wavefront into row a, column b;
 //   J, ket side:  summed per lane over the chunk's units, written once per
 //                 chunk to the chunk's partial slot;
 //   K:            four updates per element into rows a and b (LDS atomics).
-// FP32 values (the incremental passes) are accumulated in FP64.  A gather
-// sums, per AO row, the partial rows of the workgroups holding it, and per
-// ket, the slots of the chunks that cover it; G = A + A^T.  No global
-// atomics; the LDS atomics make the summation order, and so the last bits,
-// vary from run to run.
+// FP32 values (the incremental passes) are accumulated in FP64 and their
+// partial rows and slots written in FP32.  A gather sums, per AO row, the
+// partial rows of the workgroups holding it, and per ket, the slots of the
+// chunks that cover it, in FP64; G = A + A^T.  No global atomics in the
+// digestion; the LDS atomics make the summation order, and so the last
+// bits, vary from run to run.
 #include <algorithm>
 #include <cmath>
 #include <vector>
