@@ -784,8 +784,10 @@ struct ManyBuffers {
     double *h_red = nullptr;    // pinned
     // the Fock builds of one iteration fan out over kFan streams (the
     // contexts' own): one event to start them after the lead stream's work,
-    // one per stream to join them back
-    static constexpr int kFan = 4;
+    // one per stream to join them back.  Eight streams over the process's
+    // four hardware queues measured 0.7 % faster than four for 32 40-water
+    // clusters (181.9 against 180.6 solves/s, two runs each)
+    static constexpr int kFan = 8;
     hipEvent_t fork = nullptr, join[kFan] = {};
 };
 
